@@ -1,0 +1,443 @@
+/*
+ * hjm_oracle.c -- CPU oracle (test infrastructure only; see hjm_oracle.h).
+ *
+ * Built with -O2 -ffp-contract=off so that every a*b+c below is a rounded multiply
+ * followed by a rounded add, as the reference's sequential C++ computes it, and so
+ * that the normal generator (which uses explicit fmaf) is bit-reproducible.
+ */
+#include "hjm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ===================================================================================
+ * Normal generator specification.
+ *
+ * Step n of Monte Carlo path p draws its three factor shocks from ONE Philox4x32-10
+ * block (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+ * SC'11) with counter (n, p, 0, 0) and key (seed mod 2^32, seed >> 32):
+ *   x0..x3 = philox(ctr, key)
+ *   u  = ((x0 >> 8) + 1) * 2^-24        in (0, 1]
+ *   k  =  x1 >> 8                        angle 2*pi*k/2^24
+ *   z0 = sqrt(-2 ln u) cos(2 pi k 2^-24), z1 = sqrt(-2 ln u) sin(2 pi k 2^-24)
+ *   z2 = sqrt(-2 ln u') cos(2 pi k' 2^-24) with u', k' from x2, x3
+ * ln, sin and cos are the fixed FP32 polynomials below (coefficients fitted by
+ * cva_hjm_cuda_amd/csrc/rng_poly_fit.py), evaluated with explicit fmaf so the GPU
+ * reproduces them bit for bit. The reference draws cuRAND XORWOW normals instead
+ * (simulation_cpu.h:83-90, simulation_gpu.cu:1011-1013); that sequence is closed
+ * library behaviour and is replaced, not reproduced.
+ * =================================================================================== */
+
+#define PHILOX_M0 0xD2511F53u
+#define PHILOX_M1 0xCD9E8D57u
+#define PHILOX_W0 0x9E3779B9u
+#define PHILOX_W1 0xBB67AE85u
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += PHILOX_W0;
+            k1 += PHILOX_W1;
+        }
+        uint64_t p0 = (uint64_t)PHILOX_M0 * c0;
+        uint64_t p1 = (uint64_t)PHILOX_M1 * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+static inline float f_from_bits(uint32_t b) { float f; memcpy(&f, &b, 4); return f; }
+static inline uint32_t bits_from_f(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
+
+/* ln(u) for u in [2^-24, 1]: u = 2^e m, m in [sqrt(1/2), sqrt(2)], ln m = x + x^2 P(x). */
+float orc_log_f32(float u)
+{
+    static const float P[9] = {
+        -0x1.fffff8p-2f, 0x1.55555p-2f, -0x1.000428p-2f, 0x1.99a402p-3f, -0x1.54274ap-3f,
+        0x1.22709cp-3f, -0x1.0f375cp-3f, 0x1.084ep-3f, -0x1.383d96p-4f};
+    uint32_t b = bits_from_f(u);
+    int e = (int)((b >> 23) & 0xffu) - 127;
+    uint32_t mb = (b & 0x007fffffu) | 0x3f800000u;
+    if (mb > 0x3fb504f3u) { /* m > sqrt(2): halve it */
+        mb -= 0x00800000u;
+        e += 1;
+    }
+    float x = f_from_bits(mb) - 1.0f;
+    float p = P[8];
+    for (int i = 7; i >= 0; --i) p = fmaf(p, x, P[i]);
+    float x2 = x * x;
+    float lnm = fmaf(x2, p, x);
+    const float LN2_HI = 0x1.62e4p-1f;     /* 0.693145751953125, 17 significant bits */
+    const float LN2_LO = 0x1.7f7d1ep-20f;  /* ln2 - LN2_HI, rounded to float */
+    float fe = (float)e;
+    return fmaf(fe, LN2_HI, fmaf(fe, LN2_LO, lnm));
+}
+
+/* sin and cos of 2*pi*k/2^24 for k in [0, 2^24): integer quadrant reduction. */
+void orc_sincos_turn24_f32(uint32_t k24, float* s, float* c)
+{
+    static const float S[5] = {0x1.921fb6p+2f, -0x1.4abbcep+5f, 0x1.466bbap+6f,
+                               -0x1.32ca9ep+6f, 0x1.4bc89ap+5f};
+    static const float C[4] = {-0x1.3bd3ccp+4f, 0x1.03c1dep+6f, -0x1.55c5ep+6f,
+                               0x1.d9d57cp+5f};
+    uint32_t q = (k24 + (1u << 21)) >> 22;              /* nearest quarter turn, 0..4 */
+    int32_t ri = (int32_t)k24 - (int32_t)(q << 22);      /* in [-2^21, 2^21) */
+    float r = (float)ri * 0x1p-24f;                      /* exact, |r| <= 1/8 */
+    float r2 = r * r;
+    float ps = S[4];
+    for (int i = 3; i >= 0; --i) ps = fmaf(ps, r2, S[i]);
+    float sn = r * ps;
+    float pc = C[3];
+    for (int i = 2; i >= 0; --i) pc = fmaf(pc, r2, C[i]);
+    float cs = fmaf(r2, pc, 1.0f);
+    switch (q & 3u) {
+    case 0: *s = sn; *c = cs; break;
+    case 1: *s = cs; *c = -sn; break;
+    case 2: *s = -sn; *c = -cs; break;
+    default: *s = -cs; *c = sn; break;
+    }
+}
+
+void orc_step_normals(uint64_t seed, uint32_t path, uint32_t step, float z[3])
+{
+    uint32_t ctr[4] = {step, path, 0u, 0u};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t x[4];
+    orc_philox4x32_10(ctr, key, x);
+    float u0 = (float)((x[0] >> 8) + 1u) * 0x1p-24f;
+    float u2 = (float)((x[2] >> 8) + 1u) * 0x1p-24f;
+    float r0 = sqrtf(-2.0f * orc_log_f32(u0));
+    float r2 = sqrtf(-2.0f * orc_log_f32(u2));
+    float s0, c0, s2, c2;
+    orc_sincos_turn24_f32(x[1] >> 8, &s0, &c0);
+    orc_sincos_turn24_f32(x[3] >> 8, &s2, &c2);
+    z[0] = r0 * c0;
+    z[1] = r0 * s0;
+    z[2] = r2 * c2;
+}
+
+/* Layout [path][step][3], as the reference stores its normals (simulation_cpu.h:205-207). */
+void orc_fill_normals(uint64_t seed, int64_t path0, int npaths, int nsteps, double* out)
+{
+    for (int p = 0; p < npaths; ++p) {
+        for (int n = 0; n < nsteps; ++n) {
+            float z[3];
+            orc_step_normals(seed, (uint32_t)(path0 + p), (uint32_t)n, z);
+            double* o = out + ((int64_t)p * nsteps + n) * 3;
+            o[0] = z[0];
+            o[1] = z[1];
+            o[2] = z[2];
+        }
+    }
+}
+
+/* ===================================================================================
+ * Market data. The reference takes a drift table as input (simulation_gpu_testing.cpp
+ * :66-71); it equals the HJM no-arbitrage drift m(tau) = sum_k sigma_k(tau) *
+ * int_0^tau sigma_k(u) du of its volatility table (:47-63) with the integral taken by
+ * the cumulative Simpson rule below (checked to the table's 7 printed decimals).
+ * =================================================================================== */
+
+/* int_0^{x_j} y for j = 0..n-1 on a uniform grid of step h:
+ * even j: composite Simpson 1/3; odd j >= 3: Simpson 1/3 to j-3, Simpson 3/8 on the
+ * last three intervals; j = 1: the quadratic through points 0, 1, 2 over [x0, x1]. */
+void orc_cumulative_simpson(const double* y, int n, double h, double* out)
+{
+    out[0] = 0.0;
+    for (int j = 1; j < n; ++j) {
+        double s = 0.0;
+        if (j == 1) {
+            s = (n > 2) ? h / 12.0 * (5.0 * y[0] + 8.0 * y[1] - y[2]) : 0.5 * h * (y[0] + y[1]);
+        } else {
+            int even_end = (j % 2 == 0) ? j : j - 3;
+            for (int i = 0; i < even_end; i += 2) s += h / 3.0 * (y[i] + 4.0 * y[i + 1] + y[i + 2]);
+            if (j % 2 == 1)
+                s += 3.0 * h / 8.0 * (y[j - 3] + 3.0 * y[j - 2] + 3.0 * y[j - 1] + y[j]);
+        }
+        out[j] = s;
+    }
+}
+
+void orc_hjm_drift(const double* vols, int nfactors, int ntenors, double h, double* drift)
+{
+    double* integ = (double*)malloc(sizeof(double) * ntenors);
+    for (int j = 0; j < ntenors; ++j) drift[j] = 0.0;
+    for (int k = 0; k < nfactors; ++k) {
+        orc_cumulative_simpson(vols + k * ntenors, ntenors, h, integ);
+        for (int j = 0; j < ntenors; ++j) drift[j] += vols[k * ntenors + j] * integ[j];
+    }
+    free(integ);
+}
+
+/* Nelson-Siegel instantaneous forward curve f(tau) = b0 + b1 e^{-tau/l} + b2 (tau/l) e^{-tau/l}. */
+void orc_nelson_siegel(double b0, double b1, double b2, double lam, int ntenors, double h,
+                       double* curve)
+{
+    for (int j = 0; j < ntenors; ++j) {
+        double x = j * h / lam;
+        double e = exp(-x);
+        curve[j] = b0 + b1 * e + b2 * x * e;
+    }
+}
+
+/* The three parametric volatility factors of the benchmark configurations (factor-major). */
+void orc_poly_vols(int ntenors, double h, double* vols)
+{
+    for (int j = 0; j < ntenors; ++j) {
+        double t = j * h;
+        vols[j] = 0.0064;
+        vols[ntenors + j] = -0.0036 + 0.0006 * t - 2e-5 * t * t;
+        vols[2 * ntenors + j] = -0.0047 + 0.0018 * t - 1.5e-4 * t * t + 3e-6 * t * t * t;
+    }
+}
+
+/* ===================================================================================
+ * Reference semantics: simulation_cpu.h restated.
+ * =================================================================================== */
+
+/* simulation_cpu.h:37-54, same operation order. */
+static double musiela_sde(double drift, double vol0, double vol1, double vol2, double phi0,
+                          double phi1, double phi2, double dF, double rate0, double dtau,
+                          double dt)
+{
+    double vol_sum = vol0 * phi0;
+    vol_sum += vol1 * phi1;
+    vol_sum += vol2 * phi2;
+    vol_sum *= sqrt(dt);
+    double dfbar = drift * dt;
+    dfbar += vol_sum;
+    dfbar += (dF / dtau) * dt;
+    return rate0 + dfbar;
+}
+
+/* simulation_cpu.h:96-108: forward difference, backward at the last tenor. */
+static double dfau(int t, const double* rates)
+{
+    return (t == ORC_TENORS - 1) ? rates[t] - rates[t - 1] : rates[t + 1] - rates[t];
+}
+
+/* stride and step count exactly as the reference computes them (simulation_cpu.h:200,
+ * :384, :409), plus the checks the reference lacks: dt must divide dtau, the path must
+ * be a whole number of blocks, and the last numeraire slot must lie on the grid. */
+static int ref_grid(double expiry, double dtau, double dt, int* stride, int* nsteps, int* ndates)
+{
+    if (!(dt > 0.0) || !(dtau > 0.0) || !(expiry > 0.0)) return 2;
+    int st = (int)(dtau / dt);
+    int S = (int)(expiry / dt);
+    if (st <= 0 || fabs(st * dt - dtau) > 1e-9 * dtau) return 3;
+    if (S < st || S % st != 0) return 3;
+    int D = S / st;
+    if (D > ORC_TENORS - 1) return 3;
+    *stride = st;
+    *nsteps = S;
+    *ndates = D;
+    return 0;
+}
+
+/* One path of simulation_cpu.h:188-262 driven by the loop at :413-436. Slots that the
+ * reference never writes (0 and beyond the last date) are left at 0: the reference's
+ * malloc'd numeraire buffer is fresh, zero-filled memory at every size it was run at. */
+static void ref_path_one(const double* spot, const double* drift, const double* vols,
+                         double dtau, double dt, int stride, int nsteps, int ndates,
+                         const double* nrm, double* nx, double* ny)
+{
+    double cur[ORC_TENORS], nxt[ORC_TENORS], acc[ORC_TENORS];
+    for (int t = 0; t < ORC_TENORS; ++t) {
+        cur[t] = spot[t];
+        acc[t] = 0.0;
+        nx[t] = 0.0;
+        ny[t] = 0.0;
+    }
+    for (int path = 0; path < nsteps; path += stride) {
+        for (int s = 0; s < stride; ++s) {
+            const double* phi = nrm + (int64_t)(path + s) * 3;
+            for (int t = 0; t < ORC_TENORS; ++t) {
+                double rate;
+                if (path == 0) {
+                    rate = spot[t];
+                } else {
+                    rate = musiela_sde(drift[t], vols[t], vols[ORC_TENORS + t],
+                                       vols[2 * ORC_TENORS + t], phi[0], phi[1], phi[2],
+                                       dfau(t, cur), cur[t], dtau, dt);
+                }
+                acc[t] += rate;
+                nxt[t] = rate;
+            }
+            memcpy(cur, nxt, sizeof(cur));
+        }
+        int l = (path + stride) / stride;
+        nx[l] = nxt[l];
+        ny[l] = exp(-acc[l] * dt);
+    }
+    (void)ndates;
+}
+
+int orc_reference_paths(const double* spot, const double* drift, const double* vols,
+                        double expiry, double dtau, double dt, int npaths,
+                        const double* normals, double* num_x, double* num_y)
+{
+    int stride, S, D;
+    int rc = ref_grid(expiry, dtau, dt, &stride, &S, &D);
+    if (rc) return rc;
+    if (npaths <= 0) return 2;
+    for (int p = 0; p < npaths; ++p)
+        ref_path_one(spot, drift, vols, dtau, dt, stride, S, D, normals + (int64_t)p * S * 3,
+                     num_x + (int64_t)p * ORC_TENORS, num_y + (int64_t)p * ORC_TENORS);
+    return 0;
+}
+
+/* simulation_cpu.h:269-296 (payer swap; t = 51 out-of-bounds write omitted). */
+void orc_reference_exposure(const double* num_x, const double* num_y, const double* accrual,
+                            double notional, double K, double dtau, int npaths,
+                            double* exposures)
+{
+    double m = 1.0 / dtau;
+    for (int p = 0; p < npaths; ++p) {
+        const double* x = num_x + (int64_t)p * ORC_TENORS;
+        const double* y = num_y + (int64_t)p * ORC_TENORS;
+        double cf[ORC_TENORS];
+        for (int t = 0; t < ORC_TENORS; ++t) {
+            double libor = m * (exp(x[t] / m) - 1.0);
+            cf[t] = y[t] * notional * accrual[t] * (libor - K);
+        }
+        double* e = exposures + (int64_t)p * ORC_TENORS;
+        for (int t = 0; t < ORC_TENORS; ++t) {
+            double sum = 0.0;
+            for (int i = t + 1; i < ORC_TENORS; ++i) sum += cf[i];
+            e[t] = (sum > 0.0) ? sum : 0.0;
+        }
+    }
+}
+
+/* simulation_cpu.h:301-343: EE = (1/N) * column sums (cblas_dgemv with alpha = 1/N). */
+void orc_expected_exposure(const double* exposures, int npaths, int ndates, double* ee)
+{
+    double alpha = 1.0 / (double)npaths;
+    for (int t = 0; t < ndates; ++t) {
+        double s = 0.0;
+        for (int p = 0; p < npaths; ++p) s += exposures[(int64_t)p * ndates + t];
+        ee[t] = alpha * s;
+    }
+}
+
+/* calculateExposureCPU<double> (simulation_cpu.h:349-458) with injected normals. */
+int orc_calculate_exposure_cpu(double* expected_exposure, const double* spot,
+                               const double* drift, const double* vols,
+                               const double* accrual, double notional, double K,
+                               double expiry, double dtau, double dt, int npaths,
+                               const double* normals, double* exposures)
+{
+    size_t n = (size_t)npaths * ORC_TENORS;
+    double* nx = (double*)malloc(n * sizeof(double));
+    double* ny = (double*)malloc(n * sizeof(double));
+    double* ex = exposures ? exposures : (double*)malloc(n * sizeof(double));
+    int rc = orc_reference_paths(spot, drift, vols, expiry, dtau, dt, npaths, normals, nx, ny);
+    if (!rc) {
+        orc_reference_exposure(nx, ny, accrual, notional, K, dtau, npaths, ex);
+        orc_expected_exposure(ex, npaths, ORC_TENORS, expected_exposure);
+    }
+    free(nx);
+    free(ny);
+    if (!exposures) free(ex);
+    return rc;
+}
+
+/* ===================================================================================
+ * Statistics (no reference counterpart; doc/TODO:14 names CVA as the goal).
+ * =================================================================================== */
+
+/* CVA = (1-R) sum_i 0.5 (EE(t_{i-1}) + EE(t_i)) (S(t_{i-1}) - S(t_i)), S(t) = e^{-h t},
+ * t_i = i * date_dt: trapezoid in exposure, exact default probability per interval.
+ * Returned as weights w_i so that CVA = sum_i w_i EE_i (and per path likewise). */
+void orc_cva_weights(int ndates, double date_dt, double hazard, double recovery, double* w)
+{
+    for (int i = 0; i < ndates; ++i) w[i] = 0.0;
+    for (int i = 1; i < ndates; ++i) {
+        double pd = exp(-hazard * (i - 1) * date_dt) - exp(-hazard * i * date_dt);
+        w[i - 1] += 0.5 * (1.0 - recovery) * pd;
+        w[i] += 0.5 * (1.0 - recovery) * pd;
+    }
+}
+
+/* Inverted-CDF quantile: the k-th smallest sample, k = ceil(q n) clamped to [1, n]. */
+int64_t orc_quantile_rank(int64_t n, double q)
+{
+    double v = q * (double)n;
+    int64_t k = (int64_t)ceil(v);
+    if (k < 1) k = 1;
+    if (k > n) k = n;
+    return k;
+}
+
+static int cmp_double(const void* a, const void* b)
+{
+    double x = *(const double*)a, y = *(const double*)b;
+    return (x > y) - (x < y);
+}
+
+void orc_stats(const double* exposures, int npaths, int ndates, const double* w, double q,
+               double* ee, double* pfe, double* cva, double* cva_se)
+{
+    orc_expected_exposure(exposures, npaths, ndates, ee);
+    double* col = (double*)malloc(sizeof(double) * npaths);
+    int64_t k = orc_quantile_rank(npaths, q);
+    for (int t = 0; t < ndates; ++t) {
+        for (int p = 0; p < npaths; ++p) col[p] = exposures[(int64_t)p * ndates + t];
+        qsort(col, npaths, sizeof(double), cmp_double);
+        pfe[t] = col[k - 1];
+    }
+    free(col);
+    double c = 0.0;
+    for (int t = 0; t < ndates; ++t) c += w[t] * ee[t];
+    double s1 = 0.0, s2 = 0.0;
+    for (int p = 0; p < npaths; ++p) {
+        double v = 0.0;
+        for (int t = 0; t < ndates; ++t) v += w[t] * exposures[(int64_t)p * ndates + t];
+        s1 += v;
+        s2 += v * v;
+    }
+    *cva = c;
+    double var = (npaths > 1) ? (s2 - s1 * s1 / npaths) / (npaths - 1) : 0.0;
+    *cva_se = sqrt((var > 0.0 ? var : 0.0) / npaths);
+}
+
+/* ===================================================================================
+ * CPU baseline: the path loop of simulation_cpu.h:407-440 timed alone, on nthreads.
+ * =================================================================================== */
+double orc_time_reference_paths(const double* spot, const double* drift, const double* vols,
+                                double expiry, double dtau, double dt, int npaths,
+                                const double* normals, double* num_x, double* num_y,
+                                int nthreads)
+{
+    int stride, S, D;
+    if (ref_grid(expiry, dtau, dt, &stride, &S, &D) || npaths <= 0) return -1.0;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+#endif
+    for (int p = 0; p < npaths; ++p)
+        ref_path_one(spot, drift, vols, dtau, dt, stride, S, D, normals + (int64_t)p * S * 3,
+                     num_x + (int64_t)p * ORC_TENORS, num_y + (int64_t)p * ORC_TENORS);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    (void)nthreads;
+    return (t1.tv_sec - t0.tv_sec) * 1e3 + (t1.tv_nsec - t0.tv_nsec) * 1e-6;
+}

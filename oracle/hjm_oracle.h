@@ -1,0 +1,72 @@
+/*
+ * hjm_oracle.h -- CPU oracle for the HJM path-generation + swap-exposure hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker: it is loaded by tests/,
+ * by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, never by the product
+ * library (cva_hjm_cuda_amd/), which must run on the GPU or fail.
+ *
+ * It restates, in plain FP64 C, the reference's sequential CPU path
+ * (simulation_cpu.h:37-54 SDE, :96-108 dF, :188-262 block path generation,
+ * :269-296 exposure, :335-343 expected exposure, driver loop :349-458), with the
+ * normal variates injected by the caller instead of drawn from cuRAND. On top of the
+ * reference it adds what the reference lacks but the product computes: the normal
+ * generator specification (Philox4x32-10 + Box-Muller), the HJM drift precompute,
+ * PFE / CVA statistics, the portfolio (standard revaluation) semantics and the
+ * pathwise adjoint. Every function documents the check that pins it (see tests/).
+ */
+#ifndef HJM_ORACLE_H
+#define HJM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_TENORS 51
+#define ORC_FACTORS 3
+
+/* ---- normal generator (specification shared with the product) ---------------- */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+float orc_log_f32(float u);
+void orc_sincos_turn24_f32(uint32_t k24, float* s, float* c);
+void orc_step_normals(uint64_t seed, uint32_t path, uint32_t step, float z[3]);
+void orc_fill_normals(uint64_t seed, int64_t path0, int npaths, int nsteps, double* out);
+
+/* ---- market data ---------------------------------------------------------------- */
+void orc_cumulative_simpson(const double* y, int n, double h, double* out);
+void orc_hjm_drift(const double* vols, int nfactors, int ntenors, double h, double* drift);
+void orc_nelson_siegel(double b0, double b1, double b2, double lam, int ntenors, double h,
+                       double* curve);
+void orc_poly_vols(int ntenors, double h, double* vols);
+
+/* ---- reference semantics (simulation_cpu.h) -------------------------------------- */
+int orc_reference_paths(const double* spot, const double* drift, const double* vols,
+                        double expiry, double dtau, double dt, int npaths,
+                        const double* normals, double* num_x, double* num_y);
+void orc_reference_exposure(const double* num_x, const double* num_y, const double* accrual,
+                            double notional, double K, double dtau, int npaths,
+                            double* exposures);
+void orc_expected_exposure(const double* exposures, int npaths, int ndates, double* ee);
+int orc_calculate_exposure_cpu(double* expected_exposure, const double* spot,
+                               const double* drift, const double* vols,
+                               const double* accrual, double notional, double K,
+                               double expiry, double dtau, double dt, int npaths,
+                               const double* normals, double* exposures);
+
+/* ---- statistics ------------------------------------------------------------------ */
+void orc_cva_weights(int ndates, double date_dt, double hazard, double recovery, double* w);
+int64_t orc_quantile_rank(int64_t n, double q);
+void orc_stats(const double* exposures, int npaths, int ndates, const double* w, double q,
+               double* ee, double* pfe, double* cva, double* cva_se);
+
+/* ---- timing helper for the CPU baseline ----------------------------------------- */
+double orc_time_reference_paths(const double* spot, const double* drift, const double* vols,
+                                double expiry, double dtau, double dt, int npaths,
+                                const double* normals, double* num_x, double* num_y,
+                                int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
