@@ -1,0 +1,318 @@
+"""Python host layer over libhjm_cva.so (the C ABI declared in include/hjm_cva.h).
+
+Mirrors the reference's operator interface for the HJM exposure path:
+``calculate_exposure_multi_gpu`` has the argument meaning of
+``calculateExposureMultiGPU`` (simulation_gpu.h:10,12) and ``InterestRateSwap`` the
+fields of product.h:4-17. Everything runs on the GPU through the HIP library; there is
+no CPU fallback -- loading fails loudly if the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libhjm_cva.so")
+
+TENORS = 51
+FACTORS = 3
+STAT_SLOTS = TENORS + 2
+
+FP64 = 0
+FP32_KAHAN = 1
+RNG_PHILOX = 0
+RNG_HOST = 1
+
+STATUS = {
+    0: "ok", 1: "null pointer", 2: "invalid argument", 3: "inconsistent time grid",
+    4: "device error", 5: "allocation failed", 6: "launch failed", 7: "unsupported",
+}
+
+
+class HJMError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__("%s failed: %s (status %d)" % (what, STATUS.get(code, "?"), code))
+        self.code = code
+
+
+_dp = C.POINTER(C.c_double)
+
+
+class _Market(C.Structure):
+    _fields_ = [("spot_rates", _dp), ("drifts", _dp), ("volatilities", _dp), ("accrual", _dp)]
+
+
+class _Swap(C.Structure):
+    _fields_ = [("notional", C.c_double), ("fixed_rate", C.c_double), ("expiry", C.c_double),
+                ("dtau", C.c_double)]
+
+
+class _Config(C.Structure):
+    _fields_ = [("paths", C.c_int64), ("path_offset", C.c_int64), ("dt", C.c_double),
+                ("precision", C.c_int32), ("rng", C.c_int32), ("seed", C.c_uint64),
+                ("host_normals", _dp), ("hazard_rate", C.c_double), ("recovery", C.c_double),
+                ("pfe_quantile", C.c_double), ("device", C.c_int32), ("reserved", C.c_int32)]
+
+
+class _Results(C.Structure):
+    _fields_ = [("expected_exposure", _dp), ("pfe", _dp), ("cva", _dp), ("cva_stderr", _dp),
+                ("sums", _dp), ("num_x", _dp), ("num_y", _dp), ("exposures", _dp),
+                ("timings_ms", _dp)]
+
+
+_LIB = None
+
+
+def library() -> C.CDLL:
+    """Load libhjm_cva.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                "libhjm_cva.so not built (%s); run python -m cva_hjm_cuda_amd.build" % LIB_PATH)
+        lib = C.CDLL(LIB_PATH)
+        _declare(lib)
+        _LIB = lib
+    return _LIB
+
+
+def _declare(lib):
+    i, d, f = C.c_int, C.c_double, C.c_float
+    fp = C.POINTER(C.c_float)
+    lib.hjm_status_string.restype = C.c_char_p
+    lib.hjm_status_string.argtypes = [i]
+    lib.hjm_device_count.argtypes = [C.POINTER(i)]
+    lib.hjm_cva_weights.argtypes = [i, d, d, d, _dp]
+    lib.hjm_hjm_drift.argtypes = [_dp, d, _dp]
+    lib.hjm_plan_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(_Market), C.POINTER(_Swap),
+                                    C.POINTER(_Config)]
+    lib.hjm_plan_execute.argtypes = [C.c_void_p, C.POINTER(_Results)]
+    lib.hjm_plan_destroy.argtypes = [C.c_void_p]
+    lib.hjm_plan_destroy.restype = None
+    for name in ("hjm_simulate_reference", "hjm_simulate_reference_blockwise"):
+        getattr(lib, name).argtypes = [C.POINTER(_Market), C.POINTER(_Swap), C.POINTER(_Config),
+                                       C.POINTER(_Results)]
+    lib.hjm_calculate_exposure_multi_gpu_f64.argtypes = [_dp, d, d, d, d, _dp, _dp, _dp, _dp, d,
+                                                         i, i, d]
+    lib.hjm_calculate_exposure_multi_gpu_f32.argtypes = [fp, f, f, f, f, fp, fp, fp, fp, f, i, i,
+                                                         f]
+    lib.hjm_generate_normals.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_int32, C.c_int32,
+                                         _dp]
+
+
+def _ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def _check(code: int, what: str):
+    if code != 0:
+        raise HJMError(code, what)
+
+
+# ---- product and market (product.h:4-17) -----------------------------------------
+@dataclass
+class InterestRateSwap:
+    """The fields of the reference's InterestRateSwap<real> that its path reads."""
+    notional: float
+    K: float
+    expiry: float
+    dtau: float = 0.5
+    pricing_points: Optional[np.ndarray] = None
+    floating_schedule: Optional[np.ndarray] = None
+    fixed_schedule: Optional[np.ndarray] = None
+
+
+@dataclass
+class Market:
+    spot_rates: np.ndarray
+    drifts: np.ndarray
+    volatilities: np.ndarray
+    accrual: np.ndarray
+
+    def __post_init__(self):
+        for name, n in (("spot_rates", TENORS), ("drifts", TENORS),
+                        ("volatilities", FACTORS * TENORS), ("accrual", TENORS)):
+            a = np.ascontiguousarray(np.asarray(getattr(self, name), np.float64).reshape(-1))
+            if a.size != n:
+                raise ValueError("%s must have %d entries" % (name, n))
+            setattr(self, name, a)
+
+    def _c(self):
+        return _Market(_ptr(self.spot_rates), _ptr(self.drifts), _ptr(self.volatilities),
+                       _ptr(self.accrual))
+
+
+def hjm_drift(volatilities, dtau: float = 0.5) -> np.ndarray:
+    """HJM drift of the loadings by the cumulative Simpson rule (library routine)."""
+    v = np.ascontiguousarray(np.asarray(volatilities, np.float64).reshape(-1))
+    out = np.zeros(TENORS)
+    _check(library().hjm_hjm_drift(_ptr(v), dtau, _ptr(out)), "hjm_hjm_drift")
+    return out
+
+
+def cva_weights(hazard: float, recovery: float, ndates: int = TENORS, date_dt: float = 0.5):
+    out = np.zeros(ndates)
+    _check(library().hjm_cva_weights(ndates, date_dt, hazard, recovery, _ptr(out)),
+           "hjm_cva_weights")
+    return out
+
+
+def benchmark_market(dtau: float = 0.5, accrual: float = 0.5) -> Market:
+    """The benchmark market: Nelson-Siegel curve (b0=3%, b1=-2%, b2=1%, lambda=2y), the
+    three parametric loadings, their Simpson HJM drift, flat accrual."""
+    tau = np.arange(TENORS) * dtau
+    x = tau / 2.0
+    spot = 0.03 - 0.02 * np.exp(-x) + 0.01 * x * np.exp(-x)
+    vols = np.concatenate([np.full(TENORS, 0.0064),
+                           -0.0036 + 0.0006 * tau - 2e-5 * tau ** 2,
+                           -0.0047 + 0.0018 * tau - 1.5e-4 * tau ** 2 + 3e-6 * tau ** 3])
+    return Market(spot, hjm_drift(vols, dtau), vols, np.full(TENORS, accrual))
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _check(library().hjm_device_count(C.byref(n)), "hjm_device_count")
+    return n.value
+
+
+# ---- reference-semantics simulation ------------------------------------------------
+@dataclass
+class ReferenceResult:
+    expected_exposure: np.ndarray
+    pfe: np.ndarray
+    cva: float
+    cva_stderr: float
+    sums: np.ndarray
+    timings_ms: np.ndarray
+    num_x: Optional[np.ndarray] = None
+    num_y: Optional[np.ndarray] = None
+    exposures: Optional[np.ndarray] = None
+
+
+def _config(paths, dt, precision, rng, seed, normals, hazard, recovery, pfe_quantile, device,
+            path_offset):
+    return _Config(int(paths), int(path_offset), float(dt), int(precision), int(rng), int(seed),
+                   _ptr(normals), float(hazard), float(recovery), float(pfe_quantile),
+                   int(device), 0)
+
+
+def simulate_reference(market: Market, swap: InterestRateSwap, paths: int, dt: float = 0.01,
+                       precision: int = FP64, seed: int = 1234,
+                       normals: Optional[np.ndarray] = None, hazard: float = 0.02,
+                       recovery: float = 0.4, pfe_quantile: float = 0.975, device: int = 0,
+                       path_offset: int = 0, keep_paths: bool = False,
+                       blockwise: bool = False) -> ReferenceResult:
+    """Simulate the reference semantics on the GPU and return EE, PFE, CVA (+ SE).
+
+    ``normals`` (float64, [paths][expiry/dt][3]) switches to caller-supplied variates;
+    otherwise the in-kernel Philox generator with ``seed`` is used. ``blockwise`` runs the
+    reference's own kernel decomposition (the baseline form) instead.
+    """
+    lib = library()
+    rng = RNG_PHILOX
+    if normals is not None:
+        normals = np.ascontiguousarray(normals, np.float64)
+        rng = RNG_HOST
+        steps = int(round(swap.expiry / dt))
+        if normals.shape != (paths, steps, 3):
+            raise ValueError("normals must have shape (paths, expiry/dt, 3)")
+    mk = market._c()
+    sw = _Swap(swap.notional, swap.K, swap.expiry, swap.dtau)
+    cfg = _config(paths, dt, precision, rng, seed, normals, hazard, recovery, pfe_quantile,
+                  device, path_offset)
+    ee, pfe = np.zeros(TENORS), np.zeros(TENORS)
+    cva, se = np.zeros(1), np.zeros(1)
+    sums, tm = np.zeros(STAT_SLOTS), np.zeros(5)
+    nx = ny = ex = None
+    if keep_paths:
+        nx, ny, ex = (np.zeros((paths, TENORS)) for _ in range(3))
+    res = _Results(_ptr(ee), _ptr(pfe), _ptr(cva), _ptr(se), _ptr(sums), _ptr(nx), _ptr(ny),
+                   _ptr(ex), _ptr(tm))
+    fn = lib.hjm_simulate_reference_blockwise if blockwise else lib.hjm_simulate_reference
+    _check(fn(C.byref(mk), C.byref(sw), C.byref(cfg), C.byref(res)), fn.__name__)
+    return ReferenceResult(ee, pfe, float(cva[0]), float(se[0]), sums, tm, nx, ny, ex)
+
+
+class ReferencePlan:
+    """A reusable plan: device buffers and stream held across executions (benchmarks)."""
+
+    def __init__(self, market: Market, swap: InterestRateSwap, paths: int, dt: float = 0.01,
+                 precision: int = FP64, seed: int = 1234, hazard: float = 0.02,
+                 recovery: float = 0.4, pfe_quantile: float = 0.975, device: int = 0):
+        self._lib = library()
+        self._market = market
+        mk = market._c()
+        sw = _Swap(swap.notional, swap.K, swap.expiry, swap.dtau)
+        cfg = _config(paths, dt, precision, RNG_PHILOX, seed, None, hazard, recovery,
+                      pfe_quantile, device, 0)
+        self._h = C.c_void_p()
+        _check(self._lib.hjm_plan_create(C.byref(self._h), C.byref(mk), C.byref(sw),
+                                         C.byref(cfg)), "hjm_plan_create")
+        self.paths = paths
+        self.ee, self.pfe = np.zeros(TENORS), np.zeros(TENORS)
+        self.cva, self.se = np.zeros(1), np.zeros(1)
+        self.sums, self.timings_ms = np.zeros(STAT_SLOTS), np.zeros(5)
+        self._res = _Results(_ptr(self.ee), _ptr(self.pfe), _ptr(self.cva), _ptr(self.se),
+                             _ptr(self.sums), None, None, None, _ptr(self.timings_ms))
+
+    def execute(self):
+        _check(self._lib.hjm_plan_execute(self._h, C.byref(self._res)), "hjm_plan_execute")
+        return self
+
+    def close(self):
+        if self._h:
+            self._lib.hjm_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def generate_normals(seed: int, paths: int, nsteps: int, path_offset: int = 0,
+                     device: int = 0) -> np.ndarray:
+    """The in-kernel generator's normals, [paths][nsteps][3], computed on the GPU."""
+    out = np.zeros((paths, nsteps, 3))
+    _check(library().hjm_generate_normals(seed, path_offset, paths, nsteps, device, _ptr(out)),
+           "hjm_generate_normals")
+    return out
+
+
+# ---- drop-in for calculateExposureMultiGPU (simulation_gpu.h:10,12) ---------------
+def calculate_exposure_multi_gpu(expected_exposure: np.ndarray, payOff: InterestRateSwap,
+                                 accrual, spot_rates, drift, volatilities, scale: float,
+                                 num_gpus: int, scenarios: int, dt: float) -> None:
+    """Fill ``expected_exposure`` (51 entries, float64 or float32) like the reference.
+
+    float64 output runs the FP64 path, float32 output the FP32 + compensated path; the
+    scenarios are split into ``num_gpus`` contiguous shards (devices g % device_count),
+    normals come from the in-kernel generator with seed 1234; ``scale`` is unused.
+    """
+    lib = library()
+    ee = expected_exposure
+    if ee.dtype == np.float64:
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (accrual, spot_rates, drift,
+                                                              volatilities)]
+        code = lib.hjm_calculate_exposure_multi_gpu_f64(
+            ee.ctypes.data_as(_dp), payOff.notional, payOff.K, payOff.expiry, payOff.dtau,
+            *[_ptr(a) for a in arrs], float(scale), int(num_gpus), int(scenarios), float(dt))
+    elif ee.dtype == np.float32:
+        fp = C.POINTER(C.c_float)
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (accrual, spot_rates, drift,
+                                                              volatilities)]
+        code = lib.hjm_calculate_exposure_multi_gpu_f32(
+            ee.ctypes.data_as(fp), payOff.notional, payOff.K, payOff.expiry, payOff.dtau,
+            *[a.ctypes.data_as(fp) for a in arrs], float(scale), int(num_gpus),
+            int(scenarios), float(dt))
+    else:
+        raise TypeError("expected_exposure must be float64 or float32")
+    _check(code, "calculateExposureMultiGPU")

@@ -1,0 +1,688 @@
+// hjm_capi.hip -- C ABI of libhjm_cva.so: validation, host precompute, plans, drop-ins.
+//
+// Host code above the kernels. The reference's host orchestration
+// (__calculateExposureMultiGPU, simulation_gpu.cu:955-1241) allocates, generates all
+// normals, launches one kernel per step (or per 50-step block), runs a gemv and sums
+// per-GPU partials on the host; here a plan holds device buffers and a stream, one
+// launch covers every step of every path, and reductions are deterministic FP64.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "hjm_common.h"
+#include "hjm_kernels.h"
+
+using namespace hjm;
+
+#define HJM_TRY(expr)                            \
+    do {                                         \
+        const int _rc = (expr);                  \
+        if (_rc != HJM_OK) return _rc;           \
+    } while (0)
+
+#define HJM_HIP(expr, code)                      \
+    do {                                         \
+        if ((expr) != hipSuccess) return (code); \
+    } while (0)
+
+namespace {
+
+// Restores the caller's current device on scope exit.
+struct DeviceGuard {
+    int prev = -1;
+    int set(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return HJM_ERR_DEVICE;
+        if (dev < 0 || dev >= n) return HJM_ERR_ARG;
+        if (hipSetDevice(dev) != hipSuccess) return HJM_ERR_DEVICE;
+        return HJM_OK;
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+template <typename T>
+int dev_alloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) return HJM_OK;
+    if (hipMalloc((void**)p, count * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return HJM_ERR_ALLOC;
+    }
+    return HJM_OK;
+}
+
+template <typename T>
+void dev_free(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// Time grid. The reference truncates: stride = (int)(dtau/dt) (simulation_cpu.h:200,
+// simulation_gpu.cu:494), steps = (int)(expiry/dt) (:959). Here both ratios are rounded
+// to the nearest integer and must be integral to 1e-6 relative (so a float dt of 0.01f
+// still gives 50 steps per block), the horizon must be a whole number of blocks (else
+// the reference's last block reads the next path's normals), and the last numeraire
+// slot must lie on the 51-point grid (else the reference writes past the curve).
+int ref_grid(double expiry, double dtau, double dt, int* stride, int* nsteps, int* ndates)
+{
+    if (!(dt > 0.0) || !(dtau > 0.0) || !(expiry > 0.0) || !isfinite(expiry)) return HJM_ERR_ARG;
+    const double rs = dtau / dt, rS = expiry / dt;
+    if (rs > 1e7 || rS > 1e9) return HJM_ERR_GRID;
+    const int st = (int)llround(rs);
+    const int S = (int)llround(rS);
+    if (st <= 0 || fabs(rs - st) > 1e-6 * st || fabs(rS - S) > 1e-6 * S) return HJM_ERR_GRID;
+    if (S < st || S % st != 0) return HJM_ERR_GRID;
+    const int D = S / st;
+    if (D > kTenors - 1) return HJM_ERR_GRID;
+    *stride = st;
+    *nsteps = S;
+    *ndates = D;
+    return HJM_OK;
+}
+
+int check_market(const hjm_market* m)
+{
+    if (!m || !m->spot_rates || !m->drifts || !m->volatilities || !m->accrual) return HJM_ERR_NULL;
+    for (int j = 0; j < kTenors; ++j) {
+        if (!isfinite(m->spot_rates[j]) || !isfinite(m->drifts[j]) || !isfinite(m->accrual[j]))
+            return HJM_ERR_ARG;
+        for (int k = 0; k < kFactors; ++k)
+            if (!isfinite(m->volatilities[k * kTenors + j])) return HJM_ERR_ARG;
+    }
+    return HJM_OK;
+}
+
+template <typename Real>
+void fill_consts(RefStepConsts<Real>& k, const hjm_market* m, double dt, double dtau)
+{
+    for (int j = 0; j < kTenors; ++j) {
+        k.ddt[j] = (Real)(m->drifts[j] * dt);
+        k.v0[j] = (Real)m->volatilities[j];
+        k.v1[j] = (Real)m->volatilities[kTenors + j];
+        k.v2[j] = (Real)m->volatilities[2 * kTenors + j];
+        k.spot[j] = (Real)m->spot_rates[j];
+        k.drift[j] = (Real)m->drifts[j];
+    }
+    const double c = dt / dtau;
+    k.c = (Real)c;
+    k.omc = (Real)(1.0 - c);
+    k.opc = (Real)(1.0 + c);
+    k.sqrt_dt = (Real)sqrt(dt);
+    k.dtau = (Real)dtau;
+}
+
+int64_t quantile_rank(int64_t n, double q)
+{
+    int64_t k = (int64_t)ceil(q * (double)n);
+    if (k < 1) k = 1;
+    if (k > n) k = n;
+    return k;
+}
+
+void cva_weights(int ndates, double date_dt, double h, double R, double* w)
+{
+    for (int i = 0; i < ndates; ++i) w[i] = 0.0;
+    for (int i = 1; i < ndates; ++i) {
+        const double pd = exp(-h * (i - 1) * date_dt) - exp(-h * i * date_dt);
+        w[i - 1] += 0.5 * (1.0 - R) * pd;
+        w[i] += 0.5 * (1.0 - R) * pd;
+    }
+}
+
+}  // namespace
+
+struct hjm_plan_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[5] = {};
+    int precision = HJM_FP64;
+    bool host_rng = false;
+    bool do_pfe = false;
+    int64_t paths = 0;
+    int64_t nblocks = 0;
+    int64_t pfe_rank = 1;
+    RefStepConsts<double> kd{};
+    RefStepConsts<float> kf{};
+    RefGrid grid{};
+    ExposureParams ep{};
+    double* d_normals = nullptr;
+    double* d_nx = nullptr;
+    double* d_ny = nullptr;
+    double* d_ex = nullptr;
+    double* d_partials = nullptr;
+    double* d_sums = nullptr;
+    double* d_pfe = nullptr;
+};
+
+namespace {
+
+void plan_release(hjm_plan p)
+{
+    if (!p) return;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(p->device);
+    dev_free(p->d_normals);
+    dev_free(p->d_nx);
+    dev_free(p->d_ny);
+    dev_free(p->d_ex);
+    dev_free(p->d_partials);
+    dev_free(p->d_sums);
+    dev_free(p->d_pfe);
+    for (auto& e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    if (prev >= 0) (void)hipSetDevice(prev);
+    delete p;
+}
+
+int plan_init(hjm_plan p, const hjm_market* market, const hjm_swap* swap, const hjm_config* cfg)
+{
+    HJM_TRY(check_market(market));
+    if (!swap || !cfg) return HJM_ERR_NULL;
+    if (cfg->paths <= 0 || cfg->path_offset < 0 || cfg->path_offset + cfg->paths > (1ll << 32))
+        return HJM_ERR_ARG;
+    if (cfg->precision != HJM_FP64 && cfg->precision != HJM_FP32_KAHAN) return HJM_ERR_ARG;
+    if (cfg->rng != HJM_RNG_PHILOX && cfg->rng != HJM_RNG_HOST) return HJM_ERR_ARG;
+    if (cfg->rng == HJM_RNG_HOST && !cfg->host_normals) return HJM_ERR_NULL;
+    if (!isfinite(swap->notional) || !isfinite(swap->fixed_rate)) return HJM_ERR_ARG;
+    if (!(cfg->hazard_rate >= 0.0) || !(cfg->recovery >= 0.0 && cfg->recovery <= 1.0))
+        return HJM_ERR_ARG;
+    if (cfg->pfe_quantile > 1.0) return HJM_ERR_ARG;
+    int stride, S, D;
+    HJM_TRY(ref_grid(swap->expiry, swap->dtau, cfg->dt, &stride, &S, &D));
+    if ((int64_t)cfg->paths * kTenors > (1ll << 40)) return HJM_ERR_ARG;
+
+    p->device = cfg->device;
+    p->precision = cfg->precision;
+    p->host_rng = cfg->rng == HJM_RNG_HOST;
+    p->paths = cfg->paths;
+    p->nblocks = (cfg->paths + kBlock - 1) / kBlock;
+    p->do_pfe = cfg->pfe_quantile > 0.0;
+    p->pfe_rank = quantile_rank(cfg->paths, cfg->pfe_quantile);
+
+    fill_consts(p->kd, market, cfg->dt, swap->dtau);
+    fill_consts(p->kf, market, cfg->dt, swap->dtau);
+    RefGrid& g = p->grid;
+    g.stride = stride;
+    g.nsteps = S;
+    g.ndates = D;
+    g.paths = cfg->paths;
+    g.path0 = cfg->path_offset;
+    g.seed_lo = (uint32_t)cfg->seed;
+    g.seed_hi = (uint32_t)(cfg->seed >> 32);
+    g.dt = cfg->dt;
+    for (int j = 0; j < kTenors; ++j) {
+        // block 0 keeps the initial curve for `stride` steps (simulation_cpu.h:212-213, 235)
+        double a = 0.0;
+        const double f0 = (p->precision == HJM_FP64) ? market->spot_rates[j]
+                                                     : (double)(float)market->spot_rates[j];
+        for (int s = 0; s < stride; ++s) a += f0;
+        g.a0[j] = a;
+    }
+    ExposureParams& e = p->ep;
+    e.paths = cfg->paths;
+    e.ndates = D;
+    e.notional = swap->notional;
+    e.K = swap->fixed_rate;
+    e.m = 1.0 / swap->dtau;
+    for (int j = 0; j < kTenors; ++j) e.accrual[j] = market->accrual[j];
+    cva_weights(kTenors, swap->dtau, cfg->hazard_rate, cfg->recovery, e.w);
+
+    DeviceGuard guard;
+    HJM_TRY(guard.set(cfg->device));
+    HJM_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), HJM_ERR_DEVICE);
+    for (auto& ev : p->ev) HJM_HIP(hipEventCreate(&ev), HJM_ERR_DEVICE);
+    const size_t slots = (size_t)kTenors * cfg->paths;
+    HJM_TRY(dev_alloc(&p->d_nx, slots));
+    HJM_TRY(dev_alloc(&p->d_ny, slots));
+    HJM_TRY(dev_alloc(&p->d_ex, slots));
+    HJM_TRY(dev_alloc(&p->d_partials, (size_t)p->nblocks * kStatSlots));
+    HJM_TRY(dev_alloc(&p->d_sums, kStatSlots));
+    HJM_TRY(dev_alloc(&p->d_pfe, kTenors));
+    HJM_HIP(hipMemsetAsync(p->d_nx, 0, slots * sizeof(double), p->stream), HJM_ERR_DEVICE);
+    HJM_HIP(hipMemsetAsync(p->d_ny, 0, slots * sizeof(double), p->stream), HJM_ERR_DEVICE);
+    if (p->host_rng) {
+        const size_t n = (size_t)cfg->paths * S * 3;
+        HJM_TRY(dev_alloc(&p->d_normals, n));
+        HJM_HIP(hipMemcpyAsync(p->d_normals, cfg->host_normals, n * sizeof(double),
+                               hipMemcpyHostToDevice, p->stream),
+                HJM_ERR_DEVICE);
+    }
+    HJM_HIP(hipStreamSynchronize(p->stream), HJM_ERR_DEVICE);
+    return HJM_OK;
+}
+
+// Enqueue every kernel of one run on the plan's stream (asynchronous).
+int plan_launch(hjm_plan p, bool keep_exposures)
+{
+    hipStream_t st = p->stream;
+    HJM_HIP(hipEventRecord(p->ev[0], st), HJM_ERR_DEVICE);
+    hipError_t err = (p->precision == HJM_FP64)
+                         ? launch_ref_paths_f64(p->kd, p->grid, p->d_normals, p->d_nx, p->d_ny, st)
+                         : launch_ref_paths_f32k(p->kf, p->grid, p->d_normals, p->d_nx, p->d_ny, st);
+    HJM_HIP(err, HJM_ERR_LAUNCH);
+    HJM_HIP(hipEventRecord(p->ev[1], st), HJM_ERR_DEVICE);
+    double* ex = (p->do_pfe || keep_exposures) ? p->d_ex : nullptr;
+    HJM_HIP(launch_ref_exposure(p->ep, p->d_nx, p->d_ny, ex, p->d_partials, st), HJM_ERR_LAUNCH);
+    HJM_HIP(hipEventRecord(p->ev[2], st), HJM_ERR_DEVICE);
+    HJM_HIP(launch_column_sum(p->d_partials, p->nblocks, kStatSlots, p->d_sums, st), HJM_ERR_LAUNCH);
+    HJM_HIP(hipEventRecord(p->ev[3], st), HJM_ERR_DEVICE);
+    if (p->do_pfe)
+        HJM_HIP(launch_radix_select(p->d_ex, p->grid.ndates, p->paths, p->pfe_rank, p->d_pfe, st),
+                HJM_ERR_LAUNCH);
+    HJM_HIP(hipEventRecord(p->ev[4], st), HJM_ERR_DEVICE);
+    return HJM_OK;
+}
+
+void finish_stats(const double* sums, int64_t P, const double* w, double* ee, double* cva,
+                  double* se)
+{
+    const double alpha = 1.0 / (double)P;
+    double e[kTenors];
+    double c = 0.0;
+    for (int t = 0; t < kTenors; ++t) {
+        e[t] = alpha * sums[t];
+        c += w[t] * e[t];
+    }
+    if (ee) memcpy(ee, e, sizeof(e));
+    if (cva) *cva = c;
+    if (se) {
+        const double s1 = sums[kTenors], s2 = sums[kTenors + 1];
+        const double var = (P > 1) ? (s2 - s1 * s1 / (double)P) / (double)(P - 1) : 0.0;
+        *se = sqrt((var > 0.0 ? var : 0.0) / (double)P);
+    }
+}
+
+// Wait for the run, copy results to the caller, finish the statistics on the host.
+int plan_collect(hjm_plan p, hjm_results* r)
+{
+    HJM_HIP(hipStreamSynchronize(p->stream), HJM_ERR_LAUNCH);
+    double sums[kStatSlots];
+    HJM_HIP(hipMemcpy(sums, p->d_sums, sizeof(sums), hipMemcpyDeviceToHost), HJM_ERR_DEVICE);
+    if (r->sums) memcpy(r->sums, sums, sizeof(sums));
+    finish_stats(sums, p->paths, p->ep.w, r->expected_exposure, r->cva, r->cva_stderr);
+    if (r->pfe) {
+        for (int t = 0; t < kTenors; ++t) r->pfe[t] = 0.0;
+        if (p->do_pfe)
+            HJM_HIP(hipMemcpy(r->pfe, p->d_pfe, sizeof(double) * p->grid.ndates,
+                              hipMemcpyDeviceToHost),
+                    HJM_ERR_DEVICE);
+    }
+    const int64_t P = p->paths;
+    auto transpose_back = [&](const double* dsrc, double* dst) -> int {
+        std::vector<double> tmp((size_t)kTenors * P);
+        if (hipMemcpy(tmp.data(), dsrc, tmp.size() * sizeof(double), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+            return HJM_ERR_DEVICE;
+        for (int64_t i = 0; i < P; ++i)
+            for (int t = 0; t < kTenors; ++t) dst[i * kTenors + t] = tmp[(size_t)t * P + i];
+        return HJM_OK;
+    };
+    if (r->num_x) HJM_TRY(transpose_back(p->d_nx, r->num_x));
+    if (r->num_y) HJM_TRY(transpose_back(p->d_ny, r->num_y));
+    if (r->exposures) HJM_TRY(transpose_back(p->d_ex, r->exposures));
+    if (r->timings_ms) {
+        float ms = 0.f;
+        for (int i = 0; i < 4; ++i) {
+            HJM_HIP(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]), HJM_ERR_DEVICE);
+            r->timings_ms[i] = ms;
+        }
+        HJM_HIP(hipEventElapsedTime(&ms, p->ev[0], p->ev[4]), HJM_ERR_DEVICE);
+        r->timings_ms[4] = ms;
+    }
+    return HJM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hjm_status_string(int status)
+{
+    switch (status) {
+    case HJM_OK: return "ok";
+    case HJM_ERR_NULL: return "required pointer is NULL";
+    case HJM_ERR_ARG: return "invalid argument";
+    case HJM_ERR_GRID: return "inconsistent time grid (dt must divide dtau, whole blocks, "
+                              "dates within the 51-point tenor grid)";
+    case HJM_ERR_DEVICE: return "HIP device error or no device";
+    case HJM_ERR_ALLOC: return "device allocation failed";
+    case HJM_ERR_LAUNCH: return "kernel launch or execution failed";
+    case HJM_ERR_UNSUPPORTED: return "unsupported option combination";
+    default: return "unknown status";
+    }
+}
+
+int hjm_device_count(int* count)
+{
+    if (!count) return HJM_ERR_NULL;
+    *count = 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return HJM_ERR_DEVICE;
+    }
+    *count = n;
+    return HJM_OK;
+}
+
+int hjm_cva_weights(int ndates, double date_dt, double hazard_rate, double recovery,
+                    double* weights)
+{
+    if (!weights) return HJM_ERR_NULL;
+    if (ndates <= 0 || !(date_dt > 0.0) || !(hazard_rate >= 0.0) ||
+        !(recovery >= 0.0 && recovery <= 1.0))
+        return HJM_ERR_ARG;
+    cva_weights(ndates, date_dt, hazard_rate, recovery, weights);
+    return HJM_OK;
+}
+
+// Cumulative Simpson on a uniform grid: composite 1/3 rule to even nodes, 3/8 rule on
+// the last three intervals for odd nodes, the 3-point quadratic on the first interval.
+int hjm_hjm_drift(const double* vols, double dtau, double* drifts)
+{
+    if (!vols || !drifts) return HJM_ERR_NULL;
+    if (!(dtau > 0.0)) return HJM_ERR_ARG;
+    const double h = dtau;
+    for (int j = 0; j < kTenors; ++j) drifts[j] = 0.0;
+    for (int k = 0; k < kFactors; ++k) {
+        const double* y = vols + k * kTenors;
+        for (int j = 1; j < kTenors; ++j) {
+            double s = 0.0;
+            if (j == 1) {
+                s = h / 12.0 * (5.0 * y[0] + 8.0 * y[1] - y[2]);
+            } else {
+                const int even_end = (j % 2 == 0) ? j : j - 3;
+                for (int i = 0; i < even_end; i += 2) s += h / 3.0 * (y[i] + 4.0 * y[i + 1] + y[i + 2]);
+                if (j % 2 == 1) s += 3.0 * h / 8.0 * (y[j - 3] + 3.0 * y[j - 2] + 3.0 * y[j - 1] + y[j]);
+            }
+            drifts[j] += y[j] * s;
+        }
+    }
+    return HJM_OK;
+}
+
+int hjm_plan_create(hjm_plan* out, const hjm_market* market, const hjm_swap* swap,
+                    const hjm_config* config)
+{
+    if (!out) return HJM_ERR_NULL;
+    *out = nullptr;
+    hjm_plan p = new (std::nothrow) hjm_plan_s();
+    if (!p) return HJM_ERR_ALLOC;
+    const int rc = plan_init(p, market, swap, config);
+    if (rc != HJM_OK) {
+        plan_release(p);
+        return rc;
+    }
+    *out = p;
+    return HJM_OK;
+}
+
+int hjm_plan_execute(hjm_plan plan, hjm_results* results)
+{
+    if (!plan || !results) return HJM_ERR_NULL;
+    DeviceGuard guard;
+    HJM_TRY(guard.set(plan->device));
+    HJM_TRY(plan_launch(plan, results->exposures != nullptr));
+    return plan_collect(plan, results);
+}
+
+void hjm_plan_destroy(hjm_plan plan) { plan_release(plan); }
+
+int hjm_simulate_reference(const hjm_market* market, const hjm_swap* swap,
+                           const hjm_config* config, hjm_results* results)
+{
+    if (!results) return HJM_ERR_NULL;
+    hjm_plan p = nullptr;
+    HJM_TRY(hjm_plan_create(&p, market, swap, config));
+    const int rc = hjm_plan_execute(p, results);
+    hjm_plan_destroy(p);
+    return rc;
+}
+
+int hjm_simulate_reference_blockwise(const hjm_market* market, const hjm_swap* swap,
+                                     const hjm_config* cfg, hjm_results* r)
+{
+    if (!r) return HJM_ERR_NULL;
+    HJM_TRY(check_market(market));
+    if (!swap || !cfg) return HJM_ERR_NULL;
+    if (cfg->precision != HJM_FP64) return HJM_ERR_UNSUPPORTED;
+    if (cfg->paths <= 0 || cfg->paths > (1ll << 31) - 1 || cfg->path_offset < 0) return HJM_ERR_ARG;
+    if (cfg->rng == HJM_RNG_HOST && !cfg->host_normals) return HJM_ERR_NULL;
+    int stride, S, D;
+    HJM_TRY(ref_grid(swap->expiry, swap->dtau, cfg->dt, &stride, &S, &D));
+    DeviceGuard guard;
+    HJM_TRY(guard.set(cfg->device));
+    const int64_t P = cfg->paths;
+    RefStepConsts<double> k{};
+    fill_consts(k, market, cfg->dt, swap->dtau);
+    RefGrid g{};
+    g.stride = stride;
+    g.nsteps = S;
+    g.ndates = D;
+    g.paths = P;
+    g.path0 = cfg->path_offset;
+    g.dt = cfg->dt;
+    ExposureParams e{};
+    e.paths = P;
+    e.ndates = D;
+    e.notional = swap->notional;
+    e.K = swap->fixed_rate;
+    e.m = 1.0 / swap->dtau;
+    for (int j = 0; j < kTenors; ++j) e.accrual[j] = market->accrual[j];
+    cva_weights(kTenors, swap->dtau, cfg->hazard_rate, cfg->recovery, e.w);
+
+    struct Bufs {
+        double *normals = nullptr, *s0 = nullptr, *s1 = nullptr, *integ = nullptr, *nx = nullptr,
+               *ny = nullptr, *ex = nullptr, *sums = nullptr;
+        hipStream_t st = nullptr;
+        hipEvent_t ev[5] = {};
+        ~Bufs()
+        {
+            dev_free(normals);
+            dev_free(s0);
+            dev_free(s1);
+            dev_free(integ);
+            dev_free(nx);
+            dev_free(ny);
+            dev_free(ex);
+            dev_free(sums);
+            for (auto& x : ev)
+                if (x) (void)hipEventDestroy(x);
+            if (st) (void)hipStreamDestroy(st);
+        }
+    } b;
+    const size_t nn = (size_t)P * S * 3, cells = (size_t)P * kTenors;
+    HJM_HIP(hipStreamCreateWithFlags(&b.st, hipStreamNonBlocking), HJM_ERR_DEVICE);
+    for (auto& x : b.ev) HJM_HIP(hipEventCreate(&x), HJM_ERR_DEVICE);
+    HJM_TRY(dev_alloc(&b.normals, nn));
+    HJM_TRY(dev_alloc(&b.s0, cells));
+    HJM_TRY(dev_alloc(&b.s1, cells));
+    HJM_TRY(dev_alloc(&b.integ, cells));
+    HJM_TRY(dev_alloc(&b.nx, cells));
+    HJM_TRY(dev_alloc(&b.ny, cells));
+    HJM_TRY(dev_alloc(&b.ex, cells));
+    HJM_TRY(dev_alloc(&b.sums, kTenors));
+    HJM_HIP(hipMemsetAsync(b.integ, 0, cells * sizeof(double), b.st), HJM_ERR_DEVICE);
+    HJM_HIP(hipMemsetAsync(b.nx, 0, cells * sizeof(double), b.st), HJM_ERR_DEVICE);
+    HJM_HIP(hipMemsetAsync(b.ny, 0, cells * sizeof(double), b.st), HJM_ERR_DEVICE);
+    HJM_HIP(hipEventRecord(b.ev[0], b.st), HJM_ERR_DEVICE);
+    if (cfg->rng == HJM_RNG_HOST)
+        HJM_HIP(hipMemcpyAsync(b.normals, cfg->host_normals, nn * sizeof(double),
+                               hipMemcpyHostToDevice, b.st),
+                HJM_ERR_DEVICE);
+    else
+        HJM_HIP(launch_normals(cfg->seed, cfg->path_offset, P, S, b.normals, b.st), HJM_ERR_LAUNCH);
+    HJM_HIP(hipEventRecord(b.ev[1], b.st), HJM_ERR_DEVICE);
+    double* cur = b.s0;
+    double* nxt = b.s1;
+    for (int blk = 0; blk < D; ++blk) {
+        HJM_HIP(launch_blockwise_paths(k, g, blk, b.normals, cur, nxt, b.integ, b.nx, b.ny, b.st),
+                HJM_ERR_LAUNCH);
+        double* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    HJM_HIP(hipEventRecord(b.ev[2], b.st), HJM_ERR_DEVICE);
+    HJM_HIP(launch_blockwise_exposure(e, b.nx, b.ny, b.ex, b.st), HJM_ERR_LAUNCH);
+    HJM_HIP(hipEventRecord(b.ev[3], b.st), HJM_ERR_DEVICE);
+    HJM_HIP(launch_column_mean_gemv(b.ex, P, b.sums, b.st), HJM_ERR_LAUNCH);
+    HJM_HIP(hipEventRecord(b.ev[4], b.st), HJM_ERR_DEVICE);
+    HJM_HIP(hipStreamSynchronize(b.st), HJM_ERR_LAUNCH);
+    double sums[kStatSlots] = {};
+    HJM_HIP(hipMemcpy(sums, b.sums, sizeof(double) * kTenors, hipMemcpyDeviceToHost), HJM_ERR_DEVICE);
+    finish_stats(sums, P, e.w, r->expected_exposure, r->cva, nullptr);
+    if (r->cva_stderr) *r->cva_stderr = NAN;
+    if (r->sums) memcpy(r->sums, sums, sizeof(sums));
+    if (r->pfe)
+        for (int t = 0; t < kTenors; ++t) r->pfe[t] = NAN;
+    auto back = [&](const double* d, double* h, bool slot_major) -> int {
+        std::vector<double> tmp(cells);
+        if (hipMemcpy(tmp.data(), d, cells * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            return HJM_ERR_DEVICE;
+        for (int64_t i = 0; i < P; ++i)
+            for (int t = 0; t < kTenors; ++t)
+                h[i * kTenors + t] = slot_major ? tmp[(size_t)t * P + i] : tmp[(size_t)i * kTenors + t];
+        return HJM_OK;
+    };
+    if (r->num_x) HJM_TRY(back(b.nx, r->num_x, true));
+    if (r->num_y) HJM_TRY(back(b.ny, r->num_y, true));
+    if (r->exposures) HJM_TRY(back(b.ex, r->exposures, false));
+    if (r->timings_ms) {
+        // [0] normals generation, [1] path blocks, [2] exposure, [3] EE gemv, [4] total
+        float ms = 0.f;
+        for (int i = 0; i < 4; ++i) {
+            HJM_HIP(hipEventElapsedTime(&ms, b.ev[i], b.ev[i + 1]), HJM_ERR_DEVICE);
+            r->timings_ms[i] = ms;
+        }
+        HJM_HIP(hipEventElapsedTime(&ms, b.ev[0], b.ev[4]), HJM_ERR_DEVICE);
+        r->timings_ms[4] = ms;
+    }
+    return HJM_OK;
+}
+
+int hjm_generate_normals(uint64_t seed, int64_t path_offset, int64_t paths, int32_t nsteps,
+                         int32_t device, double* out)
+{
+    if (!out) return HJM_ERR_NULL;
+    if (paths <= 0 || nsteps <= 0 || path_offset < 0 || paths * (int64_t)nsteps > (1ll << 31))
+        return HJM_ERR_ARG;
+    DeviceGuard guard;
+    HJM_TRY(guard.set(device));
+    double* d = nullptr;
+    const size_t n = (size_t)paths * nsteps * 3;
+    HJM_TRY(dev_alloc(&d, n));
+    int rc = HJM_OK;
+    if (launch_normals(seed, path_offset, paths, nsteps, d, nullptr) != hipSuccess)
+        rc = HJM_ERR_LAUNCH;
+    else if (hipDeviceSynchronize() != hipSuccess)
+        rc = HJM_ERR_LAUNCH;
+    else if (hipMemcpy(out, d, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = HJM_ERR_DEVICE;
+    dev_free(d);
+    return rc;
+}
+
+// simulation_gpu.cu:955-1241 semantics: scenarios split into num_gpus contiguous shards
+// (the remainder is spread over the first shards instead of being dropped, :960), one
+// shard per device g % device_count, shards enqueued before any is awaited, partial
+// sums combined on the host in shard order and divided by the total (:1154-1179).
+static int multi_gpu_ee(double* ee, const hjm_market* mk, const hjm_swap* sw, int precision,
+                        int num_gpus, int scenarios, double dt)
+{
+    if (!ee) return HJM_ERR_NULL;
+    if (num_gpus <= 0 || scenarios <= 0) return HJM_ERR_ARG;
+    if (num_gpus > scenarios) num_gpus = scenarios;
+    int ndev = 0;
+    HJM_TRY(hjm_device_count(&ndev));
+    if (ndev <= 0) return HJM_ERR_DEVICE;
+    std::vector<hjm_plan> plans(num_gpus, nullptr);
+    int rc = HJM_OK;
+    const int64_t base = scenarios / num_gpus, rem = scenarios % num_gpus;
+    int64_t lo = 0;
+    for (int gi = 0; gi < num_gpus && rc == HJM_OK; ++gi) {
+        hjm_config c{};
+        c.paths = base + (gi < rem ? 1 : 0);
+        c.path_offset = lo;
+        lo += c.paths;
+        c.dt = dt;
+        c.precision = precision;
+        c.rng = HJM_RNG_PHILOX;
+        c.seed = 1234ull;
+        c.hazard_rate = 0.0;
+        c.recovery = 0.0;
+        c.pfe_quantile = 0.0;
+        c.device = gi % ndev;
+        rc = hjm_plan_create(&plans[gi], mk, sw, &c);
+    }
+    for (int gi = 0; gi < num_gpus && rc == HJM_OK; ++gi) {
+        DeviceGuard guard;
+        rc = guard.set(plans[gi]->device);
+        if (rc == HJM_OK) rc = plan_launch(plans[gi], false);
+    }
+    double total[kStatSlots] = {};
+    for (int gi = 0; gi < num_gpus && rc == HJM_OK; ++gi) {
+        double sums[kStatSlots];
+        hjm_results r{};
+        r.sums = sums;
+        DeviceGuard guard;
+        rc = guard.set(plans[gi]->device);
+        if (rc == HJM_OK) rc = plan_collect(plans[gi], &r);
+        if (rc == HJM_OK)
+            for (int t = 0; t < kStatSlots; ++t) total[t] += sums[t];
+    }
+    for (auto p : plans) hjm_plan_destroy(p);
+    if (rc != HJM_OK) return rc;
+    const double avg = 1.0 / (double)scenarios;
+    for (int t = 0; t < kTenors; ++t) ee[t] = avg * total[t];
+    return HJM_OK;
+}
+
+int hjm_calculate_exposure_multi_gpu_f64(double* expected_exposure, double notional, double K,
+                                         double expiry, double dtau, const double* accrual,
+                                         const double* spot_rates, const double* drift,
+                                         const double* volatilities, double scale,
+                                         int num_gpus, int scenarios, double dt)
+{
+    (void)scale;
+    const hjm_market mk{spot_rates, drift, volatilities, accrual};
+    const hjm_swap sw{notional, K, expiry, dtau};
+    return multi_gpu_ee(expected_exposure, &mk, &sw, HJM_FP64, num_gpus, scenarios, dt);
+}
+
+int hjm_calculate_exposure_multi_gpu_f32(float* expected_exposure, float notional, float K,
+                                         float expiry, float dtau, const float* accrual,
+                                         const float* spot_rates, const float* drift,
+                                         const float* volatilities, float scale,
+                                         int num_gpus, int scenarios, float dt)
+{
+    (void)scale;
+    if (!expected_exposure || !accrual || !spot_rates || !drift || !volatilities)
+        return HJM_ERR_NULL;
+    double acc[kTenors], spot[kTenors], dr[kTenors], vol[kFactors * kTenors], ee[kTenors];
+    for (int j = 0; j < kTenors; ++j) {
+        acc[j] = accrual[j];
+        spot[j] = spot_rates[j];
+        dr[j] = drift[j];
+    }
+    for (int j = 0; j < kFactors * kTenors; ++j) vol[j] = volatilities[j];
+    const hjm_market mk{spot, dr, vol, acc};
+    const hjm_swap sw{notional, K, expiry, dtau};
+    const int rc = multi_gpu_ee(ee, &mk, &sw, HJM_FP32_KAHAN, num_gpus, scenarios, (double)dt);
+    if (rc == HJM_OK)
+        for (int t = 0; t < kTenors; ++t) expected_exposure[t] = (float)ee[t];
+    return rc;
+}
+
+}  // extern "C"

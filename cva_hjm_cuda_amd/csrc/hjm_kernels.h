@@ -1,0 +1,39 @@
+// hjm_kernels.h -- launchers of the HJM kernels (internal to libhjm_cva.so).
+#pragma once
+#include "hjm_common.h"
+
+namespace hjm {
+
+struct ExposureParams {
+    int64_t paths;
+    int ndates;
+    int pad;
+    double notional, K;
+    double m;                // 1 / dtau (compounding frequency of the simple rate)
+    double accrual[kTenors];
+    double w[kTenors];       // CVA weights (1-R) x default probability shares
+};
+
+hipError_t launch_ref_paths_f64(const RefStepConsts<double>& k, const RefGrid& g,
+                                const double* normals, double* nx, double* ny, hipStream_t st);
+hipError_t launch_ref_paths_f32k(const RefStepConsts<float>& k, const RefGrid& g,
+                                 const double* normals, double* nx, double* ny, hipStream_t st);
+hipError_t launch_ref_exposure(const ExposureParams& e, const double* nx, const double* ny,
+                               double* exposures, double* partials, hipStream_t st);
+hipError_t launch_column_sum(const double* partials, int64_t rows, int cols, double* out,
+                             hipStream_t st);
+hipError_t launch_radix_select(const double* x, int rows, int64_t n, int64_t k, double* out,
+                               hipStream_t st);
+hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nsteps, double* out,
+                          hipStream_t st);
+
+// The reference's own GPU design (baseline form).
+hipError_t launch_blockwise_paths(const RefStepConsts<double>& k, const RefGrid& g, int block_index,
+                                  const double* normals, const double* state_in, double* state_out,
+                                  double* integral, double* nx, double* ny, hipStream_t st);
+hipError_t launch_blockwise_exposure(const ExposureParams& e, const double* nx, const double* ny,
+                                     double* exposures, hipStream_t st);
+hipError_t launch_column_mean_gemv(const double* exposures, int64_t paths, double* sums,
+                                   hipStream_t st);
+
+}  // namespace hjm

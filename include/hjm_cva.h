@@ -1,0 +1,155 @@
+/*
+ * hjm_cva.h -- C ABI of the MI355X HJM Monte Carlo exposure library (libhjm_cva.so).
+ *
+ * Plain C types only (pointers to host memory, sizes, POD structs); every function
+ * returns an hjm_status code instead of printing and continuing as the reference does
+ * (simulation_gpu.cu:71-106). All arrays are caller-owned host memory. Calls are
+ * synchronous and re-entrant: no process-global market data (the reference keeps it
+ * in __constant__ symbols, simulation_gpu.cu:109-119), per-call or per-plan buffers.
+ *
+ * Grid: 51 Musiela tenors tau_j = j * dtau (j = 0..50), 3 volatility factors,
+ * Euler step dt; exposure dates every dtau. The reference's public entry points
+ * (simulation_gpu.h:10,12) are re-exposed with their exact C++ signatures by
+ * include/simulation_gpu.h on top of this ABI.
+ */
+#ifndef HJM_CVA_H
+#define HJM_CVA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HJM_TENORS 51
+#define HJM_FACTORS 3
+#define HJM_MAX_SWAPS 4096
+#define HJM_MAX_SETS 512
+
+enum hjm_status_code {
+    HJM_OK = 0,
+    HJM_ERR_NULL = 1,        /* a required pointer is NULL */
+    HJM_ERR_ARG = 2,         /* invalid scalar argument (paths <= 0, dt <= 0, ...) */
+    HJM_ERR_GRID = 3,        /* dt does not divide dtau, horizon not a whole number of
+                                blocks, or a date beyond the 51-point tenor grid */
+    HJM_ERR_DEVICE = 4,      /* no HIP device, or a HIP runtime error */
+    HJM_ERR_ALLOC = 5,       /* device memory allocation failed */
+    HJM_ERR_LAUNCH = 6,      /* kernel launch or execution failed */
+    HJM_ERR_UNSUPPORTED = 7  /* option combination not implemented */
+};
+
+enum hjm_precision {
+    HJM_FP64 = 0,       /* FP64 evolution and integrals */
+    HJM_FP32_KAHAN = 1  /* FP32 evolution, Kahan-compensated FP32 integrals; FP64 stats */
+};
+
+enum hjm_rng_mode {
+    HJM_RNG_PHILOX = 0, /* in-kernel Philox4x32-10 + Box-Muller, counter (step, path) */
+    HJM_RNG_HOST = 1    /* caller-supplied normals, layout [path][step][3] */
+};
+
+/* Market data (simulation_gpu.h:10 arguments accrual, spot_rates, drift, volatilities). */
+typedef struct {
+    const double* spot_rates;   /* [51] initial forward curve f(0, tau_j) */
+    const double* drifts;       /* [51] HJM drift m(tau_j) */
+    const double* volatilities; /* [3*51] factor-major loadings sigma_k(tau_j) */
+    const double* accrual;      /* [51] year fraction of the cash flow at date j */
+} hjm_market;
+
+/* The fields of InterestRateSwap<real> that the reference reads (product.h:4-17). */
+typedef struct {
+    double notional;
+    double fixed_rate; /* K */
+    double expiry;     /* years; dates at dtau, 2 dtau, ..., expiry */
+    double dtau;       /* tenor spacing = payment period (0.5) */
+} hjm_swap;
+
+typedef struct {
+    int64_t paths;          /* Monte Carlo paths (scenarios) in this call */
+    int64_t path_offset;    /* global id of the first path: shards of one run share a
+                               seed and take disjoint offsets */
+    double dt;              /* Euler step (years) */
+    int32_t precision;      /* enum hjm_precision */
+    int32_t rng;            /* enum hjm_rng_mode */
+    uint64_t seed;          /* Philox key (HJM_RNG_PHILOX) */
+    const double* host_normals; /* HJM_RNG_HOST: [paths][expiry/dt][3] N(0,1) variates */
+    double hazard_rate;     /* flat counterparty hazard rate for CVA */
+    double recovery;        /* recovery rate for CVA */
+    double pfe_quantile;    /* e.g. 0.975; <= 0 skips the PFE selection */
+    int32_t device;         /* HIP device ordinal */
+    int32_t reserved;
+} hjm_config;
+
+/* Outputs; every pointer may be NULL (then that output is not produced / copied). */
+typedef struct {
+    double* expected_exposure; /* [51] EE(t_j), discounted as in the reference */
+    double* pfe;               /* [51] PFE quantile of the exposure at t_j */
+    double* cva;               /* [1] (1-R) sum_j w_j EE_j */
+    double* cva_stderr;        /* [1] Monte Carlo standard error of cva */
+    double* sums;              /* [53] raw FP64 sums: sum_p E_pj (51), sum_p CVA_p,
+                                  sum_p CVA_p^2 -- shards combine these */
+    double* num_x;             /* [paths][51] numeraire forward (slot j), 0 if unused */
+    double* num_y;             /* [paths][51] numeraire discount factor (slot j) */
+    double* exposures;         /* [paths][51] per-path exposure E_pj */
+    double* timings_ms;        /* [5] path kernel, exposure kernel, reduction,
+                                  PFE selection, all device work (HIP events) */
+} hjm_results;
+
+const char* hjm_status_string(int status);
+int hjm_device_count(int* count);
+
+/* CVA weights w_j = (1-R) * trapezoid share of the default probability of
+ * [t_{j-1}, t_j], S(t) = exp(-h t), t_j = j * date_dt, j = 0..ndates-1. */
+int hjm_cva_weights(int ndates, double date_dt, double hazard_rate, double recovery,
+                    double* weights);
+
+/* HJM no-arbitrage drift m(tau_j) = sum_k sigma_k(tau_j) int_0^tau_j sigma_k, cumulative
+ * Simpson on the tenor grid (reproduces the reference's drift table,
+ * simulation_gpu_testing.cpp:66-71, from its volatilities, :47-63). */
+int hjm_hjm_drift(const double* volatilities, double dtau, double* drifts);
+
+/* ---- reference semantics: single payer IRS, diagonal numeraire ------------------
+ * Path generation and exposure of simulation_cpu.h:188-296 / simulation_gpu.cu:471-587
+ * (block 0 frozen at the initial curve, numeraire slot i = tenor i after block i-1,
+ * exposure = positive part of the sum of later discounted cash flows).            */
+typedef struct hjm_plan_s* hjm_plan;
+int hjm_plan_create(hjm_plan* plan, const hjm_market* market, const hjm_swap* swap,
+                    const hjm_config* config);
+int hjm_plan_execute(hjm_plan plan, hjm_results* results);
+void hjm_plan_destroy(hjm_plan plan);
+int hjm_simulate_reference(const hjm_market* market, const hjm_swap* swap,
+                           const hjm_config* config, hjm_results* results);
+
+/* The reference's design on the GPU, kept as the "plain FP64 GPU form" baseline:
+ * all normals materialised in HBM first, one workgroup per path with one tenor per
+ * thread, curve in LDS, one launch per 50-step block (simulation_gpu.cu:471-669),
+ * exposure per workgroup (:803-858). Same outputs as hjm_simulate_reference. */
+int hjm_simulate_reference_blockwise(const hjm_market* market, const hjm_swap* swap,
+                                     const hjm_config* config, hjm_results* results);
+
+/* ---- drop-in for simulation_gpu.h:10,12 (calculateExposureMultiGPU) --------------
+ * EE only; scenarios are split into num_gpus contiguous shards run on devices
+ * g % device_count; in-kernel Philox keyed by seed 1234 (the reference's seed,
+ * simulation_gpu.cu:962); `scale` is unused as in the reference. The double overload
+ * runs HJM_FP64, the float overload HJM_FP32_KAHAN.                                */
+int hjm_calculate_exposure_multi_gpu_f64(double* expected_exposure, double notional, double K,
+                                         double expiry, double dtau, const double* accrual,
+                                         const double* spot_rates, const double* drift,
+                                         const double* volatilities, double scale,
+                                         int num_gpus, int scenarios, double dt);
+int hjm_calculate_exposure_multi_gpu_f32(float* expected_exposure, float notional, float K,
+                                         float expiry, float dtau, const float* accrual,
+                                         const float* spot_rates, const float* drift,
+                                         const float* volatilities, float scale,
+                                         int num_gpus, int scenarios, float dt);
+
+/* ---- normals (for tests and host-supplied runs): the in-kernel generator's output
+ * for paths [path_offset, path_offset + paths) and steps [0, nsteps), written as
+ * [paths][nsteps][3] doubles. Runs on the GPU.                                       */
+int hjm_generate_normals(uint64_t seed, int64_t path_offset, int64_t paths, int32_t nsteps,
+                         int32_t device, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

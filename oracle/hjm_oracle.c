@@ -234,15 +234,19 @@ static double dfau(int t, const double* rates)
     return (t == ORC_TENORS - 1) ? rates[t] - rates[t - 1] : rates[t + 1] - rates[t];
 }
 
-/* stride and step count exactly as the reference computes them (simulation_cpu.h:200,
- * :384, :409), plus the checks the reference lacks: dt must divide dtau, the path must
- * be a whole number of blocks, and the last numeraire slot must lie on the grid. */
+/* Time grid: the reference truncates stride = (int)(dtau/dt) and steps = (int)(expiry/dt)
+ * (simulation_cpu.h:200, :384, :409); both ratios are rounded here and must be integral
+ * to 1e-6 relative, the path must be a whole number of blocks, and the last numeraire
+ * slot must lie on the grid (checks the reference lacks). Identical to the reference
+ * whenever dtau/dt and expiry/dt are integers, as in every configuration. */
 static int ref_grid(double expiry, double dtau, double dt, int* stride, int* nsteps, int* ndates)
 {
     if (!(dt > 0.0) || !(dtau > 0.0) || !(expiry > 0.0)) return 2;
-    int st = (int)(dtau / dt);
-    int S = (int)(expiry / dt);
-    if (st <= 0 || fabs(st * dt - dtau) > 1e-9 * dtau) return 3;
+    double rs = dtau / dt, rS = expiry / dt;
+    if (rs > 1e7 || rS > 1e9) return 3;
+    int st = (int)llround(rs);
+    int S = (int)llround(rS);
+    if (st <= 0 || fabs(rs - st) > 1e-6 * st || fabs(rS - S) > 1e-6 * S) return 3;
     if (S < st || S % st != 0) return 3;
     int D = S / st;
     if (D > ORC_TENORS - 1) return 3;

@@ -1,0 +1,135 @@
+"""C ABI checks that need no GPU: the library loads, exports every function that
+include/hjm_cva.h declares, its host-side utilities agree with the oracle, and invalid
+arguments are rejected with status codes before any device work."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cva_hjm_cuda_amd as hjm
+from cva_hjm_cuda_amd import api
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for fn in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"\b(hjm_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_all_declared_symbols():
+    lib = hjm.library()
+    names = declared_functions()
+    assert len(names) >= 12
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_status_strings():
+    lib = hjm.library()
+    for code in range(8):
+        assert lib.hjm_status_string(code)
+    assert b"grid" in lib.hjm_status_string(3)
+
+
+def test_drift_matches_oracle(oracle):
+    import json
+    mk = json.load(open(os.path.join(ROOT, "tests", "golden", "legacy_market.json")))
+    v = np.array(mk["volatilities"])
+    assert np.allclose(hjm.hjm_drift(v), oracle.hjm_drift(v), rtol=0, atol=1e-18)
+    assert np.array_equal(np.round(hjm.hjm_drift(v), 7), np.array(mk["drifts"]))
+
+
+def test_cva_weights_match_oracle(oracle):
+    for h, R in ((0.02, 0.4), (0.05, 0.0), (0.0, 0.4)):
+        assert np.allclose(hjm.cva_weights(h, R), oracle.cva_weights(51, 0.5, h, R), rtol=1e-15,
+                           atol=0)
+
+
+def test_benchmark_market_matches_oracle(oracle):
+    m = hjm.benchmark_market()
+    assert np.allclose(m.spot_rates, oracle.nelson_siegel(), rtol=1e-15, atol=0)
+    assert np.allclose(m.volatilities, oracle.poly_vols(), rtol=0, atol=1e-16)
+    assert np.allclose(m.drifts, oracle.hjm_drift(oracle.poly_vols()), rtol=0, atol=1e-18)
+
+
+def _args(paths=16, expiry=5.0, dt=0.01, normals=None):
+    m = hjm.benchmark_market()
+    mk = m._c()
+    sw = api._Swap(1e6, 0.025, expiry, 0.5)
+    cfg = api._config(paths, dt, hjm.FP64, api.RNG_PHILOX if normals is None else api.RNG_HOST,
+                      1, normals, 0.02, 0.4, 0.975, 0, 0)
+    res = api._Results()
+    return m, mk, sw, cfg, res
+
+
+@pytest.mark.parametrize("case,code", [
+    ("null_market", 1), ("null_results", 1), ("paths0", 2), ("dt_not_dividing", 3),
+    ("partial_block", 3), ("beyond_grid", 3), ("bad_recovery", 2), ("host_rng_no_normals", 1),
+    ("bad_precision", 2)])
+def test_argument_errors_before_device(case, code):
+    lib = hjm.library()
+    m, mk, sw, cfg, res = _args()
+    mkp, resp = C.byref(mk), C.byref(res)
+    if case == "null_market":
+        mkp = None
+    elif case == "null_results":
+        resp = None
+    elif case == "paths0":
+        cfg.paths = 0
+    elif case == "dt_not_dividing":
+        cfg.dt = 0.03
+    elif case == "partial_block":
+        sw.expiry = 5.25
+    elif case == "beyond_grid":
+        sw.expiry = 26.0
+    elif case == "bad_recovery":
+        cfg.recovery = 1.5
+    elif case == "host_rng_no_normals":
+        cfg.rng = api.RNG_HOST
+    elif case == "bad_precision":
+        cfg.precision = 9
+    assert lib.hjm_simulate_reference(mkp, C.byref(sw), C.byref(cfg), resp) == code
+
+
+def test_nan_market_rejected():
+    lib = hjm.library()
+    m, mk, sw, cfg, res = _args()
+    m.volatilities[5] = np.nan
+    assert lib.hjm_simulate_reference(C.byref(mk), C.byref(sw), C.byref(cfg), C.byref(res)) == 2
+
+
+def test_dropin_argument_errors():
+    lib = hjm.library()
+    m = hjm.benchmark_market()
+    ee = np.zeros(51)
+    p = api._ptr
+    assert lib.hjm_calculate_exposure_multi_gpu_f64(None, 1.0, 0.02, 5.0, 0.5, p(m.accrual),
+                                                    p(m.spot_rates), p(m.drifts),
+                                                    p(m.volatilities), 0.0, 1, 100, 0.01) == 1
+    assert lib.hjm_calculate_exposure_multi_gpu_f64(p(ee), 1.0, 0.02, 5.0, 0.5, p(m.accrual),
+                                                    p(m.spot_rates), p(m.drifts),
+                                                    p(m.volatilities), 0.0, 0, 100, 0.01) == 2
+
+
+def test_no_cpu_fallback_without_device():
+    """On a host without a GPU the product reports a device error; it never computes."""
+    if hjm.device_count() > 0 if _has_device() else False:
+        pytest.skip("a GPU is present")
+    m = hjm.benchmark_market()
+    with pytest.raises(hjm.HJMError) as ei:
+        hjm.simulate_reference(m, hjm.InterestRateSwap(1e6, 0.025, 5.0), 64)
+    assert ei.value.code == 4
+
+
+def _has_device():
+    try:
+        return hjm.device_count() > 0
+    except hjm.HJMError:
+        return False

@@ -1,0 +1,208 @@
+"""GPU parity of the reference-semantics path (HIP kernels through the C ABI) with the
+oracle (restated simulation_cpu.h), on identical normals, plus size-independent
+properties at the full benchmark size.
+
+Tolerances: FP64 per-date EE within 1e-10 relative of the oracle (exactly 0 where the
+oracle is 0); FP32 + compensated within 1e-4 relative on CVA; PFE is the exact order
+statistic of the returned exposures; normals are bit-identical to the oracle's.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import cva_hjm_cuda_amd as hjm
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# Plain-method configuration: 1 payer IRS, N=1e6, K=2.5%, 5y, semiannual; 1000 paths of
+# host-supplied normals (seed 42); flat hazard 2%, recovery 40%.
+C1 = dict(paths=1000, expiry=5.0, notional=1e6, K=0.025, seed=42, hazard=0.02, recovery=0.4)
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def c1(oracle):
+    _flush("[gpu] config-1 oracle run")
+    m = hjm.benchmark_market()
+    S = int(C1["expiry"] / 0.01)
+    z = oracle.normals(C1["seed"], C1["paths"], S)
+    nx, ny = oracle.reference_paths(m.spot_rates, m.drifts, m.volatilities, C1["expiry"], 0.5,
+                                    0.01, z)
+    ex = oracle.reference_exposure(nx, ny, m.accrual, C1["notional"], C1["K"], 0.5)
+    w = oracle.cva_weights(51, 0.5, C1["hazard"], C1["recovery"])
+    ee, pfe, cva, se = oracle.stats(ex, w, 0.975)
+    swap = hjm.InterestRateSwap(C1["notional"], C1["K"], C1["expiry"])
+    return dict(m=m, z=z, nx=nx, ny=ny, ex=ex, ee=ee, pfe=pfe, cva=cva, se=se, swap=swap)
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    den = np.maximum(np.abs(b), 1e-300)
+    return np.where(b == 0, np.abs(a), np.abs(a - b) / den)
+
+
+def test_gpu_normals_bit_identical_to_oracle(oracle):
+    _flush("[gpu] normals")
+    for seed, p0 in ((42, 0), (7, 1000), (0x0123456789ABCDEF, 123456)):
+        g = hjm.generate_normals(seed, 48, 260, path_offset=p0)
+        o = oracle.normals(seed, 48, 260, path0=p0)
+        assert np.array_equal(g, o), seed
+
+
+def test_fp64_host_normals_matches_oracle(c1):
+    _flush("[gpu] config-1 FP64 host normals")
+    r = hjm.simulate_reference(c1["m"], c1["swap"], C1["paths"], normals=c1["z"],
+                               hazard=C1["hazard"], recovery=C1["recovery"], keep_paths=True)
+    D = 10
+    # forwards are O(1e-2) and may cross zero: absolute check; discount factors relative
+    assert np.max(np.abs(r.num_x[:, 1:D + 1] - c1["nx"][:, 1:D + 1])) < 1e-14
+    assert np.max(_rel(r.num_y[:, 1:D + 1], c1["ny"][:, 1:D + 1])) < 1e-12
+    scale = np.max(np.abs(c1["ex"]))
+    assert np.max(np.abs(r.exposures - c1["ex"])) < 1e-10 * scale
+    ee_o = c1["ee"]
+    assert np.all(r.expected_exposure[ee_o == 0] == 0)
+    assert np.max(_rel(r.expected_exposure[ee_o > 0], ee_o[ee_o > 0])) < 1e-10
+    assert np.all(r.expected_exposure[D:] == 0)
+    assert abs(r.cva / c1["cva"] - 1) < 1e-10
+    assert abs(r.cva_stderr / c1["se"] - 1) < 1e-8
+    pf = c1["pfe"]
+    assert np.max(np.abs(r.pfe - pf)) <= 1e-10 * max(1.0, np.max(pf))
+
+
+def test_fp64_in_kernel_rng_equals_host_normals(c1):
+    _flush("[gpu] config-1 FP64 in-kernel philox vs host normals")
+    a = hjm.simulate_reference(c1["m"], c1["swap"], C1["paths"], normals=c1["z"])
+    b = hjm.simulate_reference(c1["m"], c1["swap"], C1["paths"], seed=C1["seed"])
+    assert np.array_equal(a.expected_exposure, b.expected_exposure)
+    assert a.cva == b.cva and np.array_equal(a.pfe, b.pfe)
+
+
+def test_blockwise_baseline_matches_oracle(c1):
+    _flush("[gpu] config-1 blockwise baseline")
+    r = hjm.simulate_reference(c1["m"], c1["swap"], C1["paths"], normals=c1["z"], blockwise=True,
+                               keep_paths=True)
+    assert np.max(np.abs(r.num_x[:, 1:11] - c1["nx"][:, 1:11])) < 1e-15
+    assert np.max(_rel(r.num_y[:, 1:11], c1["ny"][:, 1:11])) < 1e-13
+    ee_o = c1["ee"]
+    assert np.max(_rel(r.expected_exposure[ee_o > 0], ee_o[ee_o > 0])) < 1e-12
+    r2 = hjm.simulate_reference(c1["m"], c1["swap"], C1["paths"], seed=C1["seed"], blockwise=True)
+    assert np.allclose(r2.expected_exposure, r.expected_exposure, rtol=1e-14, atol=0)
+
+
+def test_legacy_fixture_fp64_matches_oracle(oracle):
+    """The reference driver's own market data and swap (25y, N=10, K=6.7%), 2000 paths."""
+    _flush("[gpu] legacy fixture 25y FP64")
+    mk = json.load(open(os.path.join(HERE, "golden", "legacy_market.json")))
+    m = hjm.Market(mk["spot_rates"], mk["drifts"], mk["volatilities"], mk["accrual"])
+    P, S = 2000, 2500
+    z = oracle.normals(1234, P, S)
+    _, nx, ny = oracle.time_reference_paths(m.spot_rates, m.drifts, m.volatilities, 25.0, 0.5,
+                                            0.01, z, nthreads=8)
+    ex = oracle.reference_exposure(nx, ny, m.accrual, 10.0, 0.067, 0.5)
+    ee = oracle.expected_exposure(ex)
+    r = hjm.simulate_reference(m, hjm.InterestRateSwap(10.0, 0.067, 25.0), P, normals=z)
+    assert ee[50] == 0 and r.expected_exposure[50] == 0
+    assert np.max(_rel(r.expected_exposure[:50], ee[:50])) < 1e-10
+
+
+def test_fp32_compensated_error_vs_fp64(c1):
+    _flush("[gpu] config-1 FP32 compensated vs FP64")
+    a = hjm.simulate_reference(c1["m"], c1["swap"], C1["paths"], normals=c1["z"])
+    b = hjm.simulate_reference(c1["m"], c1["swap"], C1["paths"], normals=c1["z"],
+                               precision=hjm.FP32_KAHAN)
+    big = a.expected_exposure > 1e-3 * a.expected_exposure.max()
+    assert np.max(_rel(b.expected_exposure[big], a.expected_exposure[big])) < 1e-4
+    assert abs(b.cva / a.cva - 1) < 1e-4
+
+
+def test_dropin_sharding_invariance(c1):
+    _flush("[gpu] drop-in calculate_exposure_multi_gpu shards")
+    m = c1["m"]
+    swap = hjm.InterestRateSwap(1e6, 0.025, 5.0)
+    outs = []
+    for g in (1, 3, 4):
+        ee = np.zeros(51)
+        hjm.calculate_exposure_multi_gpu(ee, swap, m.accrual, m.spot_rates, m.drifts,
+                                         m.volatilities, 0.0, g, 10001, 0.01)
+        outs.append(ee)
+    for ee in outs[1:]:
+        assert np.max(_rel(ee[outs[0] > 0], outs[0][outs[0] > 0])) < 1e-13
+    ref = hjm.simulate_reference(m, swap, 10001, seed=1234)
+    assert np.max(_rel(outs[0][outs[0] > 0], ref.expected_exposure[outs[0] > 0])) < 1e-13
+    ee32 = np.zeros(51, np.float32)
+    hjm.calculate_exposure_multi_gpu(ee32, swap, m.accrual, m.spot_rates, m.drifts,
+                                     m.volatilities, 0.0, 2, 10001, 0.01)
+    big = outs[0] > 1e-3 * outs[0].max()
+    assert np.max(_rel(ee32[big].astype(np.float64), outs[0][big])) < 1e-4
+
+
+def test_cva_within_3_se_of_independent_oracle_run(oracle):
+    """In-kernel RNG: the GPU CVA and an oracle CVA on independent normals agree within
+    3 combined Monte Carlo standard errors (10y payer, 32768 GPU / 2048 oracle paths)."""
+    _flush("[gpu] CVA statistical check vs independent oracle run")
+    m = hjm.benchmark_market()
+    swap = hjm.InterestRateSwap(1e6, 0.025, 10.0)
+    g = hjm.simulate_reference(m, swap, 32768, seed=7)
+    z = oracle.normals(8, 2048, 1000)
+    _, nx, ny = oracle.time_reference_paths(m.spot_rates, m.drifts, m.volatilities, 10.0, 0.5,
+                                            0.01, z, nthreads=8)
+    ex = oracle.reference_exposure(nx, ny, m.accrual, 1e6, 0.025, 0.5)
+    _, _, cva, se = oracle.stats(ex, oracle.cva_weights(51, 0.5, 0.02, 0.4), 0.975)
+    assert abs(g.cva - cva) < 3.0 * math.sqrt(g.cva_stderr ** 2 + se ** 2)
+
+
+@pytest.fixture(scope="module")
+def full_size():
+    _flush("[gpu] precision config 131072 x 1000, FP64 and FP32")
+    m = hjm.benchmark_market()
+    swap = hjm.InterestRateSwap(1e6, 0.025, 10.0)
+    a = hjm.simulate_reference(m, swap, 131072, seed=7)
+    b = hjm.simulate_reference(m, swap, 131072, seed=7, precision=hjm.FP32_KAHAN)
+    return m, swap, a, b
+
+
+def test_full_size_properties(full_size):
+    m, swap, a, b = full_size
+    for r in (a, b):
+        assert np.all(r.expected_exposure >= 0) and np.all(r.expected_exposure[20:] == 0)
+        assert np.all(r.pfe[:20] >= r.expected_exposure[:20] - 1e-9)
+        assert r.cva > 0 and 0 < r.cva_stderr < 0.05 * r.cva
+    assert abs(b.cva / a.cva - 1) < 1e-4
+    big = a.expected_exposure > 1e-3 * a.expected_exposure.max()
+    assert np.max(_rel(b.expected_exposure[big], a.expected_exposure[big])) < 1e-4
+
+
+def test_full_size_deterministic_and_shard_additive(full_size):
+    m, swap, a, _ = full_size
+    a2 = hjm.simulate_reference(m, swap, 131072, seed=7)
+    assert np.array_equal(a.sums, a2.sums) and np.array_equal(a.pfe, a2.pfe)
+    h1 = hjm.simulate_reference(m, swap, 65536, seed=7, path_offset=0)
+    h2 = hjm.simulate_reference(m, swap, 65536, seed=7, path_offset=65536)
+    s = h1.sums + h2.sums
+    assert np.max(_rel(s[a.sums > 0], a.sums[a.sums > 0])) < 1e-12
+
+
+def test_pfe_is_exact_order_statistic(c1):
+    _flush("[gpu] PFE exact rank")
+    for P in (1000, 777, 1):
+        r = hjm.simulate_reference(c1["m"], c1["swap"], P, seed=3, keep_paths=True)
+        k = max(1, min(P, math.ceil(0.975 * P)))
+        for t in range(10):
+            col = r.exposures[:, t]
+            assert r.pfe[t] == np.partition(col, k - 1)[k - 1]
+
+
+def test_gpu_argument_errors(c1):
+    with pytest.raises(hjm.HJMError) as ei:
+        hjm.simulate_reference(c1["m"], c1["swap"], 64, device=99)
+    assert ei.value.code == 2
+    with pytest.raises(hjm.HJMError) as ei:
+        hjm.simulate_reference(c1["m"], hjm.InterestRateSwap(1e6, 0.025, 30.0), 64)
+    assert ei.value.code == 3
