@@ -4,6 +4,6 @@ The compute runs in libhjm_cva.so (hand-written HIP kernels for gfx950, C ABI in
 include/hjm_cva.h); this package is its Python host layer.
 """
 from .api import (FP32_KAHAN, FP64, RNG_HOST, RNG_PHILOX, TENORS, HJMError,  # noqa: F401
-                  InterestRateSwap, Market, ReferencePlan, ReferenceResult, benchmark_market,
-                  calculate_exposure_multi_gpu, cva_weights, device_count, generate_normals,
+                  Greeks, InterestRateSwap, Market, ReferencePlan, ReferenceResult, benchmark_market,
+                  calculate_exposure_multi_gpu, cva_greeks, cva_weights, device_count, generate_normals,
                   hjm_drift, library, simulate_reference)

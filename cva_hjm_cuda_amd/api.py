@@ -58,6 +58,11 @@ class _Config(C.Structure):
                 ("pfe_quantile", C.c_double), ("device", C.c_int32), ("reserved", C.c_int32)]
 
 
+class _Greeks(C.Structure):
+    _fields_ = [("cva", _dp), ("cva_stderr", _dp), ("d_spot", _dp), ("d_vol", _dp),
+                ("d_drift", _dp), ("d_vol_total", _dp), ("timings_ms", _dp)]
+
+
 class _Results(C.Structure):
     _fields_ = [("expected_exposure", _dp), ("pfe", _dp), ("cva", _dp), ("cva_stderr", _dp),
                 ("sums", _dp), ("num_x", _dp), ("num_y", _dp), ("exposures", _dp),
@@ -96,6 +101,8 @@ def _declare(lib):
     for name in ("hjm_simulate_reference", "hjm_simulate_reference_blockwise"):
         getattr(lib, name).argtypes = [C.POINTER(_Market), C.POINTER(_Swap), C.POINTER(_Config),
                                        C.POINTER(_Results)]
+    lib.hjm_cva_greeks_reference.argtypes = [C.POINTER(_Market), C.POINTER(_Swap),
+                                             C.POINTER(_Config), C.POINTER(_Greeks)]
     lib.hjm_calculate_exposure_multi_gpu_f64.argtypes = [_dp, d, d, d, d, _dp, _dp, _dp, _dp, d,
                                                          i, i, d]
     lib.hjm_calculate_exposure_multi_gpu_f32.argtypes = [fp, f, f, f, f, fp, fp, fp, fp, f, i, i,
@@ -238,6 +245,39 @@ def simulate_reference(market: Market, swap: InterestRateSwap, paths: int, dt: f
     fn = lib.hjm_simulate_reference_blockwise if blockwise else lib.hjm_simulate_reference
     _check(fn(C.byref(mk), C.byref(sw), C.byref(cfg), C.byref(res)), fn.__name__)
     return ReferenceResult(ee, pfe, float(cva[0]), float(se[0]), sums, tm, nx, ny, ex)
+
+
+@dataclass
+class Greeks:
+    cva: float
+    cva_stderr: float
+    d_spot: np.ndarray       # dCVA / d f(0, tau_j)                      [51]
+    d_vol: np.ndarray        # dCVA / d sigma_kj, drift table held fixed [153]
+    d_drift: np.ndarray      # dCVA / d m_j                              [51]
+    d_vol_total: np.ndarray  # d_vol + (dm/dsigma)^T d_drift (Simpson HJM drift)  [153]
+    timings_ms: np.ndarray   # forward, adjoint, total (device)
+
+
+def cva_greeks(market: Market, swap: InterestRateSwap, paths: int, dt: float = 0.01,
+               precision: int = FP32_KAHAN, seed: int = 1234,
+               normals: Optional[np.ndarray] = None, hazard: float = 0.02,
+               recovery: float = 0.4, device: int = 0, path_offset: int = 0) -> Greeks:
+    """CVA and its 51 + 153 (+ 51 drift) pathwise-adjoint sensitivities on the GPU."""
+    rng = RNG_PHILOX
+    if normals is not None:
+        normals = np.ascontiguousarray(normals, np.float64)
+        rng = RNG_HOST
+    mk = market._c()
+    sw = _Swap(swap.notional, swap.K, swap.expiry, swap.dtau)
+    cfg = _config(paths, dt, precision, rng, seed, normals, hazard, recovery, 0.0, device,
+                  path_offset)
+    cva, se = np.zeros(1), np.zeros(1)
+    ds, dv, dm, dvt, tm = np.zeros(TENORS), np.zeros(3 * TENORS), np.zeros(TENORS), \
+        np.zeros(3 * TENORS), np.zeros(3)
+    g = _Greeks(_ptr(cva), _ptr(se), _ptr(ds), _ptr(dv), _ptr(dm), _ptr(dvt), _ptr(tm))
+    _check(library().hjm_cva_greeks_reference(C.byref(mk), C.byref(sw), C.byref(cfg),
+                                              C.byref(g)), "hjm_cva_greeks_reference")
+    return Greeks(float(cva[0]), float(se[0]), ds, dv, dm, dvt, tm)
 
 
 class ReferencePlan:

@@ -12,7 +12,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libhjm_cva.so")
-SOURCES = ["hjm_capi.hip", "hjm_ref_kernels.hip", "hjm_baseline_kernels.hip"]
+SOURCES = ["hjm_capi.hip", "hjm_ref_kernels.hip", "hjm_baseline_kernels.hip",
+           "hjm_adjoint_kernels.hip"]
 ARCH = os.environ.get("HJM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

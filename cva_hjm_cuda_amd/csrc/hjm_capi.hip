@@ -139,6 +139,45 @@ void cva_weights(int ndates, double date_dt, double h, double R, double* w)
     }
 }
 
+// Cumulative Simpson integral of y on a uniform grid (the rule of hjm_hjm_drift).
+void cumulative_simpson(const double* y, double h, double* out)
+{
+    out[0] = 0.0;
+    for (int j = 1; j < kTenors; ++j) {
+        double s = 0.0;
+        if (j == 1) {
+            s = h / 12.0 * (5.0 * y[0] + 8.0 * y[1] - y[2]);
+        } else {
+            const int even_end = (j % 2 == 0) ? j : j - 3;
+            for (int i = 0; i < even_end; i += 2) s += h / 3.0 * (y[i] + 4.0 * y[i + 1] + y[i + 2]);
+            if (j % 2 == 1) s += 3.0 * h / 8.0 * (y[j - 3] + 3.0 * y[j - 2] + 3.0 * y[j - 1] + y[j]);
+        }
+        out[j] = s;
+    }
+}
+
+// out += (d m / d sigma)^T g for m_j = sum_k sigma_kj I_kj, I_k = W sigma_k:
+// d m_j / d sigma_kl = delta_jl I_kj + sigma_kj W_jl.
+void drift_vjp(const double* vols, double h, const double* g, double* out)
+{
+    double W[kTenors][kTenors];  // W[l][j] = weight of y_l in the integral to node j
+    for (int l = 0; l < kTenors; ++l) {
+        double e[kTenors] = {0.0};
+        e[l] = 1.0;
+        cumulative_simpson(e, h, W[l]);
+    }
+    for (int k = 0; k < kFactors; ++k) {
+        const double* sk = vols + k * kTenors;
+        double I[kTenors];
+        cumulative_simpson(sk, h, I);
+        for (int l = 0; l < kTenors; ++l) {
+            double a = g[l] * I[l];
+            for (int j = 0; j < kTenors; ++j) a += g[j] * sk[j] * W[l][j];
+            out[k * kTenors + l] += a;
+        }
+    }
+}
+
 }  // namespace
 
 struct hjm_plan_s {
@@ -449,6 +488,88 @@ int hjm_simulate_reference(const hjm_market* market, const hjm_swap* swap,
     const int rc = hjm_plan_execute(p, results);
     hjm_plan_destroy(p);
     return rc;
+}
+
+int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
+                             const hjm_config* cfg, hjm_greeks* gr)
+{
+    if (!gr) return HJM_ERR_NULL;
+    hjm_plan p = nullptr;
+    HJM_TRY(hjm_plan_create(&p, market, swap, cfg));
+    struct Cleanup {
+        hjm_plan p;
+        float *gx = nullptr, *gA = nullptr;
+        double *pa = nullptr, *pb = nullptr, *sa = nullptr, *sb = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Cleanup()
+        {
+            dev_free(gx);
+            dev_free(gA);
+            dev_free(pa);
+            dev_free(pb);
+            dev_free(sa);
+            dev_free(sb);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            hjm_plan_destroy(p);
+        }
+    } c{p};
+    DeviceGuard guard;
+    HJM_TRY(guard.set(p->device));
+    const int64_t P = p->paths, nb = p->nblocks;
+    const size_t cells = (size_t)kTenors * P;
+    HJM_TRY(dev_alloc(&c.gx, cells));
+    HJM_TRY(dev_alloc(&c.gA, cells));
+    HJM_TRY(dev_alloc(&c.pa, (size_t)nb * 2 * kTenors));
+    HJM_TRY(dev_alloc(&c.pb, (size_t)nb * 3 * kTenors));
+    HJM_TRY(dev_alloc(&c.sa, 2 * kTenors));
+    HJM_TRY(dev_alloc(&c.sb, 3 * kTenors));
+    HJM_HIP(hipEventCreate(&c.e0), HJM_ERR_DEVICE);
+    HJM_HIP(hipEventCreate(&c.e1), HJM_ERR_DEVICE);
+    hipStream_t st = p->stream;
+    HJM_TRY(plan_launch(p, false));
+    HJM_HIP(hipEventRecord(c.e0, st), HJM_ERR_DEVICE);
+    const double dt = p->grid.dt;
+    const double cc = dt / swap->dtau;
+    HJM_HIP(launch_ref_adjoint_seed(p->ep, dt, p->d_nx, p->d_ny, c.gx, c.gA, st), HJM_ERR_LAUNCH);
+    HJM_HIP(launch_ref_adjoint_sweep(p->grid, cc, sqrt(dt), 0, p->d_normals, c.gx, c.gA, c.pa, st),
+            HJM_ERR_LAUNCH);
+    HJM_HIP(launch_column_sum(c.pa, nb, 2 * kTenors, c.sa, st), HJM_ERR_LAUNCH);
+    HJM_HIP(launch_ref_adjoint_sweep(p->grid, cc, sqrt(dt), 1, p->d_normals, c.gx, c.gA, c.pb, st),
+            HJM_ERR_LAUNCH);
+    HJM_HIP(launch_column_sum(c.pb, nb, 3 * kTenors, c.sb, st), HJM_ERR_LAUNCH);
+    HJM_HIP(hipEventRecord(c.e1, st), HJM_ERR_DEVICE);
+    hjm_results r{};
+    r.cva = gr->cva;
+    r.cva_stderr = gr->cva_stderr;
+    HJM_TRY(plan_collect(p, &r));
+    double sa[2 * kTenors], sb[3 * kTenors];
+    HJM_HIP(hipMemcpy(sa, c.sa, sizeof(sa), hipMemcpyDeviceToHost), HJM_ERR_DEVICE);
+    HJM_HIP(hipMemcpy(sb, c.sb, sizeof(sb), hipMemcpyDeviceToHost), HJM_ERR_DEVICE);
+    const double inv = 1.0 / (double)P;
+    double dv[kFactors * kTenors], dm[kTenors];
+    for (int j = 0; j < kTenors; ++j) {
+        dv[j] = sa[j] * inv;
+        dv[kTenors + j] = sa[kTenors + j] * inv;
+        dv[2 * kTenors + j] = sb[j] * inv;
+        dm[j] = dt * sb[kTenors + j] * inv;
+        if (gr->d_spot) gr->d_spot[j] = sb[2 * kTenors + j] * inv;
+    }
+    if (gr->d_vol) memcpy(gr->d_vol, dv, sizeof(dv));
+    if (gr->d_drift) memcpy(gr->d_drift, dm, sizeof(dm));
+    if (gr->d_vol_total) {
+        memcpy(gr->d_vol_total, dv, sizeof(dv));
+        drift_vjp(market->volatilities, swap->dtau, dm, gr->d_vol_total);
+    }
+    if (gr->timings_ms) {
+        float fwd = 0.f, adj = 0.f;
+        HJM_HIP(hipEventElapsedTime(&fwd, p->ev[0], p->ev[4]), HJM_ERR_DEVICE);
+        HJM_HIP(hipEventElapsedTime(&adj, c.e0, c.e1), HJM_ERR_DEVICE);
+        gr->timings_ms[0] = fwd;
+        gr->timings_ms[1] = adj;
+        gr->timings_ms[2] = fwd + adj;
+    }
+    return HJM_OK;
 }
 
 int hjm_simulate_reference_blockwise(const hjm_market* market, const hjm_swap* swap,

@@ -27,6 +27,13 @@ hipError_t launch_radix_select(const double* x, int rows, int64_t n, int64_t k, 
 hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nsteps, double* out,
                           hipStream_t st);
 
+// Pathwise adjoint of the reference-semantics CVA.
+hipError_t launch_ref_adjoint_seed(const ExposureParams& e, double dt, const double* nx,
+                                   const double* ny, float* gx, float* gA, hipStream_t st);
+hipError_t launch_ref_adjoint_sweep(const RefGrid& rg, double c, double sqrt_dt, int pass,
+                                    const double* normals, const float* gx, const float* gA,
+                                    double* partials, hipStream_t st);
+
 // The reference's own GPU design (baseline form).
 hipError_t launch_blockwise_paths(const RefStepConsts<double>& k, const RefGrid& g, int block_index,
                                   const double* normals, const double* state_in, double* state_out,

@@ -9,6 +9,7 @@
 #include "hjm_common.h"
 #include "hjm_rng.h"
 #include "hjm_kernels.h"
+#include "hjm_valu.h"
 
 namespace hjm {
 
@@ -21,47 +22,6 @@ template <>
 __device__ __forceinline__ float fma_r<float>(float a, float b, float c)
 {
     return __builtin_fmaf(a, b, c);
-}
-
-// ---- explicit gfx950 VALU forms for the tenor update ------------------------------
-// The update of tenor j is exactly six VALU instructions on registers that stay in
-// place across steps (t is a temporary; c, 1-c in VGPRs; tenor constants in SGPRs):
-//   v_fma  t, c, f[j+1], ddt_j      v_fmac t, v0_j, s0     v_fmac t, v1_j, s1
-//   v_fmac t, v2_j, s2              v_fma  f[j], 1-c, f[j], t
-//   v_add  acc[j], acc[j], f[j]
-// Written out because the compiler otherwise picks the tied v_fmac form for the last
-// FMA, which needs a copy per tenor and, with 200+ live values, spills to AGPRs.
-__device__ __forceinline__ void op_fma_vvs(double& d, double a, double b, double c)
-{
-    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
-}
-__device__ __forceinline__ void op_fma_vvs(float& d, float a, float b, float c)
-{
-    asm("v_fma_f32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
-}
-__device__ __forceinline__ void op_fmac_sv(double& d, double a, double b)
-{
-    asm("v_fmac_f64 %0, %1, %2" : "+v"(d) : "s"(a), "v"(b));
-}
-__device__ __forceinline__ void op_fmac_sv(float& d, float a, float b)
-{
-    asm("v_fmac_f32 %0, %1, %2" : "+v"(d) : "s"(a), "v"(b));
-}
-__device__ __forceinline__ void op_fma_inplace(double& f, double a, double t)
-{
-    asm("v_fma_f64 %0, %1, %0, %2" : "+v"(f) : "v"(a), "v"(t));
-}
-__device__ __forceinline__ void op_fma_inplace(float& f, float a, float t)
-{
-    asm("v_fma_f32 %0, %1, %0, %2" : "+v"(f) : "v"(a), "v"(t));
-}
-__device__ __forceinline__ void op_add_inplace(double& a, double x)
-{
-    asm("v_add_f64 %0, %0, %1" : "+v"(a) : "v"(x));
-}
-__device__ __forceinline__ void op_add_inplace(float& a, float x)
-{
-    asm("v_add_f32 %0, %0, %1" : "+v"(a) : "v"(x));
 }
 
 template <typename T>

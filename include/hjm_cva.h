@@ -120,6 +120,23 @@ void hjm_plan_destroy(hjm_plan plan);
 int hjm_simulate_reference(const hjm_market* market, const hjm_swap* swap,
                            const hjm_config* config, hjm_results* results);
 
+/* CVA sensitivities by the pathwise adjoint (no counterpart in the reference; its
+ * roadmap names them, doc/TODO:6,14). Forward pass as hjm_simulate_reference (use
+ * HJM_FP32_KAHAN for the FP32 forward), then a backward sweep over the Euler steps per
+ * path, with FP32 adjoint state and FP64 accumulation of the Greeks across paths.     */
+typedef struct {
+    double* cva;          /* [1] */
+    double* cva_stderr;   /* [1] */
+    double* d_spot;       /* [51]  dCVA / d f(0, tau_j) */
+    double* d_vol;        /* [153] dCVA / d sigma_k(tau_j), drift table held fixed */
+    double* d_drift;      /* [51]  dCVA / d m(tau_j) */
+    double* d_vol_total;  /* [153] d_vol + (d m / d sigma)^T d_drift, m the cumulative-
+                             Simpson HJM drift of the loadings (hjm_hjm_drift) */
+    double* timings_ms;   /* [3] forward (all kernels), adjoint (all kernels), total */
+} hjm_greeks;
+int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
+                             const hjm_config* config, hjm_greeks* greeks);
+
 /* The reference's design on the GPU, kept as the "plain FP64 GPU form" baseline:
  * all normals materialised in HBM first, one workgroup per path with one tenor per
  * thread, curve in LDS, one launch per 50-step block (simulation_gpu.cu:471-669),

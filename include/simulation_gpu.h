@@ -10,7 +10,7 @@
  * keyed by seed 1234 instead of cuRAND; the remainder scenarios % num_gpus is simulated
  * instead of dropped; on failure the error is printed to stderr (as the reference does)
  * and the output is filled with NaN rather than left as whatever the device produced.
- * The status code of the last call is available from hjm_last_dropin_status().
+ * The status code of the last call on this thread is hjm_dropin::last_status().
  */
 #ifndef HJM_CVA_SIMULATION_GPU_H
 #define HJM_CVA_SIMULATION_GPU_H
@@ -38,8 +38,6 @@ inline void report(int status, real* out)
     }
 }
 }  // namespace hjm_dropin
-
-inline int hjm_last_dropin_status() { return hjm_dropin::last_status(); }
 
 inline void calculateExposureMultiGPU(double* expected_exposure, InterestRateSwap<double> payOff,
                                       double* accrual, double* spot_rates, double* drift,

@@ -424,6 +424,190 @@ void orc_stats(const double* exposures, int npaths, int ndates, const double* w,
 }
 
 /* ===================================================================================
+ * Pathwise adjoint of the per-path CVA sample J = sum_t w_t max(V_t, 0).
+ *
+ * Forward (per path): f(0) = spot, A(0) = stride * spot (frozen block), for evolution
+ * steps n = 1..N (N = (D-1) stride, normals of global step stride + n - 1):
+ *   f(n) = T f(n-1) + m dt + sqrt(dt) V phi(n),  A(n) = A(n-1) + f(n),
+ * with T = I + (dt/dtau) Dfwd. Slot i = 1..D records x_i = f_i(n_i), A_i(n_i),
+ * n_i = (i-1) stride; CF_i = e^{-A_i dt} N acc_i (m (e^{x_i/m} - 1) - K).
+ * Seeds: u_i = sum_{t<i} w_t 1[V_t > 0], gx_i = u_i e^{-A_i dt} N acc_i e^{x_i/m},
+ * gA_i = -u_i dt CF_i. Backward: mu(n) = mu(n+1) + sum_{n_i = n} gA_i e_i,
+ * lambda(n) = T^T lambda(n+1) + mu(n) + sum_{n_i = n} gx_i e_i;
+ * dJ/dV_kj = sum_n lambda_j(n) sqrt(dt) phi_k(n), dJ/dm_j = dt sum_n lambda_j(n),
+ * dJ/dspot = T^T lambda(1) + gx_1 e_1 + stride mu(0), mu(0) = mu(1) + gA_1 e_1.
+ * The step Jacobian T does not depend on the state, so no forward states are needed
+ * between dates.
+ * =================================================================================== */
+static void apply_TT(double* lam, double c)
+{
+    /* (T^T l)_j = (1-c) l_j + c l_{j-1} (1 <= j <= 48), j=0: (1-c) l_0,
+     * j=49: (1-c) l_49 + c l_48 - c l_50, j=50: (1+c) l_50 + c l_49. */
+    double out[ORC_TENORS];
+    out[0] = (1.0 - c) * lam[0];
+    for (int j = 1; j <= 48; ++j) out[j] = (1.0 - c) * lam[j] + c * lam[j - 1];
+    out[49] = (1.0 - c) * lam[49] + c * lam[48] - c * lam[50];
+    out[50] = (1.0 + c) * lam[50] + c * lam[49];
+    memcpy(lam, out, sizeof(out));
+}
+
+int orc_reference_cva_greeks(const double* spot, const double* drift, const double* vols,
+                             const double* accrual, double notional, double K, double expiry,
+                             double dtau, double dt, int npaths, const double* normals,
+                             const double* w, double* cva, double* d_spot, double* d_vol,
+                             double* d_drift)
+{
+    int stride, S, D;
+    int rc = ref_grid(expiry, dtau, dt, &stride, &S, &D);
+    if (rc) return rc;
+    if (npaths <= 0) return 2;
+    const int N = (D - 1) * stride;
+    const double c = dt / dtau, sq = sqrt(dt), m = 1.0 / dtau;
+    double gs[ORC_TENORS] = {0}, gv[3 * ORC_TENORS] = {0}, gm[ORC_TENORS] = {0}, J = 0.0;
+    for (int p = 0; p < npaths; ++p) {
+        const double* nrm = normals + (int64_t)p * S * 3;
+        double nx[ORC_TENORS], ny[ORC_TENORS];
+        ref_path_one(spot, drift, vols, dtau, dt, stride, S, D, nrm, nx, ny);
+        double cf[ORC_TENORS] = {0}, V[ORC_TENORS] = {0};
+        for (int i = 1; i <= D; ++i) {
+            double libor = m * (exp(nx[i] / m) - 1.0);
+            cf[i] = ny[i] * notional * accrual[i] * (libor - K);
+        }
+        double acc = 0.0;
+        for (int t = ORC_TENORS - 1; t >= 0; --t) {
+            if (t + 1 < ORC_TENORS) acc += cf[t + 1];
+            V[t] = acc;
+            if (acc > 0.0) J += w[t] * acc;
+        }
+        double gx[ORC_TENORS] = {0}, gA[ORC_TENORS] = {0};
+        double u = 0.0;
+        for (int i = 1; i <= D; ++i) {
+            if (V[i - 1] > 0.0) u += w[i - 1];
+            gx[i] = u * ny[i] * notional * accrual[i] * exp(nx[i] / m);
+            gA[i] = -u * dt * cf[i];
+        }
+        double lam[ORC_TENORS] = {0}, mu[ORC_TENORS] = {0};
+        for (int n = N; n >= 1; --n) {
+            if (n % stride == 0) {
+                int i = n / stride + 1;
+                lam[i] += gx[i];
+                mu[i] += gA[i];
+            }
+            const double* phi = nrm + (int64_t)(stride + n - 1) * 3;
+            for (int j = 0; j < ORC_TENORS; ++j) {
+                lam[j] += mu[j];
+                gv[j] += lam[j] * sq * phi[0];
+                gv[ORC_TENORS + j] += lam[j] * sq * phi[1];
+                gv[2 * ORC_TENORS + j] += lam[j] * sq * phi[2];
+                gm[j] += dt * lam[j];
+            }
+            apply_TT(lam, c);
+        }
+        lam[1] += gx[1];
+        mu[1] += gA[1];
+        for (int j = 0; j < ORC_TENORS; ++j) gs[j] += lam[j] + stride * mu[j];
+    }
+    const double inv = 1.0 / npaths;
+    if (cva) *cva = J * inv;
+    for (int j = 0; j < ORC_TENORS; ++j) {
+        if (d_spot) d_spot[j] = gs[j] * inv;
+        if (d_drift) d_drift[j] = gm[j] * inv;
+    }
+    if (d_vol)
+        for (int j = 0; j < 3 * ORC_TENORS; ++j) d_vol[j] = gv[j] * inv;
+    return 0;
+}
+
+void orc_drift_vjp(const double* vols, double h, const double* g, double* out)
+{
+    /* m_j = sum_k s_kj I_kj, I_k = W s_k (W: cumulative Simpson weights, linear).
+     * dm_j/ds_kl = delta_jl I_kj + s_kj W_jl. */
+    double e[ORC_TENORS], col[ORC_TENORS];
+    for (int k = 0; k < 3; ++k) {
+        const double* s = vols + k * ORC_TENORS;
+        double I[ORC_TENORS];
+        orc_cumulative_simpson(s, ORC_TENORS, h, I);
+        for (int l = 0; l < ORC_TENORS; ++l) {
+            memset(e, 0, sizeof(e));
+            e[l] = 1.0;
+            orc_cumulative_simpson(e, ORC_TENORS, h, col); /* col[j] = W_jl */
+            double acc = g[l] * I[l];
+            for (int j = 0; j < ORC_TENORS; ++j) acc += g[j] * s[j] * col[j];
+            out[k * ORC_TENORS + l] += acc;
+        }
+    }
+}
+
+/* ===================================================================================
+ * Portfolio semantics (declared in hjm_oracle.h).
+ * =================================================================================== */
+static void bond_prices(const double* f, double dtau, int kmax, double* P)
+{
+    double I = 0.0;
+    for (int k = 1; k <= kmax; ++k) {
+        if (k <= ORC_TENORS - 1) I += 0.5 * dtau * (f[k - 1] + f[k]);
+        else I += dtau * f[ORC_TENORS - 1];
+        P[k] = exp(-I);
+    }
+    P[0] = 1.0;
+}
+
+int orc_portfolio_exposure(const double* spot, const double* drift, const double* vols,
+                           double horizon, double dtau, double dt, int npaths,
+                           const double* normals, int nswaps, const int* periods,
+                           const double* sign_notional, const double* fixed_rate,
+                           const int* set_of, int nsets, double* exposures, double* curves,
+                           double* log_bank)
+{
+    if (!(dt > 0.0) || !(dtau > 0.0) || !(horizon > 0.0) || npaths <= 0 || nsets <= 0) return 2;
+    double rs = dtau / dt, rS = horizon / dt;
+    int st = (int)llround(rs), S = (int)llround(rS);
+    if (st <= 0 || fabs(rs - st) > 1e-6 * st || fabs(rS - S) > 1e-6 * S || S % st) return 3;
+    const int E = S / st;  /* last date index; dates 0..E */
+    for (int s = 0; s < nswaps; ++s)
+        if (periods[s] < 1 || periods[s] > E || set_of[s] < 0 || set_of[s] >= nsets) return 2;
+    const int nd = E + 1;
+    double* P = (double*)malloc(sizeof(double) * (E + 2));
+    double* V = (double*)malloc(sizeof(double) * nsets);
+    for (int p = 0; p < npaths; ++p) {
+        const double* nrm = normals + (int64_t)p * S * 3;
+        double cur[ORC_TENORS], nxt[ORC_TENORS];
+        memcpy(cur, spot, sizeof(cur));
+        double lnB = 0.0;
+        for (int n = 0; n <= S; ++n) {
+            if (n % st == 0) {
+                const int e = n / st;
+                if (curves) memcpy(curves + ((int64_t)p * nd + e) * ORC_TENORS, cur, sizeof(cur));
+                if (log_bank) log_bank[(int64_t)p * nd + e] = lnB;
+                bond_prices(cur, dtau, E - e, P);
+                for (int g = 0; g < nsets; ++g) V[g] = 0.0;
+                for (int s = 0; s < nswaps; ++s) {
+                    const int k = periods[s] - e;
+                    if (k <= 0) continue;
+                    double ann = 0.0;
+                    for (int j = 1; j <= k; ++j) ann += P[j];
+                    V[set_of[s]] += sign_notional[s] * (1.0 - P[k] - fixed_rate[s] * dtau * ann);
+                }
+                const double disc = exp(-lnB);
+                for (int g = 0; g < nsets; ++g)
+                    exposures[((int64_t)p * nsets + g) * nd + e] = (V[g] > 0.0 ? V[g] : 0.0) * disc;
+            }
+            if (n == S) break;
+            lnB += cur[0] * dt;
+            const double* phi = nrm + (int64_t)n * 3;
+            for (int t = 0; t < ORC_TENORS; ++t)
+                nxt[t] = musiela_sde(drift[t], vols[t], vols[ORC_TENORS + t],
+                                     vols[2 * ORC_TENORS + t], phi[0], phi[1], phi[2],
+                                     dfau(t, cur), cur[t], dtau, dt);
+            memcpy(cur, nxt, sizeof(cur));
+        }
+    }
+    free(P);
+    free(V);
+    return 0;
+}
+
+/* ===================================================================================
  * CPU baseline: the path loop of simulation_cpu.h:407-440 timed alone, on nthreads.
  * =================================================================================== */
 double orc_time_reference_paths(const double* spot, const double* drift, const double* vols,

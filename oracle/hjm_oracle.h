@@ -60,6 +60,36 @@ int64_t orc_quantile_rank(int64_t n, double q);
 void orc_stats(const double* exposures, int npaths, int ndates, const double* w, double q,
                double* ee, double* pfe, double* cva, double* cva_se);
 
+/* ---- pathwise adjoint (reference semantics) ---------------------------------------
+ * CVA = (1/P) sum_p sum_t w_t max(V_pt, 0) and its derivatives with respect to the
+ * initial curve (51), the loadings (153, drift table held fixed) and the drift table
+ * (51), by a backward sweep over the Euler steps in FP64. */
+int orc_reference_cva_greeks(const double* spot, const double* drift, const double* vols,
+                             const double* accrual, double notional, double K, double expiry,
+                             double dtau, double dt, int npaths, const double* normals,
+                             const double* w, double* cva, double* d_spot, double* d_vol,
+                             double* d_drift);
+/* Jacobian-transpose product of the cumulative-Simpson HJM drift: adds
+ * (d drift / d vols)^T g to out (153 entries). */
+void orc_drift_vjp(const double* vols, double h, const double* g, double* out);
+
+/* ---- portfolio semantics (standard revaluation; no counterpart in the reference) ----
+ * Dates t_e = e dtau, e = 0..horizon/dtau, from t = 0 without a frozen block. The curve
+ * follows the same Euler step; the numeraire is the bank account
+ * B(t_e) = exp(dt sum_{n < n_e} f_0(n)); bond prices P(t_e, t_e + k dtau) = exp(-I_k)
+ * with I_k the trapezoid integral of the simulated curve (flat beyond the last tenor);
+ * swap s (n_s periods of dtau, sign +1 payer / -1 receiver, notional N, fixed K):
+ *   V_s(t_e) = sign N (1 - P_{n_s - e} - K dtau sum_{k=1}^{n_s - e} P_k) for e < n_s;
+ * netting set value V_g = sum_{s in g} V_s; exposure E_ge = max(V_g, 0) / B(t_e).
+ * Outputs exposures [path][set][date] and optionally curves [path][date][51] and
+ * log bank account [path][date]. */
+int orc_portfolio_exposure(const double* spot, const double* drift, const double* vols,
+                           double horizon, double dtau, double dt, int npaths,
+                           const double* normals, int nswaps, const int* periods,
+                           const double* sign_notional, const double* fixed_rate,
+                           const int* set_of, int nsets, double* exposures, double* curves,
+                           double* log_bank);
+
 /* ---- timing helper for the CPU baseline ----------------------------------------- */
 double orc_time_reference_paths(const double* spot, const double* drift, const double* vols,
                                 double expiry, double dtau, double dt, int npaths,

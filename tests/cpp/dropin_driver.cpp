@@ -25,7 +25,7 @@ int main(int argc, char** argv)
     std::vector<double> ee(51);
     calculateExposureMultiGPU(ee.data(), payOff, acc.data(), spot.data(), drift.data(), vols.data(),
                               0.0, num_gpus, scenarios, 0.01);
-    if (hjm_last_dropin_status() != HJM_OK) return 1;
+    if (hjm_dropin::last_status() != HJM_OK) return 1;
 
     std::vector<float> spot_f(spot.begin(), spot.end()), drift_f(drift.begin(), drift.end()),
         vols_f(vols.begin(), vols.end()), acc_f(acc.begin(), acc.end()), sched_f(51), ee_f(51);
@@ -34,7 +34,7 @@ int main(int argc, char** argv)
                                      0.067f, (float)expiry, 0.5f);
     calculateExposureMultiGPU(ee_f.data(), payOff_f, acc_f.data(), spot_f.data(), drift_f.data(),
                               vols_f.data(), 0.0f, num_gpus, scenarios, 0.01f);
-    if (hjm_last_dropin_status() != HJM_OK) return 2;
+    if (hjm_dropin::last_status() != HJM_OK) return 2;
     for (int t = 0; t < 51; ++t) std::printf("%.17g %.9g\n", ee[t], (double)ee_f[t]);
     return 0;
 }

@@ -44,6 +44,16 @@ class Oracle:
         L.orc_quantile_rank.restype = C.c_int64
         L.orc_stats.argtypes = [_dp, C.c_int, C.c_int, _dp, C.c_double, _dp, _dp,
                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_reference_cva_greeks.argtypes = [_dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                                               C.c_double, C.c_double, C.c_double, C.c_int, _dp,
+                                               _dp, C.POINTER(C.c_double), _dp, _dp, _dp]
+        L.orc_reference_cva_greeks.restype = C.c_int
+        L.orc_drift_vjp.argtypes = [_dp, C.c_double, _dp, _dp]
+        _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        L.orc_portfolio_exposure.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, C.c_double,
+                                             C.c_int, _dp, C.c_int, _ip, _dp, _dp, _ip, C.c_int,
+                                             _dp, C.c_void_p, C.c_void_p]
+        L.orc_portfolio_exposure.restype = C.c_int
         L.orc_time_reference_paths.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, C.c_double,
                                                C.c_int, _dp, _dp, _dp, C.c_int]
         L.orc_time_reference_paths.restype = C.c_double
@@ -133,6 +143,45 @@ class Oracle:
         cva, se = C.c_double(), C.c_double()
         self.lib.orc_stats(_c(exposures), P, D, _c(w), q, ee, pfe, C.byref(cva), C.byref(se))
         return ee, pfe, cva.value, se.value
+
+    def reference_cva(self, spot, drift, vols, accrual, notional, K, expiry, dtau, dt, normals,
+                      w):
+        nx, ny = self.reference_paths(spot, drift, vols, expiry, dtau, dt, normals)
+        ex = self.reference_exposure(nx, ny, accrual, notional, K, dtau)
+        return float(np.mean(ex @ w))
+
+    def reference_cva_greeks(self, spot, drift, vols, accrual, notional, K, expiry, dtau, dt,
+                             normals, w):
+        P = normals.shape[0]
+        cva = C.c_double()
+        ds, dv, dm = np.zeros(TENORS), np.zeros(3 * TENORS), np.zeros(TENORS)
+        rc = self.lib.orc_reference_cva_greeks(_c(spot), _c(drift), _c(vols), _c(accrual),
+                                               notional, K, expiry, dtau, dt, P, _c(normals),
+                                               _c(w), C.byref(cva), ds, dv, dm)
+        if rc:
+            raise ValueError("orc_reference_cva_greeks rc=%d" % rc)
+        return cva.value, ds, dv, dm
+
+    def drift_vjp(self, vols, g, h=0.5):
+        out = np.zeros(3 * TENORS)
+        self.lib.orc_drift_vjp(_c(vols), h, _c(g), out)
+        return out
+
+    def portfolio_exposure(self, spot, drift, vols, horizon, dtau, dt, normals, pf,
+                           want_curves=False):
+        """Exposures [path][set][date] (+ curves [path][date][51], log bank [path][date])."""
+        P = normals.shape[0]
+        nd = int(round(horizon / dtau)) + 1
+        ex = np.zeros((P, pf.nsets, nd))
+        cv = np.zeros((P, nd, TENORS)) if want_curves else None
+        lb = np.zeros((P, nd)) if want_curves else None
+        rc = self.lib.orc_portfolio_exposure(
+            _c(spot), _c(drift), _c(vols), horizon, dtau, dt, P, _c(normals), pf.nswaps,
+            pf.periods, pf.sign_notional, pf.fixed_rate, pf.set_of, pf.nsets, ex,
+            cv.ctypes.data if cv is not None else None, lb.ctypes.data if lb is not None else None)
+        if rc:
+            raise ValueError("orc_portfolio_exposure rc=%d" % rc)
+        return (ex, cv, lb) if want_curves else ex
 
     def time_reference_paths(self, spot, drift, vols, expiry, dtau, dt, normals, nthreads=1):
         P = normals.shape[0]

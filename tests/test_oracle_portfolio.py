@@ -1,0 +1,111 @@
+"""Pins the oracle's portfolio (standard revaluation) semantics.
+
+* zero volatility and drift with a linear initial curve: the upwind step is exact, so
+  f(t, tau) = a + b (tau + t); the bank account, the trapezoid bond prices (exact for a
+  linear curve, flat beyond 25y) and every swap value are closed-form;
+* at t = 0 every path has the same exposure;
+* risk-neutral consistency of the HJM drift: E[P(t_e, T) / B(t_e)] equals the scheme's
+  zero-volatility value within Monte Carlo error (and does not without the drift).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cva_hjm_cuda_amd.portfolio import Portfolio, synthetic_portfolio
+
+T = 51
+
+
+def _linear_case_closed_form(a, b, dt, dtau, horizon, pf):
+    E = int(round(horizon / dtau))
+    ex = np.zeros((pf.nsets, E + 1))
+    for e in range(E + 1):
+        te = e * dtau
+        ne = int(round(te / dt))
+        lnB = a * te + b * dt * dt * ne * (ne - 1) / 2
+        P = np.ones(E + 2)
+        for k in range(1, E - e + 1):
+            if k <= 50:
+                I = (a + b * te) * (dtau * k) + b * (dtau * k) ** 2 / 2
+            else:
+                I = (a + b * te) * 25.0 + b * 25.0 ** 2 / 2 + (k - 50) * dtau * (a + b * (25 + te))
+            P[k] = math.exp(-I)
+        V = np.zeros(pf.nsets)
+        for s in range(pf.nswaps):
+            k = pf.periods[s] - e
+            if k > 0:
+                V[pf.set_of[s]] += pf.sign_notional[s] * (1 - P[k] - pf.fixed_rate[s] * dtau * P[1:k + 1].sum())
+        ex[:, e] = np.maximum(V, 0) * math.exp(-lnB)
+    return ex
+
+
+def test_linear_curve_zero_vol_closed_form(oracle):
+    a, b, dt, dtau, H = 0.02, 0.0004, 0.01, 0.5, 30.0
+    spot = a + b * np.arange(T) * dtau
+    pf = Portfolio(periods=[60, 8, 20, 60, 35], sign_notional=[1e6, -2e6, 5e5, -1e6, 3e6],
+                   fixed_rate=[0.03, 0.025, 0.01, 0.045, 0.02], set_of=[0, 0, 1, 1, 2],
+                   hazard=[0.01, 0.02, 0.03])
+    z = np.zeros((1, int(H / dt), 3))
+    ex = oracle.portfolio_exposure(spot, np.zeros(T), np.zeros(3 * T), H, dtau, dt, z, pf)[0]
+    ref = _linear_case_closed_form(a, b, dt, dtau, H, pf)
+    assert np.max(np.abs(ex - ref)) <= 1e-9 * np.max(np.abs(ref))
+    assert np.max(ref[:, :10]) > 0 and np.all(ref[:, -1] == 0)
+
+
+def test_date_zero_is_path_independent(oracle):
+    spot = oracle.nelson_siegel()
+    vols = oracle.poly_vols()
+    pf = synthetic_portfolio(24, 5, seed=1)
+    z = oracle.normals(3, 8, 600)
+    ex = oracle.portfolio_exposure(spot, oracle.hjm_drift(vols), vols, 6.0, 0.5, 0.01, z,
+                                   Portfolio(np.minimum(pf.periods, 12), pf.sign_notional,
+                                             pf.fixed_rate, pf.set_of, pf.hazard))
+    assert np.all(ex[:, :, 0] == ex[0:1, :, 0])
+
+
+def _bond(f, k):
+    I = 0.0
+    for l in range(k):
+        I += 0.25 * (f[..., l] + f[..., l + 1])
+    return np.exp(-I)
+
+
+def _discounted_bonds(oracle, drift, vols, P=4000, H=10.0):
+    spot = oracle.nelson_siegel()
+    pf = Portfolio([1], [0.0], [0.0], [0], [0.01])
+    z = oracle.normals(17, P, int(H / 0.01))
+    _, cv, lb = oracle.portfolio_exposure(spot, drift, vols, H, 0.5, 0.01, z, pf,
+                                          want_curves=True)
+    _, c0, l0 = oracle.portfolio_exposure(spot, np.zeros(T), np.zeros(3 * T), H, 0.5, 0.01,
+                                          z[:1], pf, want_curves=True)
+    out = []
+    for e, k in ((4, 6), (10, 10), (20, 20), (20, 30)):
+        d = _bond(cv[:, e, :], k) * np.exp(-lb[:, e])
+        ref = _bond(c0[0, e, :], k) * np.exp(-l0[0, e])
+        out.append((d.mean() - ref) / (d.std(ddof=1) / math.sqrt(P)))
+    return np.array(out)
+
+
+def test_discounted_bond_prices_are_martingales(oracle):
+    """E[P(t_e, T) / B(t_e)] equals the same scheme's zero-volatility value: the HJM
+    drift of the loadings removes the convexity of the stochastic part (the upwind
+    advection on the 0.5y grid biases both alike). Without the drift it does not."""
+    vols = oracle.poly_vols()
+    z = _discounted_bonds(oracle, oracle.hjm_drift(vols), vols)
+    assert np.max(np.abs(z)) < 3.5, z
+    z0 = _discounted_bonds(oracle, np.zeros(T), vols)
+    assert np.max(np.abs(z0)) > 6.0, z0
+
+
+def test_synthetic_portfolio_generator():
+    pf = synthetic_portfolio(1024, 128, seed=2024)
+    assert pf.nswaps == 1024 and pf.nsets == 128
+    assert set(np.unique(pf.set_of)) == set(range(128))
+    assert pf.periods.min() >= 4 and pf.periods.max() <= 60 and np.all(pf.periods % 2 == 0)
+    assert np.all((pf.fixed_rate >= 0.01) & (pf.fixed_rate <= 0.04))
+    assert np.all((pf.hazard >= 0.005) & (pf.hazard <= 0.05))
+    sizes = np.bincount(pf.set_of)
+    assert sizes.max() > 3 * sizes.min()
+    again = synthetic_portfolio(1024, 128, seed=2024)
+    assert np.array_equal(pf.sign_notional, again.sign_notional)
