@@ -6,4 +6,6 @@ include/hjm_cva.h); this package is its Python host layer.
 from .api import (FP32_KAHAN, FP64, RNG_HOST, RNG_PHILOX, TENORS, HJMError,  # noqa: F401
                   Greeks, InterestRateSwap, Market, ReferencePlan, ReferenceResult, benchmark_market,
                   calculate_exposure_multi_gpu, cva_greeks, cva_weights, device_count, generate_normals,
-                  hjm_drift, library, simulate_reference)
+                  hjm_drift, library, simulate_reference, PortfolioPlan, PortfolioResult,
+                  Tuning, simulate_portfolio, autotune_portfolio)
+from .portfolio import Portfolio, synthetic_portfolio  # noqa: F401

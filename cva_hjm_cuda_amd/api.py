@@ -63,6 +63,23 @@ class _Greeks(C.Structure):
                 ("d_drift", _dp), ("d_vol_total", _dp), ("timings_ms", _dp)]
 
 
+class _Portfolio(C.Structure):
+    _fields_ = [("nswaps", C.c_int32), ("nsets", C.c_int32), ("periods", C.POINTER(C.c_int32)),
+                ("sign_notional", _dp), ("fixed_rate", _dp),
+                ("netting_set", C.POINTER(C.c_int32)), ("hazard_rate", _dp),
+                ("recovery", C.c_double), ("horizon", C.c_double), ("dtau", C.c_double)]
+
+
+class _Tuning(C.Structure):
+    _fields_ = [("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("use_graph", C.c_int32), ("reserved", C.c_int32)]
+
+
+class _PfResults(C.Structure):
+    _fields_ = [("expected_exposure", _dp), ("pfe", _dp), ("cva", _dp), ("cva_stderr", _dp),
+                ("exposures", _dp), ("timings_ms", _dp)]
+
+
 class _Results(C.Structure):
     _fields_ = [("expected_exposure", _dp), ("pfe", _dp), ("cva", _dp), ("cva_stderr", _dp),
                 ("sums", _dp), ("num_x", _dp), ("num_y", _dp), ("exposures", _dp),
@@ -103,6 +120,18 @@ def _declare(lib):
                                        C.POINTER(_Results)]
     lib.hjm_cva_greeks_reference.argtypes = [C.POINTER(_Market), C.POINTER(_Swap),
                                              C.POINTER(_Config), C.POINTER(_Greeks)]
+    lib.hjm_portfolio_plan_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(_Market),
+                                              C.POINTER(_Portfolio), C.POINTER(_Config),
+                                              C.POINTER(_Tuning)]
+    lib.hjm_portfolio_plan_execute.argtypes = [C.c_void_p, C.POINTER(_PfResults)]
+    lib.hjm_portfolio_plan_destroy.argtypes = [C.c_void_p]
+    lib.hjm_portfolio_plan_destroy.restype = None
+    lib.hjm_simulate_portfolio.argtypes = [C.POINTER(_Market), C.POINTER(_Portfolio),
+                                           C.POINTER(_Config), C.POINTER(_Tuning),
+                                           C.POINTER(_PfResults)]
+    lib.hjm_portfolio_autotune.argtypes = [C.POINTER(_Market), C.POINTER(_Portfolio),
+                                           C.POINTER(_Config), C.POINTER(_Tuning),
+                                           C.POINTER(C.c_double)]
     lib.hjm_calculate_exposure_multi_gpu_f64.argtypes = [_dp, d, d, d, d, _dp, _dp, _dp, _dp, d,
                                                          i, i, d]
     lib.hjm_calculate_exposure_multi_gpu_f32.argtypes = [fp, f, f, f, f, fp, fp, fp, fp, f, i, i,
@@ -356,3 +385,105 @@ def calculate_exposure_multi_gpu(expected_exposure: np.ndarray, payOff: Interest
     else:
         raise TypeError("expected_exposure must be float64 or float32")
     _check(code, "calculateExposureMultiGPU")
+
+
+# ---- portfolio semantics ------------------------------------------------------------
+@dataclass
+class Tuning:
+    block_size: int = 256
+    blocks_per_cu: int = 0
+    use_graph: bool = False
+
+    def _c(self):
+        return _Tuning(self.block_size, self.blocks_per_cu, int(self.use_graph), 0)
+
+
+@dataclass
+class PortfolioResult:
+    expected_exposure: np.ndarray   # [nsets][ndates]
+    pfe: np.ndarray                 # [nsets][ndates]
+    cva: np.ndarray                 # [nsets]
+    cva_stderr: np.ndarray          # [nsets]
+    timings_ms: np.ndarray          # paths, valuation, statistics, PFE, total
+    exposures: Optional[np.ndarray] = None  # [paths][nsets][ndates]
+
+
+def _pf_c(pf, horizon, recovery):
+    return _Portfolio(pf.nswaps, pf.nsets, pf.periods.ctypes.data_as(C.POINTER(C.c_int32)),
+                      _ptr(pf.sign_notional), _ptr(pf.fixed_rate),
+                      pf.set_of.ctypes.data_as(C.POINTER(C.c_int32)), _ptr(pf.hazard),
+                      float(recovery), float(horizon), float(pf.dtau))
+
+
+class PortfolioPlan:
+    """Device buffers, coefficient tables and (optionally) a captured hipGraph for
+    repeated runs of one portfolio configuration."""
+
+    def __init__(self, market: Market, portfolio, paths: int, horizon: float = 30.0,
+                 dt: float = 0.01, precision: int = FP32_KAHAN, seed: int = 2024,
+                 normals: Optional[np.ndarray] = None, recovery: float = 0.4,
+                 pfe_quantile: float = 0.975, tuning: Optional[Tuning] = None, device: int = 0,
+                 path_offset: int = 0):
+        self._lib = library()
+        self._keep = (market, portfolio, normals)
+        rng = RNG_PHILOX
+        if normals is not None:
+            normals = np.ascontiguousarray(normals, np.float64)
+            self._keep = (market, portfolio, normals)
+            rng = RNG_HOST
+        mk = market._c()
+        pfc = _pf_c(portfolio, horizon, recovery)
+        cfg = _config(paths, dt, precision, rng, seed, normals, 0.0, recovery, pfe_quantile,
+                      device, path_offset)
+        tun = tuning._c() if tuning is not None else None
+        self._h = C.c_void_p()
+        _check(self._lib.hjm_portfolio_plan_create(C.byref(self._h), C.byref(mk), C.byref(pfc),
+                                                   C.byref(cfg),
+                                                   C.byref(tun) if tun is not None else None),
+               "hjm_portfolio_plan_create")
+        self.paths, self.nsets = paths, portfolio.nsets
+        self.ndates = int(round(horizon / portfolio.dtau)) + 1
+
+    def execute(self, keep_paths: bool = False) -> PortfolioResult:
+        G, D = self.nsets, self.ndates
+        ee, pfe = np.zeros((G, D)), np.zeros((G, D))
+        cva, se, tm = np.zeros(G), np.zeros(G), np.zeros(5)
+        ex = np.zeros((self.paths, G, D)) if keep_paths else None
+        r = _PfResults(_ptr(ee), _ptr(pfe), _ptr(cva), _ptr(se), _ptr(ex), _ptr(tm))
+        _check(self._lib.hjm_portfolio_plan_execute(self._h, C.byref(r)),
+               "hjm_portfolio_plan_execute")
+        return PortfolioResult(ee, pfe, cva, se, tm, ex)
+
+    def close(self):
+        if self._h:
+            self._lib.hjm_portfolio_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def simulate_portfolio(market: Market, portfolio, paths: int, keep_paths: bool = False,
+                       **kw) -> PortfolioResult:
+    plan = PortfolioPlan(market, portfolio, paths, **kw)
+    try:
+        return plan.execute(keep_paths)
+    finally:
+        plan.close()
+
+
+def autotune_portfolio(market: Market, portfolio, paths: int, horizon: float = 30.0,
+                       dt: float = 0.01, precision: int = FP32_KAHAN, seed: int = 2024,
+                       device: int = 0):
+    """Fastest (block size, persistent grid) for the path kernel; returns (Tuning, ms)."""
+    mk = market._c()
+    pfc = _pf_c(portfolio, horizon, 0.4)
+    cfg = _config(paths, dt, precision, RNG_PHILOX, seed, None, 0.0, 0.4, 0.0, device, 0)
+    t = _Tuning()
+    ms = C.c_double()
+    _check(library().hjm_portfolio_autotune(C.byref(mk), C.byref(pfc), C.byref(cfg), C.byref(t),
+                                            C.byref(ms)), "hjm_portfolio_autotune")
+    return Tuning(t.block_size, t.blocks_per_cu, bool(t.use_graph)), ms.value

@@ -13,59 +13,12 @@
 
 #include "hjm_common.h"
 #include "hjm_kernels.h"
+#include "hjm_host.h"
 
 using namespace hjm;
-
-#define HJM_TRY(expr)                            \
-    do {                                         \
-        const int _rc = (expr);                  \
-        if (_rc != HJM_OK) return _rc;           \
-    } while (0)
-
-#define HJM_HIP(expr, code)                      \
-    do {                                         \
-        if ((expr) != hipSuccess) return (code); \
-    } while (0)
+using namespace hjm_host;
 
 namespace {
-
-// Restores the caller's current device on scope exit.
-struct DeviceGuard {
-    int prev = -1;
-    int set(int dev)
-    {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        int n = 0;
-        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return HJM_ERR_DEVICE;
-        if (dev < 0 || dev >= n) return HJM_ERR_ARG;
-        if (hipSetDevice(dev) != hipSuccess) return HJM_ERR_DEVICE;
-        return HJM_OK;
-    }
-    ~DeviceGuard()
-    {
-        if (prev >= 0) (void)hipSetDevice(prev);
-    }
-};
-
-template <typename T>
-int dev_alloc(T** p, size_t count)
-{
-    *p = nullptr;
-    if (count == 0) return HJM_OK;
-    if (hipMalloc((void**)p, count * sizeof(T)) != hipSuccess) {
-        *p = nullptr;
-        (void)hipGetLastError();
-        return HJM_ERR_ALLOC;
-    }
-    return HJM_OK;
-}
-
-template <typename T>
-void dev_free(T*& p)
-{
-    if (p) (void)hipFree(p);
-    p = nullptr;
-}
 
 // Time grid. The reference truncates: stride = (int)(dtau/dt) (simulation_cpu.h:200,
 // simulation_gpu.cu:494), steps = (int)(expiry/dt) (:959). Here both ratios are rounded
@@ -88,55 +41,6 @@ int ref_grid(double expiry, double dtau, double dt, int* stride, int* nsteps, in
     *nsteps = S;
     *ndates = D;
     return HJM_OK;
-}
-
-int check_market(const hjm_market* m)
-{
-    if (!m || !m->spot_rates || !m->drifts || !m->volatilities || !m->accrual) return HJM_ERR_NULL;
-    for (int j = 0; j < kTenors; ++j) {
-        if (!isfinite(m->spot_rates[j]) || !isfinite(m->drifts[j]) || !isfinite(m->accrual[j]))
-            return HJM_ERR_ARG;
-        for (int k = 0; k < kFactors; ++k)
-            if (!isfinite(m->volatilities[k * kTenors + j])) return HJM_ERR_ARG;
-    }
-    return HJM_OK;
-}
-
-template <typename Real>
-void fill_consts(RefStepConsts<Real>& k, const hjm_market* m, double dt, double dtau)
-{
-    for (int j = 0; j < kTenors; ++j) {
-        k.ddt[j] = (Real)(m->drifts[j] * dt);
-        k.v0[j] = (Real)m->volatilities[j];
-        k.v1[j] = (Real)m->volatilities[kTenors + j];
-        k.v2[j] = (Real)m->volatilities[2 * kTenors + j];
-        k.spot[j] = (Real)m->spot_rates[j];
-        k.drift[j] = (Real)m->drifts[j];
-    }
-    const double c = dt / dtau;
-    k.c = (Real)c;
-    k.omc = (Real)(1.0 - c);
-    k.opc = (Real)(1.0 + c);
-    k.sqrt_dt = (Real)sqrt(dt);
-    k.dtau = (Real)dtau;
-}
-
-int64_t quantile_rank(int64_t n, double q)
-{
-    int64_t k = (int64_t)ceil(q * (double)n);
-    if (k < 1) k = 1;
-    if (k > n) k = n;
-    return k;
-}
-
-void cva_weights(int ndates, double date_dt, double h, double R, double* w)
-{
-    for (int i = 0; i < ndates; ++i) w[i] = 0.0;
-    for (int i = 1; i < ndates; ++i) {
-        const double pd = exp(-h * (i - 1) * date_dt) - exp(-h * i * date_dt);
-        w[i - 1] += 0.5 * (1.0 - R) * pd;
-        w[i] += 0.5 * (1.0 - R) * pd;
-    }
 }
 
 // Cumulative Simpson integral of y on a uniform grid (the rule of hjm_hjm_drift).

@@ -1,0 +1,442 @@
+// hjm_capi_portfolio.hip -- C ABI of the portfolio semantics: coefficient tables,
+// persistent-grid tuning, hipGraph capture and replay, autotuner.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "hjm_host.h"
+#include "hjm_kernels.h"
+
+using namespace hjm;
+using namespace hjm_host;
+
+struct hjm_pplan_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[5] = {};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hjm_tuning tuning{256, 0, 0, 0};
+    int precision = HJM_FP64;
+    bool do_pfe = false;
+    int64_t paths = 0;
+    int nsets = 0, ndates = 0;
+    int64_t pfe_rank = 1;
+    int grid_blocks = 0;
+    RefStepConsts<double> kd{};
+    RefStepConsts<float> kf{};
+    PfGrid grid{};
+    PfValueParams vp{};
+    std::vector<double> w;  // [nsets][ndates] CVA weights
+    double* d_normals = nullptr;
+    void* d_curves = nullptr;
+    double* d_logb = nullptr;
+    void* d_ex = nullptr;
+    double* d_partials = nullptr;
+    double* d_sums = nullptr;
+    double* d_pfe = nullptr;
+    void* d_coef = nullptr;
+    void* d_a = nullptr;
+    int* d_kmax = nullptr;
+    double* d_w = nullptr;
+};
+
+namespace {
+
+void pplan_release(hjm_portfolio_plan p)
+{
+    if (!p) return;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(p->device);
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    dev_free(p->d_normals);
+    dev_free(p->d_curves);
+    dev_free(p->d_logb);
+    dev_free(p->d_ex);
+    dev_free(p->d_partials);
+    dev_free(p->d_sums);
+    dev_free(p->d_pfe);
+    dev_free(p->d_coef);
+    dev_free(p->d_a);
+    dev_free(p->d_kmax);
+    dev_free(p->d_w);
+    for (auto& e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    if (prev >= 0) (void)hipSetDevice(prev);
+    delete p;
+}
+
+int check_portfolio(const hjm_portfolio* pf, int* ndates)
+{
+    if (!pf || !pf->periods || !pf->sign_notional || !pf->fixed_rate || !pf->netting_set ||
+        !pf->hazard_rate)
+        return HJM_ERR_NULL;
+    if (pf->nswaps <= 0 || pf->nswaps > HJM_MAX_SWAPS || pf->nsets <= 0 || pf->nsets > HJM_MAX_SETS)
+        return HJM_ERR_ARG;
+    if (!(pf->dtau > 0.0) || !(pf->horizon > 0.0) || !(pf->recovery >= 0.0 && pf->recovery <= 1.0))
+        return HJM_ERR_ARG;
+    const double r = pf->horizon / pf->dtau;
+    const int E = (int)llround(r);
+    if (E < 1 || fabs(r - E) > 1e-9 * E || E > kMaxPeriods) return HJM_ERR_GRID;
+    for (int s = 0; s < pf->nswaps; ++s) {
+        if (pf->periods[s] < 1 || pf->periods[s] > E) return HJM_ERR_ARG;
+        if (pf->netting_set[s] < 0 || pf->netting_set[s] >= pf->nsets) return HJM_ERR_ARG;
+        if (!isfinite(pf->sign_notional[s]) || !isfinite(pf->fixed_rate[s])) return HJM_ERR_ARG;
+    }
+    for (int g = 0; g < pf->nsets; ++g)
+        if (!(pf->hazard_rate[g] >= 0.0)) return HJM_ERR_ARG;
+    *ndates = E + 1;
+    return HJM_OK;
+}
+
+// V_g(t_e) = a_ge - sum_k gamma_gek P_k with, for every live swap (k_s = n_s - e > 0):
+// a += sN, gamma_{k_s} += sN (the -sN P_{n} term), gamma_j += sN K dtau for j <= k_s.
+void build_tables(const hjm_portfolio* pf, int ndates, std::vector<double>& coef,
+                  std::vector<double>& a, std::vector<int>& kmax)
+{
+    const int G = pf->nsets;
+    coef.assign((size_t)ndates * G * kMaxPeriods, 0.0);
+    a.assign((size_t)ndates * G, 0.0);
+    kmax.assign((size_t)ndates * G, 0);
+    for (int e = 0; e < ndates; ++e) {
+        for (int s = 0; s < pf->nswaps; ++s) {
+            const int k = pf->periods[s] - e;
+            if (k <= 0) continue;
+            const int g = pf->netting_set[s];
+            const size_t row = (size_t)e * G + g;
+            const double sn = pf->sign_notional[s];
+            a[row] += sn;
+            double* gam = &coef[row * kMaxPeriods];
+            gam[k - 1] += sn;
+            const double fx = sn * pf->fixed_rate[s] * pf->dtau;
+            for (int j = 1; j <= k; ++j) gam[j - 1] += fx;
+            kmax[row] = std::max(kmax[row], k);
+        }
+    }
+}
+
+int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfolio* pf,
+               const hjm_config* cfg, const hjm_tuning* tuning)
+{
+    HJM_TRY(check_market(market));
+    if (!cfg) return HJM_ERR_NULL;
+    int nd = 0;
+    HJM_TRY(check_portfolio(pf, &nd));
+    if (cfg->paths <= 0 || cfg->path_offset < 0 || cfg->path_offset + cfg->paths > (1ll << 32))
+        return HJM_ERR_ARG;
+    if (cfg->precision != HJM_FP64 && cfg->precision != HJM_FP32_KAHAN) return HJM_ERR_ARG;
+    if (cfg->rng != HJM_RNG_PHILOX && cfg->rng != HJM_RNG_HOST) return HJM_ERR_ARG;
+    if (cfg->rng == HJM_RNG_HOST && !cfg->host_normals) return HJM_ERR_NULL;
+    if (cfg->pfe_quantile > 1.0) return HJM_ERR_ARG;
+    if (tuning) {
+        if (tuning->block_size != 64 && tuning->block_size != 128 && tuning->block_size != 256)
+            return HJM_ERR_ARG;
+        if (tuning->blocks_per_cu < 0 || tuning->blocks_per_cu > 64) return HJM_ERR_ARG;
+        p->tuning = *tuning;
+    }
+    if (!(cfg->dt > 0.0)) return HJM_ERR_ARG;
+    const double rs = pf->dtau / cfg->dt;
+    const int stride = (int)llround(rs);
+    if (stride <= 0 || fabs(rs - stride) > 1e-6 * stride) return HJM_ERR_GRID;
+    const int S = stride * (nd - 1);
+
+    p->device = cfg->device;
+    p->precision = cfg->precision;
+    p->paths = cfg->paths;
+    p->nsets = pf->nsets;
+    p->ndates = nd;
+    p->do_pfe = cfg->pfe_quantile > 0.0;
+    p->pfe_rank = quantile_rank(cfg->paths, cfg->pfe_quantile);
+    fill_consts(p->kd, market, cfg->dt, pf->dtau);
+    fill_consts(p->kf, market, cfg->dt, pf->dtau);
+    PfGrid& g = p->grid;
+    g.stride = stride;
+    g.nsteps = S;
+    g.ndates_m1 = nd - 1;
+    g.paths = cfg->paths;
+    g.path0 = cfg->path_offset;
+    g.seed_lo = (uint32_t)cfg->seed;
+    g.seed_hi = (uint32_t)(cfg->seed >> 32);
+    g.dt = cfg->dt;
+    p->vp.paths = cfg->paths;
+    p->vp.nsets = pf->nsets;
+    p->vp.ndates = nd;
+    p->vp.dtau = pf->dtau;
+    p->w.assign((size_t)pf->nsets * nd, 0.0);
+    for (int s = 0; s < pf->nsets; ++s)
+        cva_weights(nd, pf->dtau, pf->hazard_rate[s], pf->recovery, &p->w[(size_t)s * nd]);
+    std::vector<double> coef, a;
+    std::vector<int> kmax;
+    build_tables(pf, nd, coef, a, kmax);
+
+    DeviceGuard guard;
+    HJM_TRY(guard.set(cfg->device));
+    int cus = 0;
+    HJM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, cfg->device),
+            HJM_ERR_DEVICE);
+    const int64_t tiles = (cfg->paths + p->tuning.block_size - 1) / p->tuning.block_size;
+    p->grid_blocks = (int)std::min<int64_t>(
+        tiles, p->tuning.blocks_per_cu > 0 ? (int64_t)p->tuning.blocks_per_cu * cus : tiles);
+    HJM_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), HJM_ERR_DEVICE);
+    for (auto& ev : p->ev) HJM_HIP(hipEventCreate(&ev), HJM_ERR_DEVICE);
+    const size_t rsz = (p->precision == HJM_FP64) ? sizeof(double) : sizeof(float);
+    const size_t P = (size_t)cfg->paths;
+    const size_t nb = (P + kBlock - 1) / kBlock;
+    char* raw = nullptr;
+    HJM_TRY(dev_alloc(&raw, (size_t)nd * kTenors * P * rsz));
+    p->d_curves = raw;
+    HJM_TRY(dev_alloc(&p->d_logb, (size_t)nd * P));
+    raw = nullptr;
+    HJM_TRY(dev_alloc(&raw, (size_t)pf->nsets * nd * P * rsz));
+    p->d_ex = raw;
+    HJM_TRY(dev_alloc(&p->d_partials, (size_t)pf->nsets * nb * (nd + 2)));
+    HJM_TRY(dev_alloc(&p->d_sums, (size_t)pf->nsets * (nd + 2)));
+    HJM_TRY(dev_alloc(&p->d_pfe, (size_t)pf->nsets * nd));
+    HJM_TRY(dev_alloc(&p->d_kmax, kmax.size()));
+    HJM_TRY(dev_alloc(&p->d_w, p->w.size()));
+    HJM_HIP(hipMemcpy(p->d_kmax, kmax.data(), kmax.size() * sizeof(int), hipMemcpyHostToDevice),
+            HJM_ERR_DEVICE);
+    HJM_HIP(hipMemcpy(p->d_w, p->w.data(), p->w.size() * sizeof(double), hipMemcpyHostToDevice),
+            HJM_ERR_DEVICE);
+    if (p->precision == HJM_FP64) {
+        double *c = nullptr, *aa = nullptr;
+        HJM_TRY(dev_alloc(&c, coef.size()));
+        p->d_coef = c;
+        HJM_TRY(dev_alloc(&aa, a.size()));
+        p->d_a = aa;
+        HJM_HIP(hipMemcpy(c, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice),
+                HJM_ERR_DEVICE);
+        HJM_HIP(hipMemcpy(aa, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice),
+                HJM_ERR_DEVICE);
+    } else {
+        std::vector<float> cf(coef.begin(), coef.end()), af(a.begin(), a.end());
+        float *c = nullptr, *aa = nullptr;
+        HJM_TRY(dev_alloc(&c, cf.size()));
+        p->d_coef = c;
+        HJM_TRY(dev_alloc(&aa, af.size()));
+        p->d_a = aa;
+        HJM_HIP(hipMemcpy(c, cf.data(), cf.size() * sizeof(float), hipMemcpyHostToDevice),
+                HJM_ERR_DEVICE);
+        HJM_HIP(hipMemcpy(aa, af.data(), af.size() * sizeof(float), hipMemcpyHostToDevice),
+                HJM_ERR_DEVICE);
+    }
+    if (cfg->rng == HJM_RNG_HOST) {
+        const size_t n = P * (size_t)S * 3;
+        HJM_TRY(dev_alloc(&p->d_normals, n));
+        HJM_HIP(hipMemcpy(p->d_normals, cfg->host_normals, n * sizeof(double), hipMemcpyHostToDevice),
+                HJM_ERR_DEVICE);
+    }
+    return HJM_OK;
+}
+
+// Enqueue the four passes (with timing events) on the plan's stream.
+int pplan_enqueue(hjm_portfolio_plan p)
+{
+    hipStream_t st = p->stream;
+    const bool f64 = p->precision == HJM_FP64;
+    const int rows = p->nsets * p->ndates;
+    HJM_HIP(hipEventRecord(p->ev[0], st), HJM_ERR_DEVICE);
+    hipError_t err = f64 ? launch_pf_paths_f64(p->kd, p->grid, p->d_normals, (double*)p->d_curves,
+                                               p->d_logb, p->tuning.block_size, p->grid_blocks, st)
+                         : launch_pf_paths_f32(p->kf, p->grid, p->d_normals, (float*)p->d_curves,
+                                               p->d_logb, p->tuning.block_size, p->grid_blocks, st);
+    HJM_HIP(err, HJM_ERR_LAUNCH);
+    HJM_HIP(hipEventRecord(p->ev[1], st), HJM_ERR_DEVICE);
+    err = f64 ? launch_pf_value_f64(p->vp, (const double*)p->d_curves, p->d_logb,
+                                    (const double*)p->d_coef, (const double*)p->d_a, p->d_kmax,
+                                    (double*)p->d_ex, st)
+              : launch_pf_value_f32(p->vp, (const float*)p->d_curves, p->d_logb,
+                                    (const float*)p->d_coef, (const float*)p->d_a, p->d_kmax,
+                                    (float*)p->d_ex, st);
+    HJM_HIP(err, HJM_ERR_LAUNCH);
+    HJM_HIP(hipEventRecord(p->ev[2], st), HJM_ERR_DEVICE);
+    err = f64 ? launch_pf_stats_f64(p->vp, (const double*)p->d_ex, p->d_w, p->d_partials,
+                                    p->d_sums, st)
+              : launch_pf_stats_f32(p->vp, (const float*)p->d_ex, p->d_w, p->d_partials,
+                                    p->d_sums, st);
+    HJM_HIP(err, HJM_ERR_LAUNCH);
+    HJM_HIP(hipEventRecord(p->ev[3], st), HJM_ERR_DEVICE);
+    if (p->do_pfe) {
+        err = f64 ? launch_radix_select((const double*)p->d_ex, rows, p->paths, p->pfe_rank,
+                                        p->d_pfe, st)
+                  : launch_radix_select_f32((const float*)p->d_ex, rows, p->paths, p->pfe_rank,
+                                            p->d_pfe, st);
+        HJM_HIP(err, HJM_ERR_LAUNCH);
+    }
+    HJM_HIP(hipEventRecord(p->ev[4], st), HJM_ERR_DEVICE);
+    return HJM_OK;
+}
+
+int pplan_run(hjm_portfolio_plan p)
+{
+    if (!p->tuning.use_graph) return pplan_enqueue(p);
+    if (!p->exec) {
+        HJM_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), HJM_ERR_DEVICE);
+        const int rc = pplan_enqueue(p);
+        hipGraph_t graph = nullptr;
+        const hipError_t end = hipStreamEndCapture(p->stream, &graph);
+        if (rc != HJM_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        HJM_HIP(end, HJM_ERR_DEVICE);
+        p->graph = graph;
+        HJM_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0), HJM_ERR_DEVICE);
+    }
+    HJM_HIP(hipGraphLaunch(p->exec, p->stream), HJM_ERR_LAUNCH);
+    return HJM_OK;
+}
+
+int pplan_collect(hjm_portfolio_plan p, hjm_portfolio_results* r)
+{
+    HJM_HIP(hipStreamSynchronize(p->stream), HJM_ERR_LAUNCH);
+    const int G = p->nsets, nd = p->ndates, nc = nd + 2;
+    std::vector<double> sums((size_t)G * nc);
+    HJM_HIP(hipMemcpy(sums.data(), p->d_sums, sums.size() * sizeof(double), hipMemcpyDeviceToHost),
+            HJM_ERR_DEVICE);
+    const double P = (double)p->paths, inv = 1.0 / P;
+    for (int g = 0; g < G; ++g) {
+        const double* sg = &sums[(size_t)g * nc];
+        double cva = 0.0;
+        for (int e = 0; e < nd; ++e) {
+            const double ee = inv * sg[e];
+            if (r->expected_exposure) r->expected_exposure[(size_t)g * nd + e] = ee;
+            cva += p->w[(size_t)g * nd + e] * ee;
+        }
+        if (r->cva) r->cva[g] = cva;
+        if (r->cva_stderr) {
+            const double s1 = sg[nd], s2 = sg[nd + 1];
+            const double var = (p->paths > 1) ? (s2 - s1 * s1 / P) / (P - 1.0) : 0.0;
+            r->cva_stderr[g] = sqrt((var > 0.0 ? var : 0.0) / P);
+        }
+    }
+    if (r->pfe) {
+        if (p->do_pfe)
+            HJM_HIP(hipMemcpy(r->pfe, p->d_pfe, sizeof(double) * G * nd, hipMemcpyDeviceToHost),
+                    HJM_ERR_DEVICE);
+        else
+            for (int i = 0; i < G * nd; ++i) r->pfe[i] = 0.0;
+    }
+    if (r->exposures) {
+        const size_t cells = (size_t)G * nd * p->paths;
+        std::vector<double> tmp(cells);
+        if (p->precision == HJM_FP64) {
+            HJM_HIP(hipMemcpy(tmp.data(), p->d_ex, cells * sizeof(double), hipMemcpyDeviceToHost),
+                    HJM_ERR_DEVICE);
+        } else {
+            std::vector<float> tf(cells);
+            HJM_HIP(hipMemcpy(tf.data(), p->d_ex, cells * sizeof(float), hipMemcpyDeviceToHost),
+                    HJM_ERR_DEVICE);
+            for (size_t i = 0; i < cells; ++i) tmp[i] = tf[i];
+        }
+        for (int64_t i = 0; i < p->paths; ++i)
+            for (int row = 0; row < G * nd; ++row)
+                r->exposures[(size_t)i * G * nd + row] = tmp[(size_t)row * p->paths + i];
+    }
+    if (r->timings_ms) {
+        float ms = 0.f;
+        for (int i = 0; i < 4; ++i) {
+            HJM_HIP(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]), HJM_ERR_DEVICE);
+            r->timings_ms[i] = ms;
+        }
+        HJM_HIP(hipEventElapsedTime(&ms, p->ev[0], p->ev[4]), HJM_ERR_DEVICE);
+        r->timings_ms[4] = ms;
+    }
+    return HJM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hjm_portfolio_plan_create(hjm_portfolio_plan* out, const hjm_market* market,
+                              const hjm_portfolio* portfolio, const hjm_config* config,
+                              const hjm_tuning* tuning)
+{
+    if (!out) return HJM_ERR_NULL;
+    *out = nullptr;
+    hjm_portfolio_plan p = new (std::nothrow) hjm_pplan_s();
+    if (!p) return HJM_ERR_ALLOC;
+    const int rc = pplan_init(p, market, portfolio, config, tuning);
+    if (rc != HJM_OK) {
+        pplan_release(p);
+        return rc;
+    }
+    *out = p;
+    return HJM_OK;
+}
+
+int hjm_portfolio_plan_execute(hjm_portfolio_plan plan, hjm_portfolio_results* results)
+{
+    if (!plan || !results) return HJM_ERR_NULL;
+    DeviceGuard guard;
+    HJM_TRY(guard.set(plan->device));
+    HJM_TRY(pplan_run(plan));
+    return pplan_collect(plan, results);
+}
+
+void hjm_portfolio_plan_destroy(hjm_portfolio_plan plan) { pplan_release(plan); }
+
+int hjm_simulate_portfolio(const hjm_market* market, const hjm_portfolio* portfolio,
+                           const hjm_config* config, const hjm_tuning* tuning,
+                           hjm_portfolio_results* results)
+{
+    if (!results) return HJM_ERR_NULL;
+    hjm_portfolio_plan p = nullptr;
+    HJM_TRY(hjm_portfolio_plan_create(&p, market, portfolio, config, tuning));
+    const int rc = hjm_portfolio_plan_execute(p, results);
+    hjm_portfolio_plan_destroy(p);
+    return rc;
+}
+
+int hjm_portfolio_autotune(const hjm_market* market, const hjm_portfolio* portfolio,
+                           const hjm_config* config, hjm_tuning* best, double* best_ms)
+{
+    if (!best || !config || !portfolio) return HJM_ERR_NULL;
+    // the path kernel's cost is linear in the horizon: tune on two dates
+    hjm_portfolio short_pf = *portfolio;
+    short_pf.horizon = 2.0 * portfolio->dtau;
+    std::vector<int32_t> per(portfolio->periods, portfolio->periods + portfolio->nswaps);
+    for (auto& x : per) x = std::min(x, 2);
+    short_pf.periods = per.data();
+    hjm_config cfg = *config;
+    cfg.pfe_quantile = 0.0;
+    if (cfg.rng == HJM_RNG_HOST) cfg.rng = HJM_RNG_PHILOX;
+    double bestv = 1e300;
+    hjm_tuning bt{256, 0, 1, 0};
+    for (int bs : {64, 128, 256}) {
+        for (int bpc : {0, 1, 2, 4, 8}) {
+            hjm_tuning t{bs, bpc, 0, 0};
+            hjm_portfolio_plan p = nullptr;
+            int rc = hjm_portfolio_plan_create(&p, market, &short_pf, &cfg, &t);
+            if (rc != HJM_OK) return rc;
+            double tot = 0.0;
+            hjm_portfolio_results r{};
+            double tm[5];
+            r.timings_ms = tm;
+            for (int rep = 0; rep < 4 && rc == HJM_OK; ++rep) {
+                DeviceGuard guard;
+                rc = guard.set(p->device);
+                if (rc == HJM_OK) rc = pplan_enqueue(p);
+                if (rc == HJM_OK) rc = pplan_collect(p, &r);
+                if (rep > 0) tot += tm[0];
+            }
+            hjm_portfolio_plan_destroy(p);
+            if (rc != HJM_OK) return rc;
+            if (tot < bestv) {
+                bestv = tot;
+                bt = hjm_tuning{bs, bpc, 1, 0};
+            }
+        }
+    }
+    *best = bt;
+    if (best_ms) *best_ms = bestv / 3.0;
+    return HJM_OK;
+}
+
+}  // extern "C"

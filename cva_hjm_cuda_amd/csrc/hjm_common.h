@@ -44,6 +44,28 @@ struct RefGrid {
     double a0[kTenors];  // frozen block-0 integral: spot_j summed stride times (FP64)
 };
 
+// Portfolio semantics: dates every dtau up to kMaxPeriods periods (30y at dtau = 0.5).
+constexpr int kMaxPeriods = 60;
+constexpr int kMaxDates = kMaxPeriods + 1;
+
+struct PfGrid {
+    int stride;      // Euler steps per date interval
+    int nsteps;      // horizon / dt
+    int ndates_m1;   // index of the last date (horizon / dtau)
+    int pad;
+    int64_t paths;
+    int64_t path0;
+    uint32_t seed_lo, seed_hi;
+    double dt;
+};
+
+struct PfValueParams {
+    int64_t paths;
+    int nsets;
+    int ndates;
+    double dtau;
+};
+
 // ---- wave / block reductions (fixed order, deterministic) ------------------------
 __device__ __forceinline__ double wave_sum(double v)
 {

@@ -34,6 +34,24 @@ hipError_t launch_ref_adjoint_sweep(const RefGrid& rg, double c, double sqrt_dt,
                                     const double* normals, const float* gx, const float* gA,
                                     double* partials, hipStream_t st);
 
+// Portfolio semantics.
+hipError_t launch_pf_paths_f64(const RefStepConsts<double>& k, const PfGrid& g, const double* normals,
+                               double* curves, double* logb, int block, int grid, hipStream_t st);
+hipError_t launch_pf_paths_f32(const RefStepConsts<float>& k, const PfGrid& g, const double* normals,
+                               float* curves, double* logb, int block, int grid, hipStream_t st);
+hipError_t launch_pf_value_f64(const PfValueParams& v, const double* curves, const double* logb,
+                               const double* coef, const double* a, const int* kmax, double* ex,
+                               hipStream_t st);
+hipError_t launch_pf_value_f32(const PfValueParams& v, const float* curves, const double* logb,
+                               const float* coef, const float* a, const int* kmax, float* ex,
+                               hipStream_t st);
+hipError_t launch_pf_stats_f64(const PfValueParams& v, const double* ex, const double* w,
+                               double* partials, double* sums, hipStream_t st);
+hipError_t launch_pf_stats_f32(const PfValueParams& v, const float* ex, const double* w,
+                               double* partials, double* sums, hipStream_t st);
+hipError_t launch_radix_select_f32(const float* x, int rows, int64_t n, int64_t k, double* out,
+                                   hipStream_t st);
+
 // The reference's own GPU design (baseline form).
 hipError_t launch_blockwise_paths(const RefStepConsts<double>& k, const RefGrid& g, int block_index,
                                   const double* normals, const double* state_in, double* state_out,

@@ -144,6 +144,59 @@ int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
 int hjm_simulate_reference_blockwise(const hjm_market* market, const hjm_swap* swap,
                                      const hjm_config* config, hjm_results* results);
 
+/* ---- portfolio semantics (batched swaps over netting sets) --------------------------
+ * Successor of the reference's single-swap product (product.h:4-17), whose diagonal
+ * numeraire (simulation_cpu.h:248-250,279-295) cannot express maturities past the
+ * 25y tenor grid. Dates t_e = e dtau, e = 0..horizon/dtau (<= 60), evolved from t = 0;
+ * bank-account numeraire B(t_e) = exp(dt sum_{n < n_e} f(t_n, 0)); bond prices
+ * P(t_e, t_e + k dtau) = exp(-trapezoid integral of the simulated curve), flat beyond
+ * 25y; swap value sign N (1 - P_n - K dtau sum_{k<=n} P_k) over its remaining n periods;
+ * exposure per netting set max(sum of its swaps, 0) / B(t_e).                      */
+typedef struct {
+    int32_t nswaps;
+    int32_t nsets;
+    const int32_t* periods;       /* [nswaps] maturity in dtau periods, 1..horizon/dtau */
+    const double* sign_notional;  /* [nswaps] +notional for payer, -notional for receiver */
+    const double* fixed_rate;     /* [nswaps] */
+    const int32_t* netting_set;   /* [nswaps] in 0..nsets-1 */
+    const double* hazard_rate;    /* [nsets] flat hazard rate of each counterparty */
+    double recovery;
+    double horizon;               /* years */
+    double dtau;                  /* date spacing = payment period (0.5) */
+} hjm_portfolio;
+
+typedef struct {
+    int32_t block_size;     /* threads per workgroup of the path kernel: 64, 128 or 256 */
+    int32_t blocks_per_cu;  /* persistent path kernel: grid = blocks_per_cu x CUs;
+                               0 = one workgroup per tile of paths */
+    int32_t use_graph;      /* capture all passes in one hipGraph, replay it per execute */
+    int32_t reserved;
+} hjm_tuning;
+
+typedef struct {
+    double* expected_exposure;  /* [nsets][ndates] */
+    double* pfe;                /* [nsets][ndates] */
+    double* cva;                /* [nsets] (1-R) sum_e w_ge EE_ge with the set's hazard */
+    double* cva_stderr;         /* [nsets] */
+    double* exposures;          /* [paths][nsets][ndates] per-path exposures, or NULL */
+    double* timings_ms;         /* [5] path kernel, valuation, statistics, PFE, total */
+} hjm_portfolio_results;
+
+typedef struct hjm_pplan_s* hjm_portfolio_plan;
+int hjm_portfolio_plan_create(hjm_portfolio_plan* plan, const hjm_market* market,
+                              const hjm_portfolio* portfolio, const hjm_config* config,
+                              const hjm_tuning* tuning /* NULL: defaults */);
+int hjm_portfolio_plan_execute(hjm_portfolio_plan plan, hjm_portfolio_results* results);
+void hjm_portfolio_plan_destroy(hjm_portfolio_plan plan);
+int hjm_simulate_portfolio(const hjm_market* market, const hjm_portfolio* portfolio,
+                           const hjm_config* config, const hjm_tuning* tuning,
+                           hjm_portfolio_results* results);
+/* Times the path kernel over block sizes {64, 128, 256} x persistent grids {0, 1, 2, 4, 8
+ * workgroups per CU} on the problem's own path count and a 2-date horizon; writes the
+ * fastest into *best (use_graph set) and its time into *best_ms. */
+int hjm_portfolio_autotune(const hjm_market* market, const hjm_portfolio* portfolio,
+                           const hjm_config* config, hjm_tuning* best, double* best_ms);
+
 /* ---- drop-in for simulation_gpu.h:10,12 (calculateExposureMultiGPU) --------------
  * EE only; scenarios are split into num_gpus contiguous shards run on devices
  * g % device_count; in-kernel Philox keyed by seed 1234 (the reference's seed,

@@ -1,0 +1,112 @@
+"""GPU parity of the portfolio semantics with the oracle, and its execution modes.
+
+FP64 on identical normals: per-path netting-set exposures, EE, CVA and SE within 1e-10
+relative of the oracle; PFE the exact order statistic; in-kernel Philox bit-identical to
+host normals; hipGraph replays bit-identical to eager runs (several replays with eager
+work in between); tuned persistent grids bit-identical to the default grid; FP32 versus
+FP64 at the batched configuration's size; shard additivity.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cva_hjm_cuda_amd as hjm
+from cva_hjm_cuda_amd.portfolio import Portfolio
+
+pytestmark = pytest.mark.gpu
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def _stats_from_exposures(ex, pf, dtau, recovery, oracle, q=0.975):
+    P, G, D = ex.shape
+    ee = ex.mean(axis=0)
+    k = max(1, min(P, math.ceil(q * P)))
+    pfe = np.sort(ex, axis=0)[k - 1]
+    cva, se = np.zeros(G), np.zeros(G)
+    for g in range(G):
+        w = oracle.cva_weights(D, dtau, pf.hazard[g], recovery)
+        per = ex[:, g, :] @ w
+        cva[g] = ee[g] @ w
+        se[g] = per.std(ddof=1) / math.sqrt(P)
+    return ee, pfe, cva, se
+
+
+@pytest.fixture(scope="module")
+def small(oracle):
+    _flush("[gpu] portfolio oracle run: 128 paths x 30y, 64 swaps, 8 sets")
+    m = hjm.benchmark_market()
+    pf = hjm.synthetic_portfolio(64, 8, seed=7)
+    z = oracle.normals(31, 128, 3000)
+    ex = oracle.portfolio_exposure(m.spot_rates, m.drifts, m.volatilities, 30.0, 0.5, 0.01, z, pf)
+    return m, pf, z, ex
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def test_portfolio_fp64_matches_oracle(small, oracle):
+    m, pf, z, ex_o = small
+    r = hjm.simulate_portfolio(m, pf, 128, horizon=30.0, precision=hjm.FP64, normals=z,
+                               keep_paths=True)
+    assert _rel(r.exposures, ex_o) < 1e-10
+    ee, pfe, cva, se = _stats_from_exposures(ex_o, pf, 0.5, 0.4, oracle)
+    assert _rel(r.expected_exposure, ee) < 1e-10
+    assert np.max(np.abs(r.cva / cva - 1)) < 1e-10
+    assert np.max(np.abs(r.cva_stderr / se - 1)) < 1e-8
+    assert _rel(r.pfe, pfe) < 1e-10
+    # the exact order statistic of the GPU's own exposures
+    k = math.ceil(0.975 * 128)
+    assert np.array_equal(r.pfe, np.sort(r.exposures, axis=0)[k - 1])
+    # t = 0 is deterministic
+    assert np.all(r.exposures[:, :, 0] == r.exposures[0:1, :, 0])
+
+
+def test_portfolio_in_kernel_rng_equals_host_normals(small):
+    m, pf, z, _ = small
+    a = hjm.simulate_portfolio(m, pf, 128, horizon=30.0, precision=hjm.FP64, normals=z)
+    b = hjm.simulate_portfolio(m, pf, 128, horizon=30.0, precision=hjm.FP64, seed=31)
+    assert np.array_equal(a.expected_exposure, b.expected_exposure)
+    assert np.array_equal(a.cva, b.cva) and np.array_equal(a.pfe, b.pfe)
+
+
+def test_graph_replays_and_tuned_grids_are_bit_identical():
+    _flush("[gpu] hipGraph replays vs eager, tuned grids")
+    m = hjm.benchmark_market()
+    pf = hjm.synthetic_portfolio(96, 12, seed=3)
+    eager = hjm.simulate_portfolio(m, pf, 20000, horizon=15.0, seed=9)
+    plan = hjm.PortfolioPlan(m, pf, 20000, horizon=15.0, seed=9,
+                             tuning=hjm.Tuning(256, 0, use_graph=True))
+    for rep in range(4):
+        r = plan.execute()
+        assert np.array_equal(r.expected_exposure, eager.expected_exposure), rep
+        assert np.array_equal(r.pfe, eager.pfe) and np.array_equal(r.cva, eager.cva), rep
+        # unrelated eager work on the device between replays
+        hjm.simulate_reference(m, hjm.InterestRateSwap(1e6, 0.02, 5.0), 512, seed=rep)
+    plan.close()
+    for t in (hjm.Tuning(64, 1), hjm.Tuning(128, 4), hjm.Tuning(256, 2, use_graph=True)):
+        r = hjm.simulate_portfolio(m, pf, 20000, horizon=15.0, seed=9, tuning=t)
+        assert np.array_equal(r.expected_exposure, eager.expected_exposure), t
+    tune, ms = hjm.autotune_portfolio(m, pf, 20000, horizon=15.0, seed=9)
+    assert tune.block_size in (64, 128, 256) and ms > 0
+
+
+def test_batched_config_fp32_vs_fp64_and_shards():
+    _flush("[gpu] batched config: 262144 paths x 30y, 256 swaps, 32 sets, FP32 vs FP64")
+    m = hjm.benchmark_market()
+    pf = hjm.synthetic_portfolio(256, 32, seed=2024)
+    a = hjm.simulate_portfolio(m, pf, 262144, horizon=30.0, precision=hjm.FP64, seed=2024)
+    b = hjm.simulate_portfolio(m, pf, 262144, horizon=30.0, precision=hjm.FP32_KAHAN, seed=2024)
+    assert np.all(a.cva > 0) and np.all(a.cva_stderr < 0.1 * a.cva)
+    assert abs(b.cva.sum() / a.cva.sum() - 1) < 1e-4
+    assert np.max(np.abs(b.cva / a.cva - 1)) < 1e-3
+    assert np.all(a.pfe >= 0) and np.all(a.expected_exposure[:, -1] == 0)
+    h1 = hjm.simulate_portfolio(m, pf, 131072, horizon=30.0, precision=hjm.FP64, seed=2024)
+    h2 = hjm.simulate_portfolio(m, pf, 131072, horizon=30.0, precision=hjm.FP64, seed=2024,
+                                path_offset=131072)
+    ee = 0.5 * (h1.expected_exposure + h2.expected_exposure)
+    assert _rel(ee, a.expected_exposure) < 1e-12
