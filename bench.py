@@ -11,7 +11,11 @@ normals (path-steps/s and the FP32-vs-FP64 EE/PFE/CVA errors), the reference's o
 design in FP64 ("plain FP64 GPU form", one timed pass), and the CPU oracle on a bounded
 sample (cpu_baseline).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config precision|batched|full]
+
+--config batched / full run the portfolio workloads instead (256 swaps in 32 netting sets
+x 262144 paths, resp. 1024 swaps in 128 sets x 1048576 paths, 30y, FP32 + compensated,
+autotuned persistent path kernel, one hipGraph replay per step).
 """
 import argparse
 import json
@@ -50,6 +54,61 @@ def run_plan(hjm, market, swap, precision, steps, warmup):
     return out
 
 
+def adjoint_ratio(hjm, market, reps=3):
+    """CVA + 204 Greeks versus forward-only device time: adjoint configuration (10y payer,
+    65536 paths x 1000 steps, FP32 forward, FP64 accumulation of the Greeks)."""
+    swap = hjm.InterestRateSwap(1e6, 0.025, 10.0)
+    hjm.cva_greeks(market, swap, 65536, DT, precision=hjm.FP32_KAHAN, seed=SEED)
+    t = np.array([hjm.cva_greeks(market, swap, 65536, DT, precision=hjm.FP32_KAHAN,
+                                 seed=SEED).timings_ms for _ in range(reps)])
+    fwd, adj = float(np.median(t[:, 0])), float(np.median(t[:, 1]))
+    return {"paths": 65536, "steps_per_path": 1000, "forward_ms": fwd, "adjoint_ms": adj,
+            "cva_plus_greeks_over_forward": (fwd + adj) / fwd, "greeks": 204}
+
+
+def run_portfolio(hjm, args, nswaps, nsets, paths):
+    market = hjm.benchmark_market()
+    pf = hjm.synthetic_portfolio(nswaps, nsets, seed=2024)
+    log("[bench] autotuning the portfolio path kernel")
+    tune, tune_ms = hjm.autotune_portfolio(market, pf, paths, 30.0, DT, seed=2024)
+    plan = hjm.PortfolioPlan(market, pf, paths, 30.0, DT, precision=hjm.FP32_KAHAN, seed=2024,
+                             tuning=tune)
+    for _ in range(args.warmup):
+        plan.execute()
+    times = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        times.append(plan.execute().timings_ms.copy())
+    wall = time.perf_counter() - t0
+    plan.close()
+    S = 3000
+    times = np.array(times)
+    kp = float(np.mean(times[:, 0]))
+    flops = 51 * 10 * paths * S   # 5 FMAs per tenor, no discount integrals
+    achieved = flops / (kp * 1e-3) / 1e12
+    value = paths * S * args.steps / wall
+    return {
+        "metric": "path-steps/s (51 tenors) FP32 vs FP64; CVA+204 Greeks time vs forward-only",
+        "value": value, "unit": "path-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic portfolio (seed 2024), in-kernel Philox normals",
+        "config": {"workload": "portfolio: %d swaps in %d netting sets, %d paths x %d steps "
+                               "(30y, dt=0.01), EE/PFE97.5/CVA per set" % (nswaps, nsets, paths, S),
+                   "paths": paths, "steps_per_path": S, "swaps": nswaps, "netting_sets": nsets,
+                   "tuning": {"block_size": tune.block_size,
+                              "blocks_per_cu": tune.blocks_per_cu, "hip_graph": True}},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["f32"],
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f32"], "traffic": None,
+                     "kernel": "pf_paths_kernel<float>", "kernel_ms": kp},
+        "kernel_ms_breakdown": {"paths": kp, "valuation": float(np.mean(times[:, 1])),
+                                "statistics": float(np.mean(times[:, 2])),
+                                "pfe_select": float(np.mean(times[:, 3])),
+                                "device_total": float(np.mean(times[:, 4]))},
+        "cpu_baseline": None,
+    }
+
+
 def cpu_baseline(market, target_s=10.0):
     """The oracle's reference loop (simulation_cpu.h:413-436) in FP64 on one thread, timed
     around the path loop only, on chunks of the same workload until ~target_s."""
@@ -75,12 +134,13 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--config", default="precision", choices=["precision", "batched", "full"])
     ap.add_argument("--no-baseline-form", action="store_true",
                     help="skip the reference-design (plain FP64) GPU run")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the path shards across GPUs by contiguous path "
-              "ranges (see DESIGN.md); runs with --gpus > 1 are not part of this build.",
+              "ranges of one seed; runs with --gpus > 1 are not provided.",
               file=sys.stderr)
         sys.exit(2)
 
@@ -92,6 +152,11 @@ def main():
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
     except Exception:
         sync = lambda: None  # noqa: E731
+
+    if args.config != "precision":
+        n = {"batched": (256, 32, 262144), "full": (1024, 128, 1048576)}[args.config]
+        print(json.dumps(run_portfolio(hjm, args, *n)), flush=True)
+        return
 
     market = hjm.benchmark_market()
     swap = hjm.InterestRateSwap(1e6, 0.025, EXPIRY)
@@ -130,6 +195,8 @@ def main():
         cva_rel=abs(f32["cva"] / f64["cva"] - 1.0),
         cva_se_rel=f64["se"] / f64["cva"])
 
+    log("[bench] adjoint configuration: CVA + 204 Greeks vs forward")
+    adj = adjoint_ratio(hjm, market)
     cpu = None if args.no_cpu else cpu_baseline(market)
     res = {
         "metric": "path-steps/s (51 tenors) FP32 vs FP64; CVA+204 Greeks time vs forward-only",
@@ -162,6 +229,7 @@ def main():
                  "fp32_over_fp64": value / f64_rate},
         "plain_fp64_gpu_form": plain,
         "fp32_vs_fp64_error": err,
+        "adjoint": adj,
         "kernel_ms_breakdown": {"paths": k32,
                                 "exposure": float(np.mean(f32["kernel_ms"][:, 1])),
                                 "reduce": float(np.mean(f32["kernel_ms"][:, 2])),

@@ -175,7 +175,7 @@ class Market:
     def __post_init__(self):
         for name, n in (("spot_rates", TENORS), ("drifts", TENORS),
                         ("volatilities", FACTORS * TENORS), ("accrual", TENORS)):
-            a = np.ascontiguousarray(np.asarray(getattr(self, name), np.float64).reshape(-1))
+            a = np.array(getattr(self, name), dtype=np.float64, copy=True).reshape(-1)
             if a.size != n:
                 raise ValueError("%s must have %d entries" % (name, n))
             setattr(self, name, a)
