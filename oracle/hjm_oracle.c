@@ -607,6 +607,123 @@ int orc_portfolio_exposure(const double* spot, const double* drift, const double
     return 0;
 }
 
+/* Portfolio adjoint. Per path J = sum_g sum_e w_ge max(V_ge, 0) e^{-lnB_e}.
+ * At date e: dJ/dV_ge = w_ge 1[V_ge > 0] D_e (D_e = e^{-lnB_e}), dJ/dlnB_e = -sum_g w_ge E_ge;
+ * dJ/dP_k = -sum_g dJ/dV_ge dV_ge/dP_k with V = sum_s sN (1 - P_n - K dtau sum_{j<=n} P_j);
+ * dJ/dI_k = -P_k dJ/dP_k; I_k = sum_{l<k} dtau/2 (f_l + f_{l+1}) (k <= 50), flat beyond:
+ * dJ/df_l = dtau/2 (S_{l+1} [l <= 49] + S_l [l >= 1]) + [l = 50] dtau sum_{k>50} (k-50) iota_k,
+ * S_m = sum_{k >= m} iota_k. lnB_e = dt sum_{n < n_e} f_0(n).
+ * Backward: lambda(n) = T^T lambda(n+1) + gbar_e [n = n_e] + mu(n) e_0,
+ * mu(n) = dt sum_{e: n_e > n} dJ/dlnB_e; dJ/dV_kj = sum_n lambda_j(n+1) sqrt(dt) phi_k(n),
+ * dJ/dm_j = dt sum_n lambda_j(n+1), dJ/dspot = lambda(0). */
+int orc_portfolio_cva_greeks(const double* spot, const double* drift, const double* vols,
+                             double horizon, double dtau, double dt, int npaths,
+                             const double* normals, int nswaps, const int* periods,
+                             const double* sign_notional, const double* fixed_rate,
+                             const int* set_of, int nsets, const double* weights,
+                             double* cva, double* d_spot, double* d_vol, double* d_drift)
+{
+    if (!(dt > 0.0) || !(dtau > 0.0) || !(horizon > 0.0) || npaths <= 0 || nsets <= 0) return 2;
+    double rs = dtau / dt, rS = horizon / dt;
+    int st = (int)llround(rs), S = (int)llround(rS);
+    if (st <= 0 || fabs(rs - st) > 1e-6 * st || fabs(rS - S) > 1e-6 * S || S % st) return 3;
+    const int E = S / st, nd = E + 1;
+    const double c = dt / dtau, sq = sqrt(dt);
+    double* ex = (double*)malloc(sizeof(double) * nsets * nd);
+    double* cv = (double*)malloc(sizeof(double) * nd * ORC_TENORS);
+    double* lb = (double*)malloc(sizeof(double) * nd);
+    double* gbar = (double*)malloc(sizeof(double) * nd * ORC_TENORS);
+    double* beta = (double*)malloc(sizeof(double) * nd);
+    double* P = (double*)malloc(sizeof(double) * (E + 2));
+    double* dP = (double*)malloc(sizeof(double) * (E + 2));
+    double* V = (double*)malloc(sizeof(double) * nsets);
+    double gs[ORC_TENORS] = {0}, gv[3 * ORC_TENORS] = {0}, gm[ORC_TENORS] = {0}, J = 0.0;
+    for (int p = 0; p < npaths; ++p) {
+        const double* nrm = normals + (int64_t)p * S * 3;
+        int rc = orc_portfolio_exposure(spot, drift, vols, horizon, dtau, dt, 1, nrm, nswaps,
+                                        periods, sign_notional, fixed_rate, set_of, nsets, ex,
+                                        cv, lb);
+        if (rc) return rc;
+        for (int e = 0; e < nd; ++e) {
+            const double* f = cv + e * ORC_TENORS;
+            const double D = exp(-lb[e]);
+            const int kmax = E - e;
+            bond_prices(f, dtau, kmax, P);
+            for (int g = 0; g < nsets; ++g) V[g] = 0.0;
+            for (int s = 0; s < nswaps; ++s) {
+                const int k = periods[s] - e;
+                if (k <= 0) continue;
+                double ann = 0.0;
+                for (int j = 1; j <= k; ++j) ann += P[j];
+                V[set_of[s]] += sign_notional[s] * (1.0 - P[k] - fixed_rate[s] * dtau * ann);
+            }
+            double b = 0.0;
+            for (int k = 0; k <= kmax; ++k) dP[k] = 0.0;
+            for (int g = 0; g < nsets; ++g) {
+                const double w = weights[g * nd + e];
+                if (V[g] > 0.0) {
+                    J += w * V[g] * D;
+                    b -= w * V[g] * D;
+                }
+            }
+            for (int s = 0; s < nswaps; ++s) {
+                const int k = periods[s] - e;
+                if (k <= 0) continue;
+                const int g = set_of[s];
+                if (!(V[g] > 0.0)) continue;
+                const double dv = weights[g * nd + e] * D * sign_notional[s];
+                dP[k] -= dv;
+                for (int j = 1; j <= k; ++j) dP[j] -= dv * fixed_rate[s] * dtau;
+            }
+            /* iota_k = dJ/dI_k = -P_k dJ/dP_k ; suffix sums */
+            double suffix[64] = {0};
+            for (int k = kmax; k >= 1; --k) suffix[k] = suffix[k + 1] + (-P[k] * dP[k]);
+            double* gb = gbar + e * ORC_TENORS;
+            for (int l = 0; l < ORC_TENORS; ++l) {
+                double v = 0.0;
+                const int lim = kmax < ORC_TENORS - 1 ? kmax : ORC_TENORS - 1;
+                if (l <= 49 && l + 1 <= lim) v += 0.5 * dtau * suffix[l + 1];
+                if (l >= 1 && l <= lim) v += 0.5 * dtau * suffix[l];
+                if (l == ORC_TENORS - 1)
+                    for (int k = ORC_TENORS; k <= kmax; ++k) v += dtau * (k - 50) * (-P[k] * dP[k]);
+                /* intervals up to node lim only: I_k for k <= 50 uses f_0..f_k */
+                gb[l] = v;
+            }
+            beta[e] = b;
+        }
+        /* backward sweep */
+        double lam[ORC_TENORS], mu0 = 0.0;
+        memcpy(lam, gbar + E * ORC_TENORS, sizeof(lam));
+        for (int n = S - 1; n >= 0; --n) {
+            const double* phi = nrm + (int64_t)n * 3;
+            for (int j = 0; j < ORC_TENORS; ++j) {
+                gv[j] += lam[j] * sq * phi[0];
+                gv[ORC_TENORS + j] += lam[j] * sq * phi[1];
+                gv[2 * ORC_TENORS + j] += lam[j] * sq * phi[2];
+                gm[j] += dt * lam[j];
+            }
+            apply_TT(lam, c);
+            if ((n + 1) % st == 0) mu0 += dt * beta[(n + 1) / st];
+            lam[0] += mu0;
+            if (n % st == 0) {
+                const double* gb = gbar + (n / st) * ORC_TENORS;
+                for (int j = 0; j < ORC_TENORS; ++j) lam[j] += gb[j];
+            }
+        }
+        for (int j = 0; j < ORC_TENORS; ++j) gs[j] += lam[j];
+    }
+    const double inv = 1.0 / npaths;
+    if (cva) *cva = J * inv;
+    for (int j = 0; j < ORC_TENORS; ++j) {
+        if (d_spot) d_spot[j] = gs[j] * inv;
+        if (d_drift) d_drift[j] = gm[j] * inv;
+    }
+    if (d_vol)
+        for (int j = 0; j < 3 * ORC_TENORS; ++j) d_vol[j] = gv[j] * inv;
+    free(ex); free(cv); free(lb); free(gbar); free(beta); free(P); free(dP); free(V);
+    return 0;
+}
+
 /* ===================================================================================
  * CPU baseline: the path loop of simulation_cpu.h:407-440 timed alone, on nthreads.
  * =================================================================================== */

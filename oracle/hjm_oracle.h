@@ -90,6 +90,15 @@ int orc_portfolio_exposure(const double* spot, const double* drift, const double
                            const int* set_of, int nsets, double* exposures, double* curves,
                            double* log_bank);
 
+/* Total portfolio CVA sum_g (1-R) sum_e w_ge EE_ge (w from each set's hazard) and its
+ * pathwise derivatives w.r.t. the initial curve (51), loadings (153), drift (51). */
+int orc_portfolio_cva_greeks(const double* spot, const double* drift, const double* vols,
+                             double horizon, double dtau, double dt, int npaths,
+                             const double* normals, int nswaps, const int* periods,
+                             const double* sign_notional, const double* fixed_rate,
+                             const int* set_of, int nsets, const double* weights,
+                             double* cva, double* d_spot, double* d_vol, double* d_drift);
+
 /* ---- timing helper for the CPU baseline ----------------------------------------- */
 double orc_time_reference_paths(const double* spot, const double* drift, const double* vols,
                                 double expiry, double dtau, double dt, int npaths,

@@ -54,6 +54,11 @@ class Oracle:
                                              C.c_int, _dp, C.c_int, _ip, _dp, _dp, _ip, C.c_int,
                                              _dp, C.c_void_p, C.c_void_p]
         L.orc_portfolio_exposure.restype = C.c_int
+        L.orc_portfolio_cva_greeks.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double,
+                                               C.c_double, C.c_int, _dp, C.c_int, _ip, _dp,
+                                               _dp, _ip, C.c_int, _dp, C.POINTER(C.c_double),
+                                               _dp, _dp, _dp]
+        L.orc_portfolio_cva_greeks.restype = C.c_int
         L.orc_time_reference_paths.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, C.c_double,
                                                C.c_int, _dp, _dp, _dp, C.c_int]
         L.orc_time_reference_paths.restype = C.c_double
@@ -182,6 +187,26 @@ class Oracle:
         if rc:
             raise ValueError("orc_portfolio_exposure rc=%d" % rc)
         return (ex, cv, lb) if want_curves else ex
+
+    def portfolio_weights(self, pf, horizon, recovery=0.4):
+        nd = int(round(horizon / pf.dtau)) + 1
+        return np.stack([self.cva_weights(nd, pf.dtau, h, recovery) for h in pf.hazard])
+
+    def portfolio_cva(self, spot, drift, vols, horizon, dtau, dt, normals, pf, w):
+        ex = self.portfolio_exposure(spot, drift, vols, horizon, dtau, dt, normals, pf)
+        return float(np.einsum("pge,ge->", ex, w) / ex.shape[0])
+
+    def portfolio_cva_greeks(self, spot, drift, vols, horizon, dtau, dt, normals, pf, w):
+        P = normals.shape[0]
+        cva = C.c_double()
+        ds, dv, dm = np.zeros(TENORS), np.zeros(3 * TENORS), np.zeros(TENORS)
+        rc = self.lib.orc_portfolio_cva_greeks(
+            _c(spot), _c(drift), _c(vols), horizon, dtau, dt, P, _c(normals), pf.nswaps,
+            pf.periods, pf.sign_notional, pf.fixed_rate, pf.set_of, pf.nsets, _c(w),
+            C.byref(cva), ds, dv, dm)
+        if rc:
+            raise ValueError("orc_portfolio_cva_greeks rc=%d" % rc)
+        return cva.value, ds, dv, dm
 
     def time_reference_paths(self, spot, drift, vols, expiry, dtau, dt, normals, nthreads=1):
         P = normals.shape[0]

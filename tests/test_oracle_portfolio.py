@@ -109,3 +109,38 @@ def test_synthetic_portfolio_generator():
     assert sizes.max() > 3 * sizes.min()
     again = synthetic_portfolio(1024, 128, seed=2024)
     assert np.array_equal(pf.sign_notional, again.sign_notional)
+
+
+@pytest.mark.parametrize("horizon,P,nsw", [(6.0, 24, 20), (30.0, 2, 10)])
+def test_portfolio_adjoint_matches_central_fd(oracle, horizon, P, nsw):
+    """Total portfolio CVA Greeks (curve, loadings, drift) against central finite
+    differences of the oracle's own CVA with common random numbers; the 30y case runs
+    swaps past the 25y grid (flat extrapolation of the curve)."""
+    spot = oracle.nelson_siegel()
+    vols = oracle.poly_vols()
+    drift = oracle.hjm_drift(vols)
+    pf = synthetic_portfolio(nsw, 4, seed=5)
+    E = int(round(horizon / 0.5))
+    pf = Portfolio(np.minimum(pf.periods, E), pf.sign_notional, pf.fixed_rate, pf.set_of,
+                   pf.hazard)
+    z = oracle.normals(41, P, int(round(horizon / 0.01)))
+    w = oracle.portfolio_weights(pf, horizon)
+    cva, ds, dv, dm = oracle.portfolio_cva_greeks(spot, drift, vols, horizon, 0.5, 0.01, z, pf, w)
+    assert cva == pytest.approx(oracle.portfolio_cva(spot, drift, vols, horizon, 0.5, 0.01, z, pf,
+                                                     w), rel=1e-12)
+    rng = np.random.default_rng(1)
+    for name, g, base, h in (("spot", ds, spot, 1e-6), ("drift", dm, drift, 1e-7),
+                             ("vols", dv, vols, 1e-7)):
+        idx = rng.choice(len(base), 12, replace=False)
+        for j in idx:
+            up, dn = base.copy(), base.copy()
+            up[j] += h
+            dn[j] -= h
+            args = {"spot": spot, "drift": drift, "vols": vols}
+            a_up = dict(args, **{name: up})
+            a_dn = dict(args, **{name: dn})
+            fd = (oracle.portfolio_cva(a_up["spot"], a_up["drift"], a_up["vols"], horizon, 0.5,
+                                       0.01, z, pf, w)
+                  - oracle.portfolio_cva(a_dn["spot"], a_dn["drift"], a_dn["vols"], horizon, 0.5,
+                                         0.01, z, pf, w)) / (2 * h)
+            assert abs(g[j] - fd) <= 1e-4 * np.max(np.abs(g)) + 1e-6 * abs(fd), (name, j, g[j], fd)
