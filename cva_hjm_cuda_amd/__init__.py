@@ -7,5 +7,6 @@ from .api import (FP32_KAHAN, FP64, RNG_HOST, RNG_PHILOX, TENORS, HJMError,  # n
                   Greeks, InterestRateSwap, Market, ReferencePlan, ReferenceResult, benchmark_market,
                   calculate_exposure_multi_gpu, cva_greeks, cva_weights, device_count, generate_normals,
                   hjm_drift, library, simulate_reference, PortfolioPlan, PortfolioResult,
-                  Tuning, simulate_portfolio, autotune_portfolio)
+                  Tuning, simulate_portfolio, autotune_portfolio, portfolio_cva_greeks)
+from .api import KERNEL_AUTO, KERNEL_MATRIX, KERNEL_VECTOR, block_map  # noqa: F401
 from .portfolio import Portfolio, synthetic_portfolio  # noqa: F401

@@ -55,7 +55,7 @@ class _Config(C.Structure):
     _fields_ = [("paths", C.c_int64), ("path_offset", C.c_int64), ("dt", C.c_double),
                 ("precision", C.c_int32), ("rng", C.c_int32), ("seed", C.c_uint64),
                 ("host_normals", _dp), ("hazard_rate", C.c_double), ("recovery", C.c_double),
-                ("pfe_quantile", C.c_double), ("device", C.c_int32), ("reserved", C.c_int32)]
+                ("pfe_quantile", C.c_double), ("device", C.c_int32), ("path_kernel", C.c_int32)]
 
 
 class _Greeks(C.Structure):
@@ -132,10 +132,14 @@ def _declare(lib):
     lib.hjm_portfolio_autotune.argtypes = [C.POINTER(_Market), C.POINTER(_Portfolio),
                                            C.POINTER(_Config), C.POINTER(_Tuning),
                                            C.POINTER(C.c_double)]
+    lib.hjm_portfolio_cva_greeks.argtypes = [C.POINTER(_Market), C.POINTER(_Portfolio),
+                                             C.POINTER(_Config), C.POINTER(_Tuning),
+                                             C.POINTER(_Greeks)]
     lib.hjm_calculate_exposure_multi_gpu_f64.argtypes = [_dp, d, d, d, d, _dp, _dp, _dp, _dp, d,
                                                          i, i, d]
     lib.hjm_calculate_exposure_multi_gpu_f32.argtypes = [fp, f, f, f, f, fp, fp, fp, fp, f, i, i,
                                                          f]
+    lib.hjm_block_map.argtypes = [_dp, _dp, d, d, fp]
     lib.hjm_generate_normals.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_int32, C.c_int32,
                                          _dp]
 
@@ -232,11 +236,14 @@ class ReferenceResult:
     exposures: Optional[np.ndarray] = None
 
 
+KERNEL_AUTO, KERNEL_VECTOR, KERNEL_MATRIX = 0, 1, 2
+
+
 def _config(paths, dt, precision, rng, seed, normals, hazard, recovery, pfe_quantile, device,
-            path_offset):
+            path_offset, path_kernel=KERNEL_AUTO):
     return _Config(int(paths), int(path_offset), float(dt), int(precision), int(rng), int(seed),
                    _ptr(normals), float(hazard), float(recovery), float(pfe_quantile),
-                   int(device), 0)
+                   int(device), int(path_kernel))
 
 
 def simulate_reference(market: Market, swap: InterestRateSwap, paths: int, dt: float = 0.01,
@@ -244,7 +251,7 @@ def simulate_reference(market: Market, swap: InterestRateSwap, paths: int, dt: f
                        normals: Optional[np.ndarray] = None, hazard: float = 0.02,
                        recovery: float = 0.4, pfe_quantile: float = 0.975, device: int = 0,
                        path_offset: int = 0, keep_paths: bool = False,
-                       blockwise: bool = False) -> ReferenceResult:
+                       blockwise: bool = False, path_kernel: int = KERNEL_AUTO) -> ReferenceResult:
     """Simulate the reference semantics on the GPU and return EE, PFE, CVA (+ SE).
 
     ``normals`` (float64, [paths][expiry/dt][3]) switches to caller-supplied variates;
@@ -262,7 +269,7 @@ def simulate_reference(market: Market, swap: InterestRateSwap, paths: int, dt: f
     mk = market._c()
     sw = _Swap(swap.notional, swap.K, swap.expiry, swap.dtau)
     cfg = _config(paths, dt, precision, rng, seed, normals, hazard, recovery, pfe_quantile,
-                  device, path_offset)
+                  device, path_offset, path_kernel)
     ee, pfe = np.zeros(TENORS), np.zeros(TENORS)
     cva, se = np.zeros(1), np.zeros(1)
     sums, tm = np.zeros(STAT_SLOTS), np.zeros(5)
@@ -290,7 +297,8 @@ class Greeks:
 def cva_greeks(market: Market, swap: InterestRateSwap, paths: int, dt: float = 0.01,
                precision: int = FP32_KAHAN, seed: int = 1234,
                normals: Optional[np.ndarray] = None, hazard: float = 0.02,
-               recovery: float = 0.4, device: int = 0, path_offset: int = 0) -> Greeks:
+               recovery: float = 0.4, device: int = 0, path_offset: int = 0,
+               path_kernel: int = KERNEL_AUTO) -> Greeks:
     """CVA and its 51 + 153 (+ 51 drift) pathwise-adjoint sensitivities on the GPU."""
     rng = RNG_PHILOX
     if normals is not None:
@@ -299,7 +307,7 @@ def cva_greeks(market: Market, swap: InterestRateSwap, paths: int, dt: float = 0
     mk = market._c()
     sw = _Swap(swap.notional, swap.K, swap.expiry, swap.dtau)
     cfg = _config(paths, dt, precision, rng, seed, normals, hazard, recovery, 0.0, device,
-                  path_offset)
+                  path_offset, path_kernel)
     cva, se = np.zeros(1), np.zeros(1)
     ds, dv, dm, dvt, tm = np.zeros(TENORS), np.zeros(3 * TENORS), np.zeros(TENORS), \
         np.zeros(3 * TENORS), np.zeros(3)
@@ -314,13 +322,14 @@ class ReferencePlan:
 
     def __init__(self, market: Market, swap: InterestRateSwap, paths: int, dt: float = 0.01,
                  precision: int = FP64, seed: int = 1234, hazard: float = 0.02,
-                 recovery: float = 0.4, pfe_quantile: float = 0.975, device: int = 0):
+                 recovery: float = 0.4, pfe_quantile: float = 0.975, device: int = 0,
+                 path_kernel: int = KERNEL_AUTO):
         self._lib = library()
         self._market = market
         mk = market._c()
         sw = _Swap(swap.notional, swap.K, swap.expiry, swap.dtau)
         cfg = _config(paths, dt, precision, RNG_PHILOX, seed, None, hazard, recovery,
-                      pfe_quantile, device, 0)
+                      pfe_quantile, device, 0, path_kernel)
         self._h = C.c_void_p()
         _check(self._lib.hjm_plan_create(C.byref(self._h), C.byref(mk), C.byref(sw),
                                          C.byref(cfg)), "hjm_plan_create")
@@ -345,6 +354,14 @@ class ReferencePlan:
             self.close()
         except Exception:
             pass
+
+
+def block_map(market: Market, dt: float = 0.01, dtau: float = 0.5) -> np.ndarray:
+    """The matrix-core kernel's affine 50-step block map, [102][220] float32."""
+    out = np.zeros((102, 220), np.float32)
+    _check(library().hjm_block_map(_ptr(market.drifts), _ptr(market.volatilities), dt, dtau,
+                                   out.ctypes.data_as(C.POINTER(C.c_float))), "hjm_block_map")
+    return out
 
 
 def generate_normals(seed: int, paths: int, nsteps: int, path_offset: int = 0,
@@ -487,3 +504,26 @@ def autotune_portfolio(market: Market, portfolio, paths: int, horizon: float = 3
     _check(library().hjm_portfolio_autotune(C.byref(mk), C.byref(pfc), C.byref(cfg), C.byref(t),
                                             C.byref(ms)), "hjm_portfolio_autotune")
     return Tuning(t.block_size, t.blocks_per_cu, bool(t.use_graph)), ms.value
+
+
+def portfolio_cva_greeks(market: Market, portfolio, paths: int, horizon: float = 30.0,
+                         dt: float = 0.01, precision: int = FP32_KAHAN, seed: int = 2024,
+                         normals: Optional[np.ndarray] = None, recovery: float = 0.4,
+                         tuning: Optional[Tuning] = None, device: int = 0) -> Greeks:
+    """Total portfolio CVA and its 51 + 153 (+ 51 drift) pathwise-adjoint sensitivities."""
+    rng = RNG_PHILOX
+    if normals is not None:
+        normals = np.ascontiguousarray(normals, np.float64)
+        rng = RNG_HOST
+    mk = market._c()
+    pfc = _pf_c(portfolio, horizon, recovery)
+    cfg = _config(paths, dt, precision, rng, seed, normals, 0.0, recovery, 0.0, device, 0)
+    tun = tuning._c() if tuning is not None else None
+    cva, se = np.zeros(1), np.zeros(1)
+    ds, dv, dm, dvt, tm = (np.zeros(TENORS), np.zeros(3 * TENORS), np.zeros(TENORS),
+                           np.zeros(3 * TENORS), np.zeros(3))
+    g = _Greeks(_ptr(cva), _ptr(se), _ptr(ds), _ptr(dv), _ptr(dm), _ptr(dvt), _ptr(tm))
+    _check(library().hjm_portfolio_cva_greeks(C.byref(mk), C.byref(pfc), C.byref(cfg),
+                                              C.byref(tun) if tun is not None else None,
+                                              C.byref(g)), "hjm_portfolio_cva_greeks")
+    return Greeks(float(cva[0]), float(se[0]), ds, dv, dm, dvt, tm)

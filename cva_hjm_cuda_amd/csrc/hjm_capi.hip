@@ -14,6 +14,7 @@
 #include "hjm_common.h"
 #include "hjm_kernels.h"
 #include "hjm_host.h"
+#include "hjm_mfma_plan.h"
 
 using namespace hjm;
 using namespace hjm_host;
@@ -43,45 +44,6 @@ int ref_grid(double expiry, double dtau, double dt, int* stride, int* nsteps, in
     return HJM_OK;
 }
 
-// Cumulative Simpson integral of y on a uniform grid (the rule of hjm_hjm_drift).
-void cumulative_simpson(const double* y, double h, double* out)
-{
-    out[0] = 0.0;
-    for (int j = 1; j < kTenors; ++j) {
-        double s = 0.0;
-        if (j == 1) {
-            s = h / 12.0 * (5.0 * y[0] + 8.0 * y[1] - y[2]);
-        } else {
-            const int even_end = (j % 2 == 0) ? j : j - 3;
-            for (int i = 0; i < even_end; i += 2) s += h / 3.0 * (y[i] + 4.0 * y[i + 1] + y[i + 2]);
-            if (j % 2 == 1) s += 3.0 * h / 8.0 * (y[j - 3] + 3.0 * y[j - 2] + 3.0 * y[j - 1] + y[j]);
-        }
-        out[j] = s;
-    }
-}
-
-// out += (d m / d sigma)^T g for m_j = sum_k sigma_kj I_kj, I_k = W sigma_k:
-// d m_j / d sigma_kl = delta_jl I_kj + sigma_kj W_jl.
-void drift_vjp(const double* vols, double h, const double* g, double* out)
-{
-    double W[kTenors][kTenors];  // W[l][j] = weight of y_l in the integral to node j
-    for (int l = 0; l < kTenors; ++l) {
-        double e[kTenors] = {0.0};
-        e[l] = 1.0;
-        cumulative_simpson(e, h, W[l]);
-    }
-    for (int k = 0; k < kFactors; ++k) {
-        const double* sk = vols + k * kTenors;
-        double I[kTenors];
-        cumulative_simpson(sk, h, I);
-        for (int l = 0; l < kTenors; ++l) {
-            double a = g[l] * I[l];
-            for (int j = 0; j < kTenors; ++j) a += g[j] * sk[j] * W[l][j];
-            out[k * kTenors + l] += a;
-        }
-    }
-}
-
 }  // namespace
 
 struct hjm_plan_s {
@@ -91,6 +53,7 @@ struct hjm_plan_s {
     int precision = HJM_FP64;
     bool host_rng = false;
     bool do_pfe = false;
+    bool use_mfma = false;
     int64_t paths = 0;
     int64_t nblocks = 0;
     int64_t pfe_rank = 1;
@@ -98,6 +61,8 @@ struct hjm_plan_s {
     RefStepConsts<float> kf{};
     RefGrid grid{};
     ExposureParams ep{};
+    MfmaGrid mg{};
+    float* d_wfrag = nullptr;
     double* d_normals = nullptr;
     double* d_nx = nullptr;
     double* d_ny = nullptr;
@@ -116,6 +81,7 @@ void plan_release(hjm_plan p)
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(p->device);
     dev_free(p->d_normals);
+    dev_free(p->d_wfrag);
     dev_free(p->d_nx);
     dev_free(p->d_ny);
     dev_free(p->d_ex);
@@ -142,9 +108,14 @@ int plan_init(hjm_plan p, const hjm_market* market, const hjm_swap* swap, const 
     if (!(cfg->hazard_rate >= 0.0) || !(cfg->recovery >= 0.0 && cfg->recovery <= 1.0))
         return HJM_ERR_ARG;
     if (cfg->pfe_quantile > 1.0) return HJM_ERR_ARG;
+    if (cfg->path_kernel < HJM_KERNEL_AUTO || cfg->path_kernel > HJM_KERNEL_MATRIX) return HJM_ERR_ARG;
     int stride, S, D;
     HJM_TRY(ref_grid(swap->expiry, swap->dtau, cfg->dt, &stride, &S, &D));
     if ((int64_t)cfg->paths * kTenors > (1ll << 40)) return HJM_ERR_ARG;
+    const bool mfma_ok = cfg->precision == HJM_FP32_KAHAN && cfg->rng == HJM_RNG_PHILOX &&
+                         stride == kMfmaStride;
+    if (cfg->path_kernel == HJM_KERNEL_MATRIX && !mfma_ok) return HJM_ERR_UNSUPPORTED;
+    p->use_mfma = mfma_ok && cfg->path_kernel != HJM_KERNEL_VECTOR;
 
     p->device = cfg->device;
     p->precision = cfg->precision;
@@ -193,6 +164,25 @@ int plan_init(hjm_plan p, const hjm_market* market, const hjm_swap* swap, const 
     HJM_TRY(dev_alloc(&p->d_partials, (size_t)p->nblocks * kStatSlots));
     HJM_TRY(dev_alloc(&p->d_sums, kStatSlots));
     HJM_TRY(dev_alloc(&p->d_pfe, kTenors));
+    if (p->use_mfma) {
+        std::vector<float> wf;
+        build_mfma_block_map(market->drifts, market->volatilities, cfg->dt, swap->dtau, wf);
+        HJM_TRY(dev_alloc(&p->d_wfrag, wf.size()));
+        HJM_HIP(hipMemcpyAsync(p->d_wfrag, wf.data(), wf.size() * sizeof(float),
+                               hipMemcpyHostToDevice, p->stream),
+                HJM_ERR_DEVICE);
+        MfmaGrid& mg = p->mg;
+        mg.ndates = D;
+        mg.paths = cfg->paths;
+        mg.path0 = cfg->path_offset;
+        mg.seed_lo = g.seed_lo;
+        mg.seed_hi = g.seed_hi;
+        mg.dt = cfg->dt;
+        for (int j = 0; j < kTenors; ++j) {
+            mg.a0[j] = g.a0[j];
+            mg.spot[j] = (float)market->spot_rates[j];
+        }
+    }
     HJM_HIP(hipMemsetAsync(p->d_nx, 0, slots * sizeof(double), p->stream), HJM_ERR_DEVICE);
     HJM_HIP(hipMemsetAsync(p->d_ny, 0, slots * sizeof(double), p->stream), HJM_ERR_DEVICE);
     if (p->host_rng) {
@@ -211,7 +201,8 @@ int plan_launch(hjm_plan p, bool keep_exposures)
 {
     hipStream_t st = p->stream;
     HJM_HIP(hipEventRecord(p->ev[0], st), HJM_ERR_DEVICE);
-    hipError_t err = (p->precision == HJM_FP64)
+    hipError_t err = p->use_mfma ? launch_ref_paths_mfma(p->mg, p->d_wfrag, p->d_nx, p->d_ny, st)
+                     : (p->precision == HJM_FP64)
                          ? launch_ref_paths_f64(p->kd, p->grid, p->d_normals, p->d_nx, p->d_ny, st)
                          : launch_ref_paths_f32k(p->kf, p->grid, p->d_normals, p->d_nx, p->d_ny, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
@@ -594,6 +585,23 @@ int hjm_simulate_reference_blockwise(const hjm_market* market, const hjm_swap* s
         HJM_HIP(hipEventElapsedTime(&ms, b.ev[0], b.ev[4]), HJM_ERR_DEVICE);
         r->timings_ms[4] = ms;
     }
+    return HJM_OK;
+}
+
+int hjm_block_map(const double* drifts, const double* vols, double dt, double dtau, float* out)
+{
+    if (!drifts || !vols || !out) return HJM_ERR_NULL;
+    if (!(dt > 0.0) || !(dtau > 0.0) || fabs(dtau / dt - kMfmaStride) > 1e-6 * kMfmaStride)
+        return HJM_ERR_GRID;
+    std::vector<float> wf;
+    build_mfma_block_map(drifts, vols, dt, dtau, wf);
+    const int K = 4 * kMfmaKChunks;
+    for (int m = 0; m < kMfmaMTiles; ++m)
+        for (int kc = 0; kc < kMfmaKChunks; ++kc)
+            for (int l = 0; l < kWave; ++l) {
+                const int row = 16 * m + (l % 16), kidx = 4 * kc + l / 16;
+                if (row < 2 * kTenors) out[(size_t)row * K + kidx] = wf[((size_t)m * kMfmaKChunks + kc) * kWave + l];
+            }
     return HJM_OK;
 }
 

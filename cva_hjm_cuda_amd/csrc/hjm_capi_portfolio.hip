@@ -394,6 +394,101 @@ int hjm_simulate_portfolio(const hjm_market* market, const hjm_portfolio* portfo
     return rc;
 }
 
+int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* portfolio,
+                             const hjm_config* config, const hjm_tuning* tuning, hjm_greeks* gr)
+{
+    if (!gr || !config) return HJM_ERR_NULL;
+    hjm_config cfg = *config;
+    cfg.pfe_quantile = 0.0;
+    hjm_portfolio_plan p = nullptr;
+    HJM_TRY(hjm_portfolio_plan_create(&p, market, portfolio, &cfg, tuning));
+    struct Cleanup {
+        hjm_portfolio_plan p;
+        float *gbar = nullptr, *beta = nullptr;
+        double *pa = nullptr, *pb = nullptr, *sa = nullptr, *sb = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Cleanup()
+        {
+            dev_free(gbar);
+            dev_free(beta);
+            dev_free(pa);
+            dev_free(pb);
+            dev_free(sa);
+            dev_free(sb);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            hjm_portfolio_plan_destroy(p);
+        }
+    } c{p};
+    DeviceGuard guard;
+    HJM_TRY(guard.set(p->device));
+    const int64_t P = p->paths;
+    const int nd = p->ndates;
+    const int64_t nb = (P + kBlock - 1) / kBlock;
+    HJM_TRY(dev_alloc(&c.gbar, (size_t)nd * kTenors * P));
+    HJM_TRY(dev_alloc(&c.beta, (size_t)nd * P));
+    HJM_TRY(dev_alloc(&c.pa, (size_t)nb * 2 * kTenors));
+    HJM_TRY(dev_alloc(&c.pb, (size_t)nb * 3 * kTenors));
+    HJM_TRY(dev_alloc(&c.sa, 2 * kTenors));
+    HJM_TRY(dev_alloc(&c.sb, 3 * kTenors));
+    HJM_HIP(hipEventCreate(&c.e0), HJM_ERR_DEVICE);
+    HJM_HIP(hipEventCreate(&c.e1), HJM_ERR_DEVICE);
+    hipStream_t st = p->stream;
+    HJM_TRY(pplan_enqueue(p));
+    HJM_HIP(hipEventRecord(c.e0, st), HJM_ERR_DEVICE);
+    const bool f64 = p->precision == HJM_FP64;
+    hipError_t err = f64 ? launch_pf_adjoint_seed_f64(p->vp, (const double*)p->d_curves, p->d_logb,
+                                                      (const double*)p->d_coef, (const double*)p->d_a,
+                                                      p->d_kmax, p->d_w, c.gbar, c.beta, st)
+                         : launch_pf_adjoint_seed_f32(p->vp, (const float*)p->d_curves, p->d_logb,
+                                                      (const float*)p->d_coef, (const float*)p->d_a,
+                                                      p->d_kmax, p->d_w, c.gbar, c.beta, st);
+    HJM_HIP(err, HJM_ERR_LAUNCH);
+    const double cc = p->grid.dt / portfolio->dtau;
+    HJM_HIP(launch_pf_adjoint_sweep(p->grid, cc, 0, p->d_normals, c.gbar, c.beta, c.pa, st),
+            HJM_ERR_LAUNCH);
+    HJM_HIP(launch_column_sum(c.pa, nb, 2 * kTenors, c.sa, st), HJM_ERR_LAUNCH);
+    HJM_HIP(launch_pf_adjoint_sweep(p->grid, cc, 1, p->d_normals, c.gbar, c.beta, c.pb, st),
+            HJM_ERR_LAUNCH);
+    HJM_HIP(launch_column_sum(c.pb, nb, 3 * kTenors, c.sb, st), HJM_ERR_LAUNCH);
+    HJM_HIP(hipEventRecord(c.e1, st), HJM_ERR_DEVICE);
+    std::vector<double> cvas(p->nsets);
+    hjm_portfolio_results r{};
+    r.cva = cvas.data();
+    HJM_TRY(pplan_collect(p, &r));
+    double sa[2 * kTenors], sb[3 * kTenors];
+    HJM_HIP(hipMemcpy(sa, c.sa, sizeof(sa), hipMemcpyDeviceToHost), HJM_ERR_DEVICE);
+    HJM_HIP(hipMemcpy(sb, c.sb, sizeof(sb), hipMemcpyDeviceToHost), HJM_ERR_DEVICE);
+    double total = 0.0;
+    for (double v : cvas) total += v;
+    if (gr->cva) *gr->cva = total;
+    if (gr->cva_stderr) *gr->cva_stderr = NAN;
+    const double inv = 1.0 / (double)P, dt = p->grid.dt;
+    double dv[kFactors * kTenors], dm[kTenors];
+    for (int j = 0; j < kTenors; ++j) {
+        dv[j] = sa[j] * inv;
+        dv[kTenors + j] = sa[kTenors + j] * inv;
+        dv[2 * kTenors + j] = sb[j] * inv;
+        dm[j] = dt * sb[kTenors + j] * inv;
+        if (gr->d_spot) gr->d_spot[j] = sb[2 * kTenors + j] * inv;
+    }
+    if (gr->d_vol) memcpy(gr->d_vol, dv, sizeof(dv));
+    if (gr->d_drift) memcpy(gr->d_drift, dm, sizeof(dm));
+    if (gr->d_vol_total) {
+        memcpy(gr->d_vol_total, dv, sizeof(dv));
+        drift_vjp(market->volatilities, portfolio->dtau, dm, gr->d_vol_total);
+    }
+    if (gr->timings_ms) {
+        float fwd = 0.f, adj = 0.f;
+        HJM_HIP(hipEventElapsedTime(&fwd, p->ev[0], p->ev[4]), HJM_ERR_DEVICE);
+        HJM_HIP(hipEventElapsedTime(&adj, c.e0, c.e1), HJM_ERR_DEVICE);
+        gr->timings_ms[0] = fwd;
+        gr->timings_ms[1] = adj;
+        gr->timings_ms[2] = fwd + adj;
+    }
+    return HJM_OK;
+}
+
 int hjm_portfolio_autotune(const hjm_market* market, const hjm_portfolio* portfolio,
                            const hjm_config* config, hjm_tuning* best, double* best_ms)
 {

@@ -44,6 +44,18 @@ struct RefGrid {
     double a0[kTenors];  // frozen block-0 integral: spot_j summed stride times (FP64)
 };
 
+// Matrix-core FP32 path kernel (reference semantics, 50-step blocks).
+struct MfmaGrid {
+    int ndates;
+    int pad;
+    int64_t paths;
+    int64_t path0;
+    uint32_t seed_lo, seed_hi;
+    double dt;
+    double a0[kTenors];   // frozen block-0 integrals (FP64)
+    float spot[kTenors];
+};
+
 // Portfolio semantics: dates every dtau up to kMaxPeriods periods (30y at dtau = 0.5).
 constexpr int kMaxPeriods = 60;
 constexpr int kMaxDates = kMaxPeriods + 1;

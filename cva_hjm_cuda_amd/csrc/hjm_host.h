@@ -108,4 +108,43 @@ inline void cva_weights(int ndates, double date_dt, double h, double R, double* 
 }
 
 
+// Cumulative Simpson integral of y on a uniform grid (the rule of hjm_hjm_drift).
+inline void cumulative_simpson(const double* y, double h, double* out)
+{
+    out[0] = 0.0;
+    for (int j = 1; j < kTenors; ++j) {
+        double s = 0.0;
+        if (j == 1) {
+            s = h / 12.0 * (5.0 * y[0] + 8.0 * y[1] - y[2]);
+        } else {
+            const int even_end = (j % 2 == 0) ? j : j - 3;
+            for (int i = 0; i < even_end; i += 2) s += h / 3.0 * (y[i] + 4.0 * y[i + 1] + y[i + 2]);
+            if (j % 2 == 1) s += 3.0 * h / 8.0 * (y[j - 3] + 3.0 * y[j - 2] + 3.0 * y[j - 1] + y[j]);
+        }
+        out[j] = s;
+    }
+}
+
+// out += (d m / d sigma)^T g for m_j = sum_k sigma_kj I_kj, I_k = W sigma_k:
+// d m_j / d sigma_kl = delta_jl I_kj + sigma_kj W_jl.
+inline void drift_vjp(const double* vols, double h, const double* g, double* out)
+{
+    double W[kTenors][kTenors];  // W[l][j] = weight of y_l in the integral to node j
+    for (int l = 0; l < kTenors; ++l) {
+        double e[kTenors] = {0.0};
+        e[l] = 1.0;
+        cumulative_simpson(e, h, W[l]);
+    }
+    for (int k = 0; k < kFactors; ++k) {
+        const double* sk = vols + k * kTenors;
+        double I[kTenors];
+        cumulative_simpson(sk, h, I);
+        for (int l = 0; l < kTenors; ++l) {
+            double a = g[l] * I[l];
+            for (int j = 0; j < kTenors; ++j) a += g[j] * sk[j] * W[l][j];
+            out[k * kTenors + l] += a;
+        }
+    }
+}
+
 }  // namespace hjm_host

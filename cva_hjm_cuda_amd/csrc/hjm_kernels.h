@@ -27,6 +27,9 @@ hipError_t launch_radix_select(const double* x, int rows, int64_t n, int64_t k, 
 hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nsteps, double* out,
                           hipStream_t st);
 
+hipError_t launch_ref_paths_mfma(const MfmaGrid& g, const float* wfrag, double* nx, double* ny,
+                                 hipStream_t st);
+
 // Pathwise adjoint of the reference-semantics CVA.
 hipError_t launch_ref_adjoint_seed(const ExposureParams& e, double dt, const double* nx,
                                    const double* ny, float* gx, float* gA, hipStream_t st);
@@ -50,6 +53,18 @@ hipError_t launch_pf_stats_f64(const PfValueParams& v, const double* ex, const d
 hipError_t launch_pf_stats_f32(const PfValueParams& v, const float* ex, const double* w,
                                double* partials, double* sums, hipStream_t st);
 hipError_t launch_radix_select_f32(const float* x, int rows, int64_t n, int64_t k, double* out,
+                                   hipStream_t st);
+
+hipError_t launch_pf_adjoint_seed_f64(const PfValueParams& v, const double* curves,
+                                      const double* logb, const double* coef, const double* a,
+                                      const int* kmax, const double* w, float* gbar, float* beta,
+                                      hipStream_t st);
+hipError_t launch_pf_adjoint_seed_f32(const PfValueParams& v, const float* curves,
+                                      const double* logb, const float* coef, const float* a,
+                                      const int* kmax, const double* w, float* gbar, float* beta,
+                                      hipStream_t st);
+hipError_t launch_pf_adjoint_sweep(const PfGrid& g, double c, int pass, const double* normals,
+                                   const float* gbar, const float* beta, double* partials,
                                    hipStream_t st);
 
 // The reference's own GPU design (baseline form).

@@ -1,0 +1,117 @@
+// hjm_mfma_plan.h -- host construction of the 50-step block map for the matrix-core
+// FP32 path kernel (reference semantics).
+//
+// One block of `stride` Euler steps is affine in the curve f, the scaled shocks
+// s_{t,k} = sqrt(dt) phi_k(t) and 1:
+//   f_out  = T^S f_in + sum_t T^{S-1-t} (m dt + V s_t)
+//   dA     = sum_{r=1}^{S} f_r = (sum_{r=1}^{S} T^r) f_in + sum_t (sum_{q=0}^{S-1-t} T^q)(m dt + V s_t)
+// with T = I + (dt/dtau) Dfwd (backward difference on the last tenor). The kernel
+// computes [f_out; dA] (102 rows) = W x [f_in; phi (3 per step); 1] with MFMA 16x16x4 f32
+// tiles. Rows: F_j at 0..50, dA_j at 51..101 (padded to 7 M-tiles of 16). K is laid out
+// in chunks of 4 so that each lane group g = lane/16 of a chunk is one K index:
+//   chunk kc < 16      : the previous block's output registers, kc = 4 m + r holds rows
+//                        16 m + 4 g + r (zero columns where that row is not a curve row)
+//   chunk 16 + 3 q + k : factor k of step t = 4 q + g (t < S); step S (q = 12, g = 2) k = 0
+//                        is the constant 1 carrying the drift; other padding is zero.
+// Wfrag[m][kc][lane] = W[16 m + lane % 16][K(kc, lane / 16)] is the A-operand fragment.
+#pragma once
+#include <vector>
+
+#include "hjm_common.h"
+
+namespace hjm {
+
+constexpr int kMfmaStride = 50;   // steps per block handled by the matrix-core kernel
+constexpr int kMfmaMTiles = 7;    // 102 output rows -> 7 x 16
+constexpr int kMfmaFChunks = 16;  // previous block's rows 0..63 as B operand
+constexpr int kMfmaQuads = 13;    // ceil(50 / 4) step quads
+constexpr int kMfmaKChunks = kMfmaFChunks + 3 * kMfmaQuads;  // 55
+constexpr int kMfmaFragFloats = kMfmaMTiles * kMfmaKChunks * kWave;
+
+inline void build_mfma_block_map(const double* drift, const double* vols, double dt, double dtau,
+                                 std::vector<float>& wfrag)
+{
+    const int n = kTenors, S = kMfmaStride;
+    const double c = dt / dtau, sq = sqrt(dt);
+    auto matmul = [n](const std::vector<double>& A, const std::vector<double>& B) {
+        std::vector<double> C((size_t)n * n, 0.0);
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < n; ++k) {
+                const double a = A[(size_t)i * n + k];
+                if (a == 0.0) continue;
+                for (int j = 0; j < n; ++j) C[(size_t)i * n + j] += a * B[(size_t)k * n + j];
+            }
+        return C;
+    };
+    std::vector<double> T((size_t)n * n, 0.0);
+    for (int j = 0; j < n - 1; ++j) {
+        T[(size_t)j * n + j] = 1.0 - c;
+        T[(size_t)j * n + j + 1] = c;
+    }
+    T[(size_t)(n - 1) * n + n - 1] = 1.0 + c;
+    T[(size_t)(n - 1) * n + n - 2] = -c;
+    // Tp[s] = T^s, Cs[s] = sum_{q=0}^{s} T^q
+    std::vector<std::vector<double>> Tp(S + 1), Cs(S + 1);
+    Tp[0].assign((size_t)n * n, 0.0);
+    for (int j = 0; j < n; ++j) Tp[0][(size_t)j * n + j] = 1.0;
+    Cs[0] = Tp[0];
+    for (int s = 1; s <= S; ++s) {
+        Tp[s] = matmul(T, Tp[s - 1]);
+        Cs[s] = Cs[s - 1];
+        for (size_t i = 0; i < Cs[s].size(); ++i) Cs[s][i] += Tp[s][i];
+    }
+    const int rows = 2 * n, K = 4 * kMfmaKChunks;
+    std::vector<double> W((size_t)kMfmaMTiles * 16 * K, 0.0);
+    auto wref = [&](int row, int kidx) -> double& { return W[(size_t)row * K + kidx]; };
+    auto mv = [n](const std::vector<double>& M, const double* v, int j) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += M[(size_t)j * n + i] * v[i];
+        return s;
+    };
+    std::vector<double> mdt(n);
+    for (int i = 0; i < n; ++i) mdt[i] = drift[i] * dt;
+    for (int kc = 0; kc < kMfmaKChunks; ++kc) {
+        for (int g = 0; g < 4; ++g) {
+            const int kidx = 4 * kc + g;
+            if (kc < kMfmaFChunks) {
+                const int m = kc / 4, r = kc % 4, R = 16 * m + 4 * g + r;
+                if (R >= n) continue;  // rows 51..63 of the previous output are not curve rows
+                for (int j = 0; j < n; ++j) {
+                    wref(j, kidx) = Tp[S][(size_t)j * n + R];
+                    double sa = 0.0;
+                    for (int q = 1; q <= S; ++q) sa += Tp[q][(size_t)j * n + R];
+                    wref(n + j, kidx) = sa;
+                }
+            } else {
+                const int q = (kc - kMfmaFChunks) / 3, k = (kc - kMfmaFChunks) % 3;
+                const int t = 4 * q + g;
+                if (t < S) {
+                    for (int j = 0; j < n; ++j) {
+                        wref(j, kidx) = sq * mv(Tp[S - 1 - t], vols + k * n, j);
+                        wref(n + j, kidx) = sq * mv(Cs[S - 1 - t], vols + k * n, j);
+                    }
+                } else if (t == S && k == 0) {
+                    for (int j = 0; j < n; ++j) {
+                        double sf = 0.0, sa = 0.0;
+                        for (int tt = 0; tt < S; ++tt) {
+                            sf += mv(Tp[S - 1 - tt], mdt.data(), j);
+                            sa += mv(Cs[S - 1 - tt], mdt.data(), j);
+                        }
+                        wref(j, kidx) = sf;
+                        wref(n + j, kidx) = sa;
+                    }
+                }
+            }
+        }
+    }
+    (void)rows;
+    wfrag.assign(kMfmaFragFloats, 0.f);
+    for (int m = 0; m < kMfmaMTiles; ++m)
+        for (int kc = 0; kc < kMfmaKChunks; ++kc)
+            for (int l = 0; l < kWave; ++l) {
+                const int row = 16 * m + (l % 16), kidx = 4 * kc + l / 16;
+                wfrag[((size_t)m * kMfmaKChunks + kc) * kWave + l] = (float)W[(size_t)row * K + kidx];
+            }
+}
+
+}  // namespace hjm

@@ -342,3 +342,257 @@ hipError_t launch_radix_select_f32(const float* x, int rows, int64_t n, int64_t 
 }
 
 }  // namespace hjm
+
+namespace hjm {
+
+// ---- portfolio adjoint ----------------------------------------------------------------
+// Per path J = sum_g sum_e w_ge max(V_ge, 0) e^{-lnB_e}. Seeds at each date from the
+// curve checkpoint: dJ/dP_k = -sum_g w_ge 1[V_ge > 0] D_e gamma_gek (V = a - gamma.P),
+// iota_k = dJ/dI_k = -P_k dJ/dP_k, S_m = sum_{k >= m} iota_k,
+// gbar_l = dtau/2 (S_{l+1} [l <= 49] + S_l [l >= 1]) + [l = 50] dtau sum_{k>50} (k-50) iota_k,
+// beta_e = dJ/dlnB_e = -sum_g w_ge max(V_ge, 0) D_e. Stored [date][tenor][path], [date][path].
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void pf_adjoint_seed_kernel(const PfValueParams v,
+                                                                 const Real* __restrict__ curves,
+                                                                 const double* __restrict__ logb,
+                                                                 const Real* __restrict__ coef,
+                                                                 const Real* __restrict__ aconst,
+                                                                 const int* __restrict__ kmax,
+                                                                 const double* __restrict__ w,
+                                                                 float* __restrict__ gbar,
+                                                                 float* __restrict__ beta)
+{
+    const int64_t P = v.paths;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = blockIdx.y;
+    if (i >= P) return;
+    Real f[kTenors];
+#pragma unroll
+    for (int j = 0; j < kTenors; ++j) f[j] = curves[((int64_t)e * kTenors + j) * P + i];
+    const double D = exp(-logb[(int64_t)e * P + i]);
+    constexpr int KP = kMaxPeriods;
+    Real Pk[KP], dP[KP];
+    Real I = (Real)0;
+    const Real hw = (Real)(0.5 * v.dtau);
+#pragma unroll
+    for (int kk = 1; kk <= KP; ++kk) {
+        if (kk <= kTenors - 1) I += hw * (f[kk - 1] + f[kk]);
+        else I += (Real)v.dtau * f[kTenors - 1];
+        Pk[kk - 1] = exp(-I);
+        dP[kk - 1] = (Real)0;
+    }
+    kconst_ptr_pf<Real> cp = (kconst_ptr_pf<Real>)coef;
+    kconst_ptr_pf<Real> ap = (kconst_ptr_pf<Real>)aconst;
+    kconst_ptr_pf<int> kp = (kconst_ptr_pf<int>)kmax;
+    kconst_ptr_pf<double> wp = (kconst_ptr_pf<double>)w;
+    double b = 0.0;
+    for (int gset = 0; gset < v.nsets; ++gset) {
+        const int64_t row = (int64_t)e * v.nsets + gset;
+        const int K = kp[row];
+        kconst_ptr_pf<Real> gam = cp + row * KP;
+        Real V = ap[row];
+#pragma unroll
+        for (int k0 = 0; k0 < KP; k0 += 4) {
+            if (k0 < K) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) V -= gam[k0 + u] * Pk[k0 + u];
+            }
+        }
+        if (V > (Real)0) {
+            const double wg = wp[(int64_t)gset * v.ndates + e];
+            b -= wg * (double)V * D;
+            const Real a = (Real)(wg * D);  // dJ/dV_g; dV_g/dP_k = -gamma_k
+#pragma unroll
+            for (int k0 = 0; k0 < KP; k0 += 4) {
+                if (k0 < K) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) dP[k0 + u] -= a * gam[k0 + u];
+                }
+            }
+        }
+    }
+    // suffix sums of iota_k = -P_k dP_k and the extrapolation moment
+    Real S = (Real)0, ext = (Real)0;
+    Real Sk[KP + 2];
+    Sk[KP + 1] = (Real)0;
+    Sk[KP] = (Real)0;
+#pragma unroll
+    for (int kk = KP; kk >= 1; --kk) {
+        const Real iota = -Pk[kk - 1] * dP[kk - 1];
+        S += iota;
+        Sk[kk] = S;
+        if (kk > kTenors - 1) ext += (Real)(kk - (kTenors - 1)) * iota;
+    }
+    float* gb = gbar + (int64_t)e * kTenors * P + i;
+#pragma unroll
+    for (int l = 0; l < kTenors; ++l) {
+        Real val = (Real)0;
+        if (l <= kTenors - 2) val += hw * Sk[l + 1];
+        if (l >= 1) val += hw * Sk[l];
+        if (l == kTenors - 1) val += (Real)v.dtau * ext;
+        gb[(int64_t)l * P] = (float)val;
+    }
+    beta[(int64_t)e * P + i] = (float)b;
+}
+
+// Reverse sweep over the whole horizon (n = S-1 .. 0): grads with lambda(n+1) and the
+// shocks of step n, lambda <- T^T lambda + mu e_0 (mu = dt sum of beta of later dates),
+// + gbar_e at date steps. Pass 0: factors 0, 1; pass 1: factor 2, drift, curve.
+template <int kPass, bool kHostNormals>
+__global__ __launch_bounds__(kBlock) void pf_adjoint_sweep_kernel(const PfGrid g, float c,
+                                                                  const double* __restrict__ normals,
+                                                                  const float* __restrict__ gbar,
+                                                                  const float* __restrict__ beta,
+                                                                  double* __restrict__ partials)
+{
+    constexpr int kAcc = 3 * kTenors;
+    __shared__ double acc[kBlock / kWave][kAcc];
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    for (int k = lane; k < kAcc; k += kWave) acc[wave][k] = 0.0;
+    __syncthreads();
+    const int64_t P = g.paths;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool active = i < P;
+    const int64_t ii = active ? i : 0;
+    const uint32_t pid = (uint32_t)(g.path0 + ii);
+    const float sqdt = (float)sqrt(g.dt), dtf = (float)g.dt;
+    float cv = c, mcv = -c, omc = 1.0f - c, opc = 1.0f + c;
+    asm volatile("" : "+v"(cv), "+v"(mcv), "+v"(omc), "+v"(opc));
+    float lam[kTenors], ga[kTenors], gb[kTenors];
+    const int E = g.ndates_m1;
+#pragma unroll
+    for (int j = 0; j < kTenors; ++j) {
+        lam[j] = active ? gbar[((int64_t)E * kTenors + j) * P + i] : 0.f;
+        ga[j] = gb[j] = 0.f;
+    }
+    float mu = 0.f;
+    for (int e = E - 1; e >= 0; --e) {
+        mu += active ? dtf * beta[(int64_t)(e + 1) * P + i] : 0.f;
+        for (int s = g.stride - 1; s >= 0; --s) {
+            const int n = e * g.stride + s;
+            float z0, z1, z2;
+            if constexpr (kHostNormals) {
+                const double* q = normals + ((int64_t)ii * g.nsteps + n) * 3;
+                z0 = (float)q[0];
+                z1 = (float)q[1];
+                z2 = (float)q[2];
+            } else {
+                const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n);
+                z0 = z.z0;
+                z1 = z.z1;
+                z2 = z.z2;
+            }
+            if constexpr (kPass == 0) {
+                const float s0 = sqdt * z0, s1 = sqdt * z1;
+#pragma unroll
+                for (int j = 0; j < kTenors; ++j) {
+                    op_fmac_vv(ga[j], lam[j], s0);
+                    op_fmac_vv(gb[j], lam[j], s1);
+                }
+            } else {
+                const float s2 = sqdt * z2;
+#pragma unroll
+                for (int j = 0; j < kTenors; ++j) {
+                    op_fmac_vv(ga[j], lam[j], s2);
+                    op_add_inplace(gb[j], lam[j]);
+                }
+            }
+            // lambda <- T^T lambda (+ mu on tenor 0), descending
+            float t50, t49;
+            op_fma_vvv(t50, cv, lam[49], 0.f);
+            op_fma_vvv(t49, cv, lam[48], 0.f);
+            op_fma_vvv(t49, mcv, lam[50], t49);
+            op_fma_inplace(lam[50], opc, t50);
+            op_fma_inplace(lam[49], omc, t49);
+            constexpr int G = 4;
+#pragma unroll
+            for (int j0 = 48; j0 >= 1; j0 -= G) {
+                float t[G];
+#pragma unroll
+                for (int u = 0; u < G; ++u)
+                    if (j0 - u >= 1) op_fma_vvv(t[u], cv, lam[j0 - u - 1], 0.f);
+#pragma unroll
+                for (int u = 0; u < G; ++u)
+                    if (j0 - u >= 1) op_fma_inplace(lam[j0 - u], omc, t[u]);
+            }
+            op_fma_inplace(lam[0], omc, mu);
+        }
+        // date e: the valuation at e depends on f(n_e)
+#pragma unroll
+        for (int j = 0; j < kTenors; ++j)
+            lam[j] += active ? gbar[((int64_t)e * kTenors + j) * P + i] : 0.f;
+#pragma unroll
+        for (int j = 0; j < kTenors; ++j) {
+            const double a = wave_sum((double)ga[j]), bb = wave_sum((double)gb[j]);
+            if (lane == 0) {
+                acc[wave][j] += a;
+                acc[wave][kTenors + j] += bb;
+            }
+            ga[j] = gb[j] = 0.f;
+        }
+    }
+    if constexpr (kPass == 1) {
+#pragma unroll
+        for (int j = 0; j < kTenors; ++j) {
+            const double r = wave_sum((double)lam[j]);
+            if (lane == 0) acc[wave][2 * kTenors + j] += r;
+        }
+    }
+    __syncthreads();
+    constexpr int nacc = (kPass == 0) ? 2 * kTenors : 3 * kTenors;
+    for (int k = threadIdx.x; k < nacc; k += kBlock) {
+        double s = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < kBlock / kWave; ++ww) s += acc[ww][k];
+        partials[(int64_t)blockIdx.x * nacc + k] = s;
+    }
+}
+
+template <typename Real>
+static hipError_t pf_adj_seed_launch(const PfValueParams& v, const Real* curves, const double* logb,
+                                     const Real* coef, const Real* a, const int* kmax,
+                                     const double* w, float* gbar, float* beta, hipStream_t st)
+{
+    const dim3 grid((unsigned)((v.paths + kBlock - 1) / kBlock), (unsigned)v.ndates);
+    pf_adjoint_seed_kernel<Real><<<grid, kBlock, 0, st>>>(v, curves, logb, coef, a, kmax, w, gbar,
+                                                          beta);
+    return hipGetLastError();
+}
+
+hipError_t launch_pf_adjoint_seed_f64(const PfValueParams& v, const double* curves,
+                                      const double* logb, const double* coef, const double* a,
+                                      const int* kmax, const double* w, float* gbar, float* beta,
+                                      hipStream_t st)
+{
+    return pf_adj_seed_launch<double>(v, curves, logb, coef, a, kmax, w, gbar, beta, st);
+}
+
+hipError_t launch_pf_adjoint_seed_f32(const PfValueParams& v, const float* curves,
+                                      const double* logb, const float* coef, const float* a,
+                                      const int* kmax, const double* w, float* gbar, float* beta,
+                                      hipStream_t st)
+{
+    return pf_adj_seed_launch<float>(v, curves, logb, coef, a, kmax, w, gbar, beta, st);
+}
+
+hipError_t launch_pf_adjoint_sweep(const PfGrid& g, double c, int pass, const double* normals,
+                                   const float* gbar, const float* beta, double* partials,
+                                   hipStream_t st)
+{
+    const dim3 grid((unsigned)((g.paths + kBlock - 1) / kBlock));
+    const float cf = (float)c;
+    if (normals) {
+        if (pass == 0)
+            pf_adjoint_sweep_kernel<0, true><<<grid, kBlock, 0, st>>>(g, cf, normals, gbar, beta, partials);
+        else
+            pf_adjoint_sweep_kernel<1, true><<<grid, kBlock, 0, st>>>(g, cf, normals, gbar, beta, partials);
+    } else {
+        if (pass == 0)
+            pf_adjoint_sweep_kernel<0, false><<<grid, kBlock, 0, st>>>(g, cf, normals, gbar, beta, partials);
+        else
+            pf_adjoint_sweep_kernel<1, false><<<grid, kBlock, 0, st>>>(g, cf, normals, gbar, beta, partials);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace hjm

@@ -48,6 +48,12 @@ enum hjm_rng_mode {
     HJM_RNG_HOST = 1    /* caller-supplied normals, layout [path][step][3] */
 };
 
+enum hjm_path_kernel {
+    HJM_KERNEL_AUTO = 0,   /* matrix cores for FP32 + in-kernel RNG when dt = dtau / 50 */
+    HJM_KERNEL_VECTOR = 1, /* one path per lane, every step on the vector ALUs */
+    HJM_KERNEL_MATRIX = 2  /* FP32 only: 50-step blocks as MFMA GEMMs (else HJM_ERR_UNSUPPORTED) */
+};
+
 /* Market data (simulation_gpu.h:10 arguments accrual, spot_rates, drift, volatilities). */
 typedef struct {
     const double* spot_rates;   /* [51] initial forward curve f(0, tau_j) */
@@ -77,7 +83,7 @@ typedef struct {
     double recovery;        /* recovery rate for CVA */
     double pfe_quantile;    /* e.g. 0.975; <= 0 skips the PFE selection */
     int32_t device;         /* HIP device ordinal */
-    int32_t reserved;
+    int32_t path_kernel;    /* enum hjm_path_kernel (reference semantics only) */
 } hjm_config;
 
 /* Outputs; every pointer may be NULL (then that output is not produced / copied). */
@@ -191,6 +197,14 @@ void hjm_portfolio_plan_destroy(hjm_portfolio_plan plan);
 int hjm_simulate_portfolio(const hjm_market* market, const hjm_portfolio* portfolio,
                            const hjm_config* config, const hjm_tuning* tuning,
                            hjm_portfolio_results* results);
+/* Total portfolio CVA (sum over netting sets) and its pathwise-adjoint sensitivities to
+ * the 51 initial forwards, 153 loadings and 51 drifts (hjm_greeks; cva_stderr = NaN:
+ * per-set standard errors come from hjm_simulate_portfolio). Forward pass as
+ * hjm_simulate_portfolio without PFE; the curve checkpoints at every date seed the
+ * backward sweep. timings_ms: forward, adjoint, total. */
+int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* portfolio,
+                             const hjm_config* config, const hjm_tuning* tuning,
+                             hjm_greeks* greeks);
 /* Times the path kernel over block sizes {64, 128, 256} x persistent grids {0, 1, 2, 4, 8
  * workgroups per CU} on the problem's own path count and a 2-date horizon; writes the
  * fastest into *best (use_graph set) and its time into *best_ms. */
@@ -212,6 +226,12 @@ int hjm_calculate_exposure_multi_gpu_f32(float* expected_exposure, float notiona
                                          const float* spot_rates, const float* drift,
                                          const float* volatilities, float scale,
                                          int num_gpus, int scenarios, float dt);
+
+/* The affine 50-step block map of the matrix-core kernel as a dense [102][220] FP32
+ * matrix (rows: curve after the block, then its 50-step integral increment; columns:
+ * K order of the kernel), for inspection and tests. Host only. */
+int hjm_block_map(const double* drifts, const double* volatilities, double dt, double dtau,
+                  float* out);
 
 /* ---- normals (for tests and host-supplied runs): the in-kernel generator's output
  * for paths [path_offset, path_offset + paths) and steps [0, nsteps), written as

@@ -1,0 +1,90 @@
+"""The matrix-core kernel's 50-step block map (host side, no GPU).
+
+W maps [f_in; phi; 1] to [f after 50 Euler steps; sum of the 50 curves], in the K order the
+kernel feeds the matrix cores. Pinned two ways: against 50 explicit FP64 Euler steps of the
+Musiela SDE written out here (random curve and shocks), and chained over the blocks of a
+5-year run against the oracle's reference paths (num_x, num_y) on the same normals.
+"""
+import numpy as np
+import pytest
+
+import cva_hjm_cuda_amd as hjm
+
+TENORS, STRIDE, KCH, FCH = 51, 50, 55, 16
+
+
+def _k_vector(f_in, phi):
+    """Input column in the kernel's K order: f rows by (chunk, lane group), then shocks."""
+    x = np.zeros(4 * KCH)
+    for kc in range(KCH):
+        for g in range(4):
+            k = 4 * kc + g
+            if kc < FCH:
+                R = 16 * (kc // 4) + 4 * g + kc % 4
+                if R < TENORS:
+                    x[k] = f_in[R]
+            else:
+                q, fac = divmod(kc - FCH, 3)
+                t = 4 * q + g
+                if t < STRIDE:
+                    x[k] = phi[t, fac]
+                elif t == STRIDE and fac == 0:
+                    x[k] = 1.0
+    return x
+
+
+def _euler(f, drift, vols, phi, dt, dtau):
+    area = np.zeros(TENORS)
+    for t in range(STRIDE):
+        d = np.empty(TENORS)
+        d[:-1] = f[1:] - f[:-1]
+        d[-1] = f[-1] - f[-2]
+        f = f + drift * dt + np.sqrt(dt) * (vols.T @ phi[t]) + d * dt / dtau
+        area += f
+    return f, area
+
+
+@pytest.fixture(scope="module")
+def market():
+    return hjm.benchmark_market()
+
+
+def test_block_map_equals_fifty_euler_steps(market):
+    W = hjm.block_map(market).astype(np.float64)
+    assert W.shape == (102, 220)
+    rng = np.random.default_rng(5)
+    vols = market.volatilities.reshape(3, TENORS)
+    for _ in range(3):
+        f_in = 0.03 + 0.01 * rng.standard_normal(TENORS)
+        phi = rng.standard_normal((STRIDE, 3))
+        f_out, area = _euler(f_in, market.drifts, vols, phi, 0.01, 0.5)
+        y = W @ _k_vector(f_in, phi)
+        np.testing.assert_allclose(y[:TENORS], f_out, rtol=0, atol=2e-8)
+        np.testing.assert_allclose(y[TENORS:], area, rtol=2e-7, atol=0)
+    # the columns of rows 51..63 of the previous output are zero (not curve rows)
+    for kc in range(FCH):
+        for g in range(4):
+            if 16 * (kc // 4) + 4 * g + kc % 4 >= TENORS:
+                assert not np.any(W[:, 4 * kc + g])
+
+
+def test_block_map_chain_matches_oracle_paths(market, oracle):
+    expiry, dtau, dt, P = 5.0, 0.5, 0.01, 6
+    z = oracle.normals(17, P, int(round(expiry / dt)))
+    nx, ny = oracle.reference_paths(market.spot_rates, market.drifts, market.volatilities,
+                                    expiry, dtau, dt, z)
+    W = hjm.block_map(market).astype(np.float64)
+    D = int(round(expiry / dtau))
+    for p in range(P):
+        f = market.spot_rates.copy()
+        A = STRIDE * market.spot_rates.copy()  # block 0 holds the spot curve
+        for b in range(1, D):
+            y = W @ _k_vector(f, z[p, b * STRIDE:(b + 1) * STRIDE])
+            f, A = y[:TENORS], A + y[TENORS:]
+            assert abs(f[b + 1] - nx[p, b + 1]) < 5e-8, (p, b)
+            assert abs(np.exp(-A[b + 1] * dt) / ny[p, b + 1] - 1) < 1e-7, (p, b)
+
+
+def test_block_map_rejects_other_grids(market):
+    with pytest.raises(hjm.HJMError):
+        hjm.block_map(market, dt=0.02)

@@ -19,6 +19,8 @@ def declared_functions():
     for fn in sorted(os.listdir(os.path.join(ROOT, "include"))):
         if fn.endswith(".h"):
             text = open(os.path.join(ROOT, "include", fn)).read()
+            text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+            text = re.sub(r"//[^\n]*", " ", text)
             names |= set(re.findall(r"\b(hjm_[a-z0-9_]+)\s*\(", text))
     return sorted(names)
 

@@ -110,3 +110,27 @@ def test_batched_config_fp32_vs_fp64_and_shards():
                                 path_offset=131072)
     ee = 0.5 * (h1.expected_exposure + h2.expected_exposure)
     assert _rel(ee, a.expected_exposure) < 1e-12
+
+
+def test_portfolio_greeks_match_oracle_adjoint(small, oracle):
+    _flush("[gpu] portfolio adjoint vs oracle adjoint, 128 paths x 30y, FP64 forward")
+    m, pf, z, _ = small
+    w = oracle.portfolio_weights(pf, 30.0)
+    cva, ds, dv, dm = oracle.portfolio_cva_greeks(m.spot_rates, m.drifts, m.volatilities, 30.0,
+                                                  0.5, 0.01, z, pf, w)
+    g = hjm.portfolio_cva_greeks(m, pf, 128, horizon=30.0, precision=hjm.FP64, normals=z)
+    assert g.cva == pytest.approx(cva, rel=1e-10)
+    for got, ref in ((g.d_spot, ds), (g.d_vol, dv), (g.d_drift, dm)):
+        assert np.max(np.abs(got - ref)) <= 1e-4 * np.max(np.abs(ref))
+
+
+def test_full_config_greeks_cost_ratio():
+    _flush("[gpu] portfolio Greeks cost: 65536 paths x 30y, 1024 swaps in 128 sets, FP32")
+    m = hjm.benchmark_market()
+    pf = hjm.synthetic_portfolio(1024, 128, seed=2024)
+    g = hjm.portfolio_cva_greeks(m, pf, 65536, horizon=30.0)
+    ratio = g.timings_ms[2] / g.timings_ms[0]
+    _flush("  forward %.2f ms, adjoint %.2f ms, ratio %.2f"
+           % (g.timings_ms[0], g.timings_ms[1], ratio))
+    assert g.cva > 0 and np.all(np.isfinite(g.d_vol_total))
+    assert ratio <= 4.0
