@@ -32,14 +32,16 @@ sys.path.insert(0, ROOT)
 FLOPS_PER_PATH_STEP = 561          # 51 tenors x (1 sub + 4 FMA + 1 add + 1 accumulate)
 PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}   # MI355X published vector peaks
 PATHS, EXPIRY, DT, SEED = 131072, 10.0, 0.01, 7
+KERNEL_NAMES = {"matrix": "ref_paths_mfma_kernel", "vector": "ref_paths_kernel<float,compensated,philox>"}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_plan(hjm, market, swap, precision, steps, warmup):
-    plan = hjm.ReferencePlan(market, swap, PATHS, DT, precision=precision, seed=SEED)
+def run_plan(hjm, market, swap, precision, steps, warmup, path_kernel=0):
+    plan = hjm.ReferencePlan(market, swap, PATHS, DT, precision=precision, seed=SEED,
+                             path_kernel=path_kernel)
     for _ in range(warmup):
         plan.execute()
     kernel_ms = []
@@ -137,6 +139,8 @@ def main():
     ap.add_argument("--config", default="precision", choices=["precision", "batched", "full"])
     ap.add_argument("--no-baseline-form", action="store_true",
                     help="skip the reference-design (plain FP64) GPU run")
+    ap.add_argument("--path-kernel", default="matrix", choices=["matrix", "vector"],
+                    help="FP32 path kernel timed as the headline: matrix cores or VALU")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the path shards across GPUs by contiguous path "
@@ -165,15 +169,22 @@ def main():
 
     log("[bench] FP64 path kernel")
     f64 = run_plan(hjm, market, swap, hjm.FP64, max(2, args.steps // 2), 1)
-    log("[bench] FP32 + compensated path kernel (timed)")
+    other = "vector" if args.path_kernel == "matrix" else "matrix"
+    kernels = {"matrix": hjm.KERNEL_MATRIX, "vector": hjm.KERNEL_VECTOR}
+    log("[bench] FP32 + compensated, %s path kernel" % other)
+    alt = run_plan(hjm, market, swap, hjm.FP32_KAHAN, max(2, args.steps // 2), 1,
+                   kernels[other])
+    log("[bench] FP32 + compensated, %s path kernel (timed)" % args.path_kernel)
     sync()
-    f32 = run_plan(hjm, market, swap, hjm.FP32_KAHAN, args.steps, args.warmup)
+    f32 = run_plan(hjm, market, swap, hjm.FP32_KAHAN, args.steps, args.warmup,
+                   kernels[args.path_kernel])
     sync()
 
     value = PATHS * S * args.steps / f32["wall"]
     f64_rate = PATHS * S * f64["kernel_ms"].shape[0] / f64["wall"]
     k32 = float(np.mean(f32["kernel_ms"][:, 0]))
     k64 = float(np.mean(f64["kernel_ms"][:, 0]))
+    kalt = float(np.mean(alt["kernel_ms"][:, 0]))
     flops = FLOPS_PER_PATH_STEP * PATHS * evolved
     achieved = flops / (k32 * 1e-3) / 1e12
 
@@ -217,12 +228,17 @@ def main():
                                "integrals, EE/PFE97.5/CVA in FP64" % (PATHS, S),
                    "paths": PATHS, "steps_per_path": S, "tenors": 51, "factors": 3,
                    "parallelism": "single GPU, one path per lane"},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["f32"],
+        "roofline": {"bound": "mfma" if args.path_kernel == "matrix" else "valu",
+                     "achieved": achieved, "peak": PEAK_TFLOPS["f32"],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f32"],
                      "traffic": None,
-                     "kernel": "ref_paths_kernel<float,compensated,philox>",
+                     "kernel": KERNEL_NAMES[args.path_kernel],
                      "kernel_ms": k32,
                      "flops_per_launch": flops},
+        "fp32_" + other: {"kernel": KERNEL_NAMES[other], "kernel_ms": kalt,
+                          "path_steps_per_s": PATHS * S * alt["kernel_ms"].shape[0] / alt["wall"],
+                          "roofline_frac": flops / (kalt * 1e-3) / 1e12 / PEAK_TFLOPS["f32"],
+                          "cva_rel_vs_headline": abs(alt["cva"] / f32["cva"] - 1.0)},
         "cpu_baseline": cpu,
         "fp64": {"path_steps_per_s": f64_rate, "kernel_ms": k64,
                  "roofline_frac": flops / (k64 * 1e-3) / 1e12 / PEAK_TFLOPS["f64"],
