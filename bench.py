@@ -111,23 +111,25 @@ def run_portfolio(hjm, args, nswaps, nsets, paths):
     }
 
 
-def cpu_baseline(market, target_s=10.0):
-    """The oracle's reference loop (simulation_cpu.h:413-436) in FP64 on one thread, timed
-    around the path loop only, on chunks of the same workload until ~target_s."""
+def cpu_baseline(market, target_s=10.0, nthreads=1):
+    """The oracle's reference loop (simulation_cpu.h:413-436) in FP64 on `nthreads` threads
+    (OpenMP over paths), timed around the path loop only, on chunks of the same workload
+    until ~target_s."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     orc = oracle_lib.load()
     S = int(round(EXPIRY / DT))
-    chunk, done, ms = 1024, 0, 0.0
+    chunk, done, ms = 1024 * nthreads, 0, 0.0
     while ms < target_s * 1e3 and done < 64 * chunk:
         z = orc.normals(SEED, chunk, S, path0=done)
         t, _, _ = orc.time_reference_paths(market.spot_rates, market.drifts,
-                                           market.volatilities, EXPIRY, 0.5, DT, z, nthreads=1)
+                                           market.volatilities, EXPIRY, 0.5, DT, z,
+                                           nthreads=nthreads)
         ms += t
         done += chunk
-    return dict(value=done * S / (ms * 1e-3), unit="path-steps/s", cores=1, kind="port",
-                sample="%d of the %d paths x %d steps, FP64 reference loop, 1 thread, "
-                       "normals generated outside the timed region" % (done, PATHS, S))
+    return dict(value=done * S / (ms * 1e-3), unit="path-steps/s", cores=nthreads, kind="port",
+                sample="%d of the %d paths x %d steps, FP64 reference loop, %d thread(s), "
+                       "normals generated outside the timed region" % (done, PATHS, S, nthreads))
 
 
 def main():
@@ -141,6 +143,10 @@ def main():
                     help="skip the reference-design (plain FP64) GPU run")
     ap.add_argument("--path-kernel", default="matrix", choices=["matrix", "vector"],
                     help="FP32 path kernel timed as the headline: matrix cores or VALU")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="time only the headline kernel sequence (for counter runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run only a plain device copy of MIB MiB (calibrates HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the path shards across GPUs by contiguous path "
@@ -157,6 +163,17 @@ def main():
     except Exception:
         sync = lambda: None  # noqa: E731
 
+    if args.copy_calibration:
+        # torch's elementwise copy kernel: reads and writes exactly MIB MiB each
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        sync()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 3}), flush=True)
+        return
+
     if args.config != "precision":
         n = {"batched": (256, 32, 262144), "full": (1024, 128, 1048576)}[args.config]
         print(json.dumps(run_portfolio(hjm, args, *n)), flush=True)
@@ -167,10 +184,17 @@ def main():
     S = int(round(EXPIRY / DT))
     evolved = S - 50                           # block 0 is frozen at the initial curve
 
-    log("[bench] FP64 path kernel")
-    f64 = run_plan(hjm, market, swap, hjm.FP64, max(2, args.steps // 2), 1)
     other = "vector" if args.path_kernel == "matrix" else "matrix"
     kernels = {"matrix": hjm.KERNEL_MATRIX, "vector": hjm.KERNEL_VECTOR}
+    if args.headline_only:
+        f32 = run_plan(hjm, market, swap, hjm.FP32_KAHAN, args.steps, args.warmup,
+                       kernels[args.path_kernel])
+        print(json.dumps({"headline_only": True, "kernel": KERNEL_NAMES[args.path_kernel],
+                          "kernel_ms": float(np.mean(f32["kernel_ms"][:, 0])),
+                          "path_steps_per_s": PATHS * S * args.steps / f32["wall"]}), flush=True)
+        return
+    log("[bench] FP64 path kernel")
+    f64 = run_plan(hjm, market, swap, hjm.FP64, max(2, args.steps // 2), 1)
     log("[bench] FP32 + compensated, %s path kernel" % other)
     alt = run_plan(hjm, market, swap, hjm.FP32_KAHAN, max(2, args.steps // 2), 1,
                    kernels[other])
@@ -208,7 +232,11 @@ def main():
 
     log("[bench] adjoint configuration: CVA + 204 Greeks vs forward")
     adj = adjoint_ratio(hjm, market)
-    cpu = None if args.no_cpu else cpu_baseline(market)
+    cpu = cpu_all = None
+    if not args.no_cpu:
+        log("[bench] CPU oracle, 1 thread and all cores")
+        cpu = cpu_baseline(market)
+        cpu_all = cpu_baseline(market, nthreads=min(16, os.cpu_count() or 1))
     res = {
         "metric": "path-steps/s (51 tenors) FP32 vs FP64; CVA+204 Greeks time vs forward-only",
         "value": value,
@@ -240,6 +268,7 @@ def main():
                           "roofline_frac": flops / (kalt * 1e-3) / 1e12 / PEAK_TFLOPS["f32"],
                           "cva_rel_vs_headline": abs(alt["cva"] / f32["cva"] - 1.0)},
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
         "fp64": {"path_steps_per_s": f64_rate, "kernel_ms": k64,
                  "roofline_frac": flops / (k64 * 1e-3) / 1e12 / PEAK_TFLOPS["f64"],
                  "fp32_over_fp64": value / f64_rate},

@@ -139,7 +139,7 @@ def _declare(lib):
                                                          i, i, d]
     lib.hjm_calculate_exposure_multi_gpu_f32.argtypes = [fp, f, f, f, f, fp, fp, fp, fp, f, i, i,
                                                          f]
-    lib.hjm_block_map.argtypes = [_dp, _dp, d, d, fp]
+    lib.hjm_block_map.argtypes = [_dp, _dp, d, d, C.c_int32, fp]
     lib.hjm_generate_normals.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_int32, C.c_int32,
                                          _dp]
 
@@ -356,10 +356,13 @@ class ReferencePlan:
             pass
 
 
-def block_map(market: Market, dt: float = 0.01, dtau: float = 0.5) -> np.ndarray:
-    """The matrix-core kernel's affine 50-step block map, [102][220] float32."""
-    out = np.zeros((102, 220), np.float32)
+def block_map(market: Market, dt: float = 0.01, dtau: float = 0.5,
+              portfolio: bool = False) -> np.ndarray:
+    """A matrix-core path kernel's affine 50-step block map, float32 with 220 columns:
+    [102][220] for the reference semantics, [52][220] for the portfolio semantics."""
+    out = np.zeros((52 if portfolio else 102, 220), np.float32)
     _check(library().hjm_block_map(_ptr(market.drifts), _ptr(market.volatilities), dt, dtau,
+                                   int(bool(portfolio)),
                                    out.ctypes.data_as(C.POINTER(C.c_float))), "hjm_block_map")
     return out
 
@@ -440,7 +443,7 @@ class PortfolioPlan:
                  dt: float = 0.01, precision: int = FP32_KAHAN, seed: int = 2024,
                  normals: Optional[np.ndarray] = None, recovery: float = 0.4,
                  pfe_quantile: float = 0.975, tuning: Optional[Tuning] = None, device: int = 0,
-                 path_offset: int = 0):
+                 path_offset: int = 0, path_kernel: int = KERNEL_AUTO):
         self._lib = library()
         self._keep = (market, portfolio, normals)
         rng = RNG_PHILOX
@@ -451,7 +454,7 @@ class PortfolioPlan:
         mk = market._c()
         pfc = _pf_c(portfolio, horizon, recovery)
         cfg = _config(paths, dt, precision, rng, seed, normals, 0.0, recovery, pfe_quantile,
-                      device, path_offset)
+                      device, path_offset, path_kernel)
         tun = tuning._c() if tuning is not None else None
         self._h = C.c_void_p()
         _check(self._lib.hjm_portfolio_plan_create(C.byref(self._h), C.byref(mk), C.byref(pfc),
@@ -495,7 +498,9 @@ def simulate_portfolio(market: Market, portfolio, paths: int, keep_paths: bool =
 def autotune_portfolio(market: Market, portfolio, paths: int, horizon: float = 30.0,
                        dt: float = 0.01, precision: int = FP32_KAHAN, seed: int = 2024,
                        device: int = 0):
-    """Fastest (block size, persistent grid) for the path kernel; returns (Tuning, ms)."""
+    """Fastest (block size, persistent grid) for the VALU path kernel; returns (Tuning, ms).
+    The matrix-core path kernel (FP32, in-kernel normals, AUTO or MATRIX) has a fixed launch
+    shape and ignores the tuning."""
     mk = market._c()
     pfc = _pf_c(portfolio, horizon, 0.4)
     cfg = _config(paths, dt, precision, RNG_PHILOX, seed, None, 0.0, 0.4, 0.0, device, 0)
@@ -509,7 +514,8 @@ def autotune_portfolio(market: Market, portfolio, paths: int, horizon: float = 3
 def portfolio_cva_greeks(market: Market, portfolio, paths: int, horizon: float = 30.0,
                          dt: float = 0.01, precision: int = FP32_KAHAN, seed: int = 2024,
                          normals: Optional[np.ndarray] = None, recovery: float = 0.4,
-                         tuning: Optional[Tuning] = None, device: int = 0) -> Greeks:
+                         tuning: Optional[Tuning] = None, device: int = 0,
+                         path_kernel: int = KERNEL_AUTO) -> Greeks:
     """Total portfolio CVA and its 51 + 153 (+ 51 drift) pathwise-adjoint sensitivities."""
     rng = RNG_PHILOX
     if normals is not None:
@@ -517,7 +523,8 @@ def portfolio_cva_greeks(market: Market, portfolio, paths: int, horizon: float =
         rng = RNG_HOST
     mk = market._c()
     pfc = _pf_c(portfolio, horizon, recovery)
-    cfg = _config(paths, dt, precision, rng, seed, normals, 0.0, recovery, 0.0, device, 0)
+    cfg = _config(paths, dt, precision, rng, seed, normals, 0.0, recovery, 0.0, device, 0,
+                  path_kernel)
     tun = tuning._c() if tuning is not None else None
     cva, se = np.zeros(1), np.zeros(1)
     ds, dv, dm, dvt, tm = (np.zeros(TENORS), np.zeros(3 * TENORS), np.zeros(TENORS),

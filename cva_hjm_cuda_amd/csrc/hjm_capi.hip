@@ -165,6 +165,7 @@ int plan_init(hjm_plan p, const hjm_market* market, const hjm_swap* swap, const 
     HJM_TRY(dev_alloc(&p->d_sums, kStatSlots));
     HJM_TRY(dev_alloc(&p->d_pfe, kTenors));
     if (p->use_mfma) {
+        HJM_HIP(mfma_kernels_prepare(), HJM_ERR_DEVICE);
         std::vector<float> wf;
         build_mfma_block_map(market->drifts, market->volatilities, cfg->dt, swap->dtau, wf);
         HJM_TRY(dev_alloc(&p->d_wfrag, wf.size()));
@@ -588,19 +589,25 @@ int hjm_simulate_reference_blockwise(const hjm_market* market, const hjm_swap* s
     return HJM_OK;
 }
 
-int hjm_block_map(const double* drifts, const double* vols, double dt, double dtau, float* out)
+int hjm_block_map(const double* drifts, const double* vols, double dt, double dtau,
+                  int32_t portfolio, float* out)
 {
     if (!drifts || !vols || !out) return HJM_ERR_NULL;
+    if (portfolio != 0 && portfolio != 1) return HJM_ERR_ARG;
     if (!(dt > 0.0) || !(dtau > 0.0) || fabs(dtau / dt - kMfmaStride) > 1e-6 * kMfmaStride)
         return HJM_ERR_GRID;
     std::vector<float> wf;
-    build_mfma_block_map(drifts, vols, dt, dtau, wf);
+    build_mfma_block_map(drifts, vols, dt, dtau, wf,
+                         portfolio ? MfmaMap::kPortfolio : MfmaMap::kReference);
     const int K = 4 * kMfmaKChunks;
-    for (int m = 0; m < kMfmaMTiles; ++m)
+    const int mtiles = portfolio ? kMfmaPfMTiles : kMfmaMTiles;
+    const int rows = portfolio ? kTenors + 1 : 2 * kTenors;
+    for (int m = 0; m < mtiles; ++m)
         for (int kc = 0; kc < kMfmaKChunks; ++kc)
             for (int l = 0; l < kWave; ++l) {
                 const int row = 16 * m + (l % 16), kidx = 4 * kc + l / 16;
-                if (row < 2 * kTenors) out[(size_t)row * K + kidx] = wf[((size_t)m * kMfmaKChunks + kc) * kWave + l];
+                if (row < rows)
+                    out[(size_t)row * K + kidx] = wf[((size_t)m * kMfmaKChunks + kc) * kWave + l];
             }
     return HJM_OK;
 }

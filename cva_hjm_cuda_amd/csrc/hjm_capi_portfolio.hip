@@ -8,6 +8,7 @@
 
 #include "hjm_host.h"
 #include "hjm_kernels.h"
+#include "hjm_mfma_plan.h"
 
 using namespace hjm;
 using namespace hjm_host;
@@ -25,6 +26,9 @@ struct hjm_pplan_s {
     int nsets = 0, ndates = 0;
     int64_t pfe_rank = 1;
     int grid_blocks = 0;
+    bool use_mfma = false;
+    MfmaGrid mg{};
+    float* d_wfrag = nullptr;
     RefStepConsts<double> kd{};
     RefStepConsts<float> kf{};
     PfGrid grid{};
@@ -64,6 +68,7 @@ void pplan_release(hjm_portfolio_plan p)
     dev_free(p->d_a);
     dev_free(p->d_kmax);
     dev_free(p->d_w);
+    dev_free(p->d_wfrag);
     for (auto& e : p->ev)
         if (e) (void)hipEventDestroy(e);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -144,6 +149,11 @@ int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfol
     const int stride = (int)llround(rs);
     if (stride <= 0 || fabs(rs - stride) > 1e-6 * stride) return HJM_ERR_GRID;
     const int S = stride * (nd - 1);
+    if (cfg->path_kernel < HJM_KERNEL_AUTO || cfg->path_kernel > HJM_KERNEL_MATRIX) return HJM_ERR_ARG;
+    const bool mfma_ok = cfg->precision == HJM_FP32_KAHAN && cfg->rng == HJM_RNG_PHILOX &&
+                         stride == kMfmaStride;
+    if (cfg->path_kernel == HJM_KERNEL_MATRIX && !mfma_ok) return HJM_ERR_UNSUPPORTED;
+    p->use_mfma = mfma_ok && cfg->path_kernel != HJM_KERNEL_VECTOR;
 
     p->device = cfg->device;
     p->precision = cfg->precision;
@@ -231,6 +241,23 @@ int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfol
         HJM_HIP(hipMemcpy(p->d_normals, cfg->host_normals, n * sizeof(double), hipMemcpyHostToDevice),
                 HJM_ERR_DEVICE);
     }
+    if (p->use_mfma) {
+        HJM_HIP(mfma_kernels_prepare(), HJM_ERR_DEVICE);
+        std::vector<float> wf;
+        build_mfma_block_map(market->drifts, market->volatilities, cfg->dt, pf->dtau, wf,
+                             MfmaMap::kPortfolio);
+        HJM_TRY(dev_alloc(&p->d_wfrag, wf.size()));
+        HJM_HIP(hipMemcpy(p->d_wfrag, wf.data(), wf.size() * sizeof(float), hipMemcpyHostToDevice),
+                HJM_ERR_DEVICE);
+        MfmaGrid& mg = p->mg;
+        mg.ndates = nd;
+        mg.paths = cfg->paths;
+        mg.path0 = cfg->path_offset;
+        mg.seed_lo = g.seed_lo;
+        mg.seed_hi = g.seed_hi;
+        mg.dt = cfg->dt;
+        for (int j = 0; j < kTenors; ++j) mg.spot[j] = (float)market->spot_rates[j];
+    }
     return HJM_OK;
 }
 
@@ -241,10 +268,12 @@ int pplan_enqueue(hjm_portfolio_plan p)
     const bool f64 = p->precision == HJM_FP64;
     const int rows = p->nsets * p->ndates;
     HJM_HIP(hipEventRecord(p->ev[0], st), HJM_ERR_DEVICE);
-    hipError_t err = f64 ? launch_pf_paths_f64(p->kd, p->grid, p->d_normals, (double*)p->d_curves,
-                                               p->d_logb, p->tuning.block_size, p->grid_blocks, st)
-                         : launch_pf_paths_f32(p->kf, p->grid, p->d_normals, (float*)p->d_curves,
-                                               p->d_logb, p->tuning.block_size, p->grid_blocks, st);
+    hipError_t err =
+        p->use_mfma ? launch_pf_paths_mfma(p->mg, p->d_wfrag, (float*)p->d_curves, p->d_logb, st)
+        : f64 ? launch_pf_paths_f64(p->kd, p->grid, p->d_normals, (double*)p->d_curves, p->d_logb,
+                                    p->tuning.block_size, p->grid_blocks, st)
+              : launch_pf_paths_f32(p->kf, p->grid, p->d_normals, (float*)p->d_curves, p->d_logb,
+                                    p->tuning.block_size, p->grid_blocks, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
     HJM_HIP(hipEventRecord(p->ev[1], st), HJM_ERR_DEVICE);
     err = f64 ? launch_pf_value_f64(p->vp, (const double*)p->d_curves, p->d_logb,
@@ -502,6 +531,7 @@ int hjm_portfolio_autotune(const hjm_market* market, const hjm_portfolio* portfo
     hjm_config cfg = *config;
     cfg.pfe_quantile = 0.0;
     if (cfg.rng == HJM_RNG_HOST) cfg.rng = HJM_RNG_PHILOX;
+    cfg.path_kernel = HJM_KERNEL_VECTOR;  // the matrix-core kernel has a fixed launch shape
     double bestv = 1e300;
     hjm_tuning bt{256, 0, 1, 0};
     for (int bs : {64, 128, 256}) {

@@ -27,8 +27,13 @@ hipError_t launch_radix_select(const double* x, int rows, int64_t n, int64_t k, 
 hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nsteps, double* out,
                           hipStream_t st);
 
+// Matrix-core FP32 path kernels (50 steps per date); prepare once per plan (sets the
+// kernels' dynamic LDS limits) before launching or capturing them.
+hipError_t mfma_kernels_prepare();
 hipError_t launch_ref_paths_mfma(const MfmaGrid& g, const float* wfrag, double* nx, double* ny,
                                  hipStream_t st);
+hipError_t launch_pf_paths_mfma(const MfmaGrid& g, const float* wfrag, float* curves, double* logb,
+                                hipStream_t st);
 
 // Pathwise adjoint of the reference-semantics CVA.
 hipError_t launch_ref_adjoint_seed(const ExposureParams& e, double dt, const double* nx,

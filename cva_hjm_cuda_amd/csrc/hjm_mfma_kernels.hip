@@ -1,13 +1,17 @@
-// hjm_mfma_kernels.hip -- FP32 reference-semantics path kernel on the matrix cores.
+// hjm_mfma_kernels.hip -- FP32 path kernels on the matrix cores.
 //
-// Each 50-step block is the affine map [f_out; dA] = W [f_in; phi; 1] (hjm_mfma_plan.h).
-// A wave owns 32 paths = 2 N-tiles of 16; per block it issues 7 x 2 x 55
-// v_mfma_f32_16x16x4f32 (A = W fragments from LDS, B = the previous block's accumulators
-// or freshly generated shocks) while the VALU generates the next shocks. The accumulator
-// layout of a 16x16 tile -- lane l holds rows 4 (l / 16) + r, r = 0..3, of column l % 16 --
-// is used directly as the next block's B operand (W's columns are permuted to match), so
-// the curve never leaves registers. The integrals dA are Kahan-added into (A, C) per
-// block. One workgroup of 8 waves (2 per SIMD) per CU shares W (98 KB) in LDS.
+// Each 50-step block is the affine map [f_out; integral rows] = W [f_in; phi; 1]
+// (hjm_mfma_plan.h). A wave owns 32 paths = 2 N-tiles of 16; per block it issues
+// MT x 2 x 55 v_mfma_f32_16x16x4f32 (A = W fragments from LDS, B = the previous block's
+// accumulators or freshly generated shocks) while the VALU generates the next shocks. The
+// accumulator layout of a 16x16 tile -- lane l holds rows 4 (l / 16) + r, r = 0..3, of
+// column l % 16 -- is used directly as the next block's B operand (W's columns are
+// permuted to match), so the curve never leaves registers. One workgroup of 8 waves
+// shares W in LDS.
+//   ref_paths_mfma_kernel: reference semantics (7 M-tiles): the block integrals dA are
+//     Kahan-added into (A, C); numeraire slot b+1 is recorded after block b.
+//   pf_paths_mfma_kernel: portfolio semantics (4 M-tiles): the curve is checkpointed at
+//     every date and the short rate's left-point sum (row 51) extends the log bank account.
 #include "hjm_common.h"
 #include "hjm_kernels.h"
 #include "hjm_mfma_plan.h"
@@ -25,6 +29,65 @@ __device__ __forceinline__ f4_t mfma16(float a, float b, f4_t c)
 constexpr int kNT = 2;                       // N-tiles (16 paths each) per wave
 constexpr int kMfmaBlock = 512;              // 8 waves
 constexpr int kMfmaPathsPerBlock = kMfmaBlock / kWave * kNT * 16;
+
+// One 50-step block: acc = W x [fin; shocks of steps b*50 .. b*50+49; 1].
+template <int MT>
+__device__ __forceinline__ void mfma_block(const float* __restrict__ lds_w, const float (&fin)[kNT][16],
+                                           f4_t (&acc)[MT][kNT], const MfmaGrid& g,
+                                           const int64_t (&path)[kNT], int b, int grp, int lane)
+{
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < kNT; ++n) acc[m][n] = f4_t{0.f, 0.f, 0.f, 0.f};
+    // the previous block's curve
+#pragma unroll
+    for (int kc = 0; kc < kMfmaFChunks; ++kc) {
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const float a = lds_w[(m * kMfmaKChunks + kc) * kWave + lane];
+#pragma unroll
+            for (int n = 0; n < kNT; ++n) acc[m][n] = mfma16(a, fin[n][kc], acc[m][n]);
+        }
+        // keep the fragment loads of later chunks from being hoisted (register pressure)
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // shocks: step 4 q + grp of this block, three factors per step; generated one quad
+    // ahead so the VALU works on quad q+1 while the matrix cores consume quad q. Lanes
+    // past the block's last step hold 0, except the constant 1 of the drift.
+    float zc[kNT][3], zn[kNT][3];
+    auto gen = [&](int q, float (&z)[kNT][3]) {
+        const int t = 4 * q + grp;
+        const int tc = t < kMfmaStride ? t : kMfmaStride - 1;
+#pragma unroll
+        for (int n = 0; n < kNT; ++n) {
+            const Normals3 v = step_normals(g.seed_lo, g.seed_hi, (uint32_t)(g.path0 + path[n]),
+                                            (uint32_t)(b * kMfmaStride + tc));
+            const bool in = t < kMfmaStride;
+            z[n][0] = in ? v.z0 : (t == kMfmaStride ? 1.f : 0.f);
+            z[n][1] = in ? v.z1 : 0.f;
+            z[n][2] = in ? v.z2 : 0.f;
+        }
+    };
+    gen(0, zc);
+    for (int q = 0; q < kMfmaQuads; ++q) {
+        gen(q + 1 < kMfmaQuads ? q + 1 : q, zn);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int kc = kMfmaFChunks + 3 * q + k;
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                const float a = lds_w[(m * kMfmaKChunks + kc) * kWave + lane];
+#pragma unroll
+                for (int n = 0; n < kNT; ++n) acc[m][n] = mfma16(a, zc[n][k], acc[m][n]);
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < kNT; ++n)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) zc[n][k] = zn[n][k];
+    }
+}
 
 __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGrid g,
                                                                 const float* __restrict__ wfrag,
@@ -47,7 +110,7 @@ __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGr
         live[n] = path[n] < P;
     }
 
-    // curve rows 16 m + 4 grp + r of the four N-tiles (B operand of the F chunks)
+    // curve rows 16 m + 4 grp + r of the N-tiles (B operand of the F chunks)
     float fin[kNT][16];
     // Kahan (A, C) of the rows of tiles 3..6 (dA_j at row 51 + j)
     float Ak[kNT][16], Ck[kNT][16];
@@ -82,57 +145,7 @@ __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGr
 
     for (int b = 1; b < g.ndates; ++b) {
         f4_t acc[kMfmaMTiles][kNT];
-#pragma unroll
-        for (int m = 0; m < kMfmaMTiles; ++m)
-#pragma unroll
-            for (int n = 0; n < kNT; ++n) acc[m][n] = f4_t{0.f, 0.f, 0.f, 0.f};
-        // the previous block's curve
-#pragma unroll
-        for (int kc = 0; kc < kMfmaFChunks; ++kc) {
-#pragma unroll
-            for (int m = 0; m < kMfmaMTiles; ++m) {
-                const float a = lds_w[(m * kMfmaKChunks + kc) * kWave + lane];
-#pragma unroll
-                for (int n = 0; n < kNT; ++n) acc[m][n] = mfma16(a, fin[n][kc], acc[m][n]);
-            }
-            // keep the fragment loads of later chunks from being hoisted (register pressure)
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        // shocks: step 4 q + grp of this block, three factors per step; generated one
-        // quad ahead so the VALU works on quad q+1 while the matrix cores consume quad q.
-        // Lanes past the block's last step hold 0, except the constant 1 of the drift.
-        float zc[kNT][3], zn[kNT][3];
-        auto gen = [&](int q, float (&z)[kNT][3]) {
-            const int t = 4 * q + grp;
-            const int tc = t < kMfmaStride ? t : kMfmaStride - 1;
-#pragma unroll
-            for (int n = 0; n < kNT; ++n) {
-                const Normals3 v = step_normals(g.seed_lo, g.seed_hi, (uint32_t)(g.path0 + path[n]),
-                                                (uint32_t)(b * kMfmaStride + tc));
-                const bool in = t < kMfmaStride;
-                z[n][0] = in ? v.z0 : (t == kMfmaStride ? 1.f : 0.f);
-                z[n][1] = in ? v.z1 : 0.f;
-                z[n][2] = in ? v.z2 : 0.f;
-            }
-        };
-        gen(0, zc);
-        for (int q = 0; q < kMfmaQuads; ++q) {
-            gen(q + 1 < kMfmaQuads ? q + 1 : q, zn);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const int kc = kMfmaFChunks + 3 * q + k;
-#pragma unroll
-                for (int m = 0; m < kMfmaMTiles; ++m) {
-                    const float a = lds_w[(m * kMfmaKChunks + kc) * kWave + lane];
-#pragma unroll
-                    for (int n = 0; n < kNT; ++n) acc[m][n] = mfma16(a, zc[n][k], acc[m][n]);
-                }
-            }
-#pragma unroll
-            for (int n = 0; n < kNT; ++n)
-#pragma unroll
-                for (int k = 0; k < 3; ++k) zc[n][k] = zn[n][k];
-        }
+        mfma_block<kMfmaMTiles>(lds_w, fin, acc, g, path, b, grp, lane);
         // integrals: Kahan-add the block's dA (tiles 3..6); curve rows become next input
 #pragma unroll
         for (int n = 0; n < kNT; ++n) {
@@ -176,19 +189,94 @@ __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGr
     }
 }
 
+// Portfolio semantics: dates e = 0 .. ndates-1 every 50 steps from t = 0. At each date the
+// curve goes to curves[e][tenor][path] and the log bank account dt sum_{n < 50 e} f_n(0)
+// to logb[e][path]; the block's row 51 (left-point sum of the short rate) extends it.
+__global__ __launch_bounds__(kMfmaBlock) void pf_paths_mfma_kernel(const MfmaGrid g,
+                                                               const float* __restrict__ wfrag,
+                                                               float* __restrict__ curves,
+                                                               double* __restrict__ logb)
+{
+    extern __shared__ float lds_w[];
+    for (int t = threadIdx.x; t < kMfmaPfFragFloats; t += kMfmaBlock) lds_w[t] = wfrag[t];
+    __syncthreads();
+
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const int grp = lane >> 4, col = lane & 15;
+    const int64_t P = g.paths;
+    const int64_t base = (int64_t)blockIdx.x * kMfmaPathsPerBlock + (int64_t)wv * kNT * 16;
+    int64_t path[kNT];
+    bool live[kNT];
+    float fin[kNT][16];
+    double lnB[kNT];
+#pragma unroll
+    for (int n = 0; n < kNT; ++n) {
+        path[n] = base + 16 * n + col;
+        live[n] = path[n] < P;
+        lnB[n] = 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int R = 16 * m + 4 * grp + r;
+                fin[n][4 * m + r] = (R < kTenors) ? g.spot[R < kTenors ? R : 0] : 0.f;
+            }
+    }
+    // row 51 (the short-rate sum) is register 4 * 3 + 3 of lane group 0
+    constexpr int kSumReg = 4 * (kTenors / 16) + kTenors % 4;
+    const bool sum_lane = grp == (kTenors % 16) / 4;
+    for (int e = 0;; ++e) {
+#pragma unroll
+        for (int n = 0; n < kNT; ++n) {
+            if (!live[n]) continue;
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int R = 16 * m + 4 * grp + r;
+                    if (R < kTenors) curves[((int64_t)e * kTenors + R) * P + path[n]] = fin[n][4 * m + r];
+                }
+            if (sum_lane) logb[(int64_t)e * P + path[n]] = lnB[n];
+        }
+        if (e + 1 >= g.ndates) break;
+        f4_t acc[kMfmaPfMTiles][kNT];
+        mfma_block<kMfmaPfMTiles>(lds_w, fin, acc, g, path, e, grp, lane);
+#pragma unroll
+        for (int n = 0; n < kNT; ++n) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) fin[n][4 * m + r] = acc[m][n][r];
+            lnB[n] += (double)fin[n][kSumReg] * g.dt;
+        }
+    }
+}
+
+hipError_t mfma_kernels_prepare()
+{
+    hipError_t e = hipFuncSetAttribute((const void*)ref_paths_mfma_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)(sizeof(float) * kMfmaFragFloats));
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void*)pf_paths_mfma_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(sizeof(float) * kMfmaPfFragFloats));
+}
+
 hipError_t launch_ref_paths_mfma(const MfmaGrid& g, const float* wfrag, double* nx, double* ny,
                                  hipStream_t st)
 {
     const dim3 grid((unsigned)((g.paths + kMfmaPathsPerBlock - 1) / kMfmaPathsPerBlock));
-    const size_t lds = sizeof(float) * kMfmaFragFloats;
-    static bool attr_set = false;
-    if (!attr_set) {
-        const hipError_t e = hipFuncSetAttribute((const void*)ref_paths_mfma_kernel,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        attr_set = true;
-    }
-    ref_paths_mfma_kernel<<<grid, kMfmaBlock, lds, st>>>(g, wfrag, nx, ny);
+    ref_paths_mfma_kernel<<<grid, kMfmaBlock, sizeof(float) * kMfmaFragFloats, st>>>(g, wfrag, nx, ny);
+    return hipGetLastError();
+}
+
+hipError_t launch_pf_paths_mfma(const MfmaGrid& g, const float* wfrag, float* curves, double* logb,
+                                hipStream_t st)
+{
+    const dim3 grid((unsigned)((g.paths + kMfmaPathsPerBlock - 1) / kMfmaPathsPerBlock));
+    pf_paths_mfma_kernel<<<grid, kMfmaBlock, sizeof(float) * kMfmaPfFragFloats, st>>>(g, wfrag, curves,
+                                                                                      logb);
     return hipGetLastError();
 }
 

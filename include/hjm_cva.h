@@ -83,7 +83,7 @@ typedef struct {
     double recovery;        /* recovery rate for CVA */
     double pfe_quantile;    /* e.g. 0.975; <= 0 skips the PFE selection */
     int32_t device;         /* HIP device ordinal */
-    int32_t path_kernel;    /* enum hjm_path_kernel (reference semantics only) */
+    int32_t path_kernel;    /* enum hjm_path_kernel (reference and portfolio path kernels) */
 } hjm_config;
 
 /* Outputs; every pointer may be NULL (then that output is not produced / copied). */
@@ -171,6 +171,9 @@ typedef struct {
     double dtau;                  /* date spacing = payment period (0.5) */
 } hjm_portfolio;
 
+/* Launch shape of the VALU path kernel (FP64, host normals, or path_kernel = VECTOR); the
+ * matrix-core FP32 path kernel has a fixed shape (512 threads, 256 paths per workgroup)
+ * and uses only use_graph. */
 typedef struct {
     int32_t block_size;     /* threads per workgroup of the path kernel: 64, 128 or 256 */
     int32_t blocks_per_cu;  /* persistent path kernel: grid = blocks_per_cu x CUs;
@@ -227,11 +230,13 @@ int hjm_calculate_exposure_multi_gpu_f32(float* expected_exposure, float notiona
                                          const float* volatilities, float scale,
                                          int num_gpus, int scenarios, float dt);
 
-/* The affine 50-step block map of the matrix-core kernel as a dense [102][220] FP32
- * matrix (rows: curve after the block, then its 50-step integral increment; columns:
- * K order of the kernel), for inspection and tests. Host only. */
+/* The affine 50-step block map of a matrix-core path kernel as a dense FP32 matrix with
+ * 220 columns in the kernel's K order, for inspection and tests. Host only.
+ * portfolio = 0: reference semantics, [102][220] (rows: curve after the block, then the
+ * block's discount-integral increments); portfolio = 1: [52][220] (rows: curve after the
+ * block, then the short rate's left-point sum over the block's 50 steps). */
 int hjm_block_map(const double* drifts, const double* volatilities, double dt, double dtau,
-                  float* out);
+                  int32_t portfolio, float* out);
 
 /* ---- normals (for tests and host-supplied runs): the in-kernel generator's output
  * for paths [path_offset, path_offset + paths) and steps [0, nsteps), written as

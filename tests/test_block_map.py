@@ -1,9 +1,11 @@
-"""The matrix-core kernel's 50-step block map (host side, no GPU).
+"""The matrix-core kernels' 50-step block maps (host side, no GPU).
 
-W maps [f_in; phi; 1] to [f after 50 Euler steps; sum of the 50 curves], in the K order the
-kernel feeds the matrix cores. Pinned two ways: against 50 explicit FP64 Euler steps of the
-Musiela SDE written out here (random curve and shocks), and chained over the blocks of a
-5-year run against the oracle's reference paths (num_x, num_y) on the same normals.
+Reference semantics: W maps [f_in; phi; 1] to [f after 50 Euler steps; sum of the 50
+curves after each step], in the K order the kernel feeds the matrix cores. Pinned two ways:
+against 50 explicit FP64 Euler steps of the Musiela SDE written out here (random curve and
+shocks), and chained over the blocks of a 5-year run against the oracle's reference paths
+(num_x, num_y) on the same normals. Portfolio semantics: [f after 50 steps; left-point sum
+of the short rate f(., 0) over the 50 steps], against the same explicit steps.
 """
 import numpy as np
 import pytest
@@ -33,9 +35,11 @@ def _k_vector(f_in, phi):
     return x
 
 
-def _euler(f, drift, vols, phi, dt, dtau):
+def _euler(f, drift, vols, phi, dt, dtau, left=None):
     area = np.zeros(TENORS)
     for t in range(STRIDE):
+        if left is not None:
+            left[0] += f[0]
         d = np.empty(TENORS)
         d[:-1] = f[1:] - f[:-1]
         d[-1] = f[-1] - f[-2]
@@ -85,6 +89,35 @@ def test_block_map_chain_matches_oracle_paths(market, oracle):
             assert abs(np.exp(-A[b + 1] * dt) / ny[p, b + 1] - 1) < 1e-7, (p, b)
 
 
+def test_portfolio_block_map_equals_fifty_euler_steps(market):
+    W = hjm.block_map(market, portfolio=True).astype(np.float64)
+    assert W.shape == (52, 220)
+    rng = np.random.default_rng(9)
+    vols = market.volatilities.reshape(3, TENORS)
+    for _ in range(3):
+        f_in = 0.03 + 0.01 * rng.standard_normal(TENORS)
+        phi = rng.standard_normal((STRIDE, 3))
+        left = [0.0]
+        f_out, _ = _euler(f_in, market.drifts, vols, phi, 0.01, 0.5, left)
+        y = W @ _k_vector(f_in, phi)
+        np.testing.assert_allclose(y[:TENORS], f_out, rtol=0, atol=2e-8)
+        assert abs(y[TENORS] / left[0] - 1) < 2e-7
+    # the curve rows are the reference map's
+    Wr = hjm.block_map(market).astype(np.float64)
+    assert np.array_equal(W[:TENORS], Wr[:TENORS])
+    # a 30y chain (60 blocks) against explicit steps: curve and log bank account
+    f, fe, lnb, lnb_e = market.spot_rates.copy(), market.spot_rates.copy(), 0.0, [0.0]
+    for b in range(60):
+        phi = rng.standard_normal((STRIDE, 3))
+        y = W @ _k_vector(f, phi)
+        f, lnb = y[:TENORS], lnb + 0.01 * y[TENORS]
+        fe, _ = _euler(fe, market.drifts, vols, phi, 0.01, 0.5, lnb_e)
+    assert np.max(np.abs(f - fe)) < 1e-7
+    assert abs(lnb / (0.01 * lnb_e[0]) - 1) < 1e-6
+
+
 def test_block_map_rejects_other_grids(market):
     with pytest.raises(hjm.HJMError):
         hjm.block_map(market, dt=0.02)
+    with pytest.raises(hjm.HJMError):
+        hjm.block_map(market, dt=0.02, portfolio=True)

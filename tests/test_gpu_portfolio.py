@@ -74,6 +74,22 @@ def test_portfolio_in_kernel_rng_equals_host_normals(small):
     assert np.array_equal(a.cva, b.cva) and np.array_equal(a.pfe, b.pfe)
 
 
+def test_portfolio_matrix_kernel_matches_oracle(small):
+    _flush("[gpu] portfolio matrix-core FP32 path kernel vs FP64 oracle, 128 paths x 30y")
+    m, pf, z, ex_o = small
+    r = hjm.simulate_portfolio(m, pf, 128, horizon=30.0, precision=hjm.FP32_KAHAN, seed=31,
+                               keep_paths=True, path_kernel=hjm.KERNEL_MATRIX)
+    v = hjm.simulate_portfolio(m, pf, 128, horizon=30.0, precision=hjm.FP32_KAHAN, seed=31,
+                               keep_paths=True, path_kernel=hjm.KERNEL_VECTOR)
+    scale = np.max(np.abs(ex_o))
+    assert np.max(np.abs(r.exposures - ex_o)) <= 1e-4 * scale
+    assert np.max(np.abs(v.exposures - ex_o)) <= 1e-4 * scale
+    assert np.all(r.exposures[:, :, 0] == r.exposures[0:1, :, 0])
+    with pytest.raises(hjm.HJMError):
+        hjm.simulate_portfolio(m, pf, 128, horizon=30.0, precision=hjm.FP64, seed=31,
+                               path_kernel=hjm.KERNEL_MATRIX)
+
+
 def test_graph_replays_and_tuned_grids_are_bit_identical():
     _flush("[gpu] hipGraph replays vs eager, tuned grids")
     m = hjm.benchmark_market()
@@ -88,9 +104,14 @@ def test_graph_replays_and_tuned_grids_are_bit_identical():
         # unrelated eager work on the device between replays
         hjm.simulate_reference(m, hjm.InterestRateSwap(1e6, 0.02, 5.0), 512, seed=rep)
     plan.close()
+    # persistent grids of the VALU path kernel
+    eager_v = hjm.simulate_portfolio(m, pf, 20000, horizon=15.0, seed=9,
+                                     path_kernel=hjm.KERNEL_VECTOR)
     for t in (hjm.Tuning(64, 1), hjm.Tuning(128, 4), hjm.Tuning(256, 2, use_graph=True)):
-        r = hjm.simulate_portfolio(m, pf, 20000, horizon=15.0, seed=9, tuning=t)
-        assert np.array_equal(r.expected_exposure, eager.expected_exposure), t
+        r = hjm.simulate_portfolio(m, pf, 20000, horizon=15.0, seed=9, tuning=t,
+                                   path_kernel=hjm.KERNEL_VECTOR)
+        assert np.array_equal(r.expected_exposure, eager_v.expected_exposure), t
+    assert np.max(np.abs(eager_v.cva / eager.cva - 1)) < 1e-4
     tune, ms = hjm.autotune_portfolio(m, pf, 20000, horizon=15.0, seed=9)
     assert tune.block_size in (64, 128, 256) and ms > 0
 
