@@ -1,7 +1,8 @@
 """Build libhjm_cva.so (HIP, gfx950) in-tree with hipcc.
 
 Usage: python -m cva_hjm_cuda_amd.build [--verbose]
-Output: cva_hjm_cuda_amd/lib/libhjm_cva.so
+Output: cva_hjm_cuda_amd/lib/libhjm_cva.so and the command-line driver
+cva_hjm_cuda_amd/lib/hjm_cli (host C++ over the library, rpath $ORIGIN).
 """
 import os
 import subprocess
@@ -12,6 +13,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libhjm_cva.so")
+CLI = os.path.join(LIBDIR, "hjm_cli")
 SOURCES = ["hjm_capi.hip", "hjm_ref_kernels.hip", "hjm_baseline_kernels.hip",
            "hjm_adjoint_kernels.hip", "hjm_portfolio_kernels.hip",
            "hjm_capi_portfolio.hip", "hjm_mfma_kernels.hip"]
@@ -46,7 +48,23 @@ def build(verbose=False, extra=()):
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed with exit code %d" % res.returncode)
     os.replace(tmp, LIB)
+    build_cli(verbose)
     return LIB
+
+
+def build_cli(verbose=False):
+    tmp = CLI + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-Wextra",
+           "-I" + os.path.join(ROOT, "include"), "-o", tmp, os.path.join(CSRC, "hjm_cli.cpp"),
+           "-L" + LIBDIR, "-lhjm_cva", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ (driver) failed with exit code %d" % res.returncode)
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":
