@@ -80,6 +80,18 @@ class _PfResults(C.Structure):
                 ("exposures", _dp), ("timings_ms", _dp)]
 
 
+class _Calibration(C.Structure):
+    _fields_ = [("curves", _dp), ("ncurves", C.c_int64), ("window", C.c_int64),
+                ("step", C.c_int64), ("nwindows", C.c_int32), ("nfactors", C.c_int32),
+                ("obs_per_year", C.c_double), ("dtau", C.c_double),
+                ("fit_degrees", C.POINTER(C.c_int32)), ("device", C.c_int32)]
+
+
+class _CalibResults(C.Structure):
+    _fields_ = [("eigenvalues", _dp), ("volatilities", _dp), ("drifts", _dp),
+                ("raw_loadings", _dp), ("timings_ms", _dp)]
+
+
 class _Results(C.Structure):
     _fields_ = [("expected_exposure", _dp), ("pfe", _dp), ("cva", _dp), ("cva_stderr", _dp),
                 ("sums", _dp), ("num_x", _dp), ("num_y", _dp), ("exposures", _dp),
@@ -140,6 +152,7 @@ def _declare(lib):
     lib.hjm_calculate_exposure_multi_gpu_f32.argtypes = [fp, f, f, f, f, fp, fp, fp, fp, f, i, i,
                                                          f]
     lib.hjm_block_map.argtypes = [_dp, _dp, d, d, C.c_int32, fp]
+    lib.hjm_calibrate_pca.argtypes = [C.POINTER(_Calibration), C.POINTER(_CalibResults)]
     lib.hjm_generate_normals.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_int32, C.c_int32,
                                          _dp]
 
@@ -534,3 +547,46 @@ def portfolio_cva_greeks(market: Market, portfolio, paths: int, horizon: float =
                                               C.byref(tun) if tun is not None else None,
                                               C.byref(g)), "hjm_portfolio_cva_greeks")
     return Greeks(float(cva[0]), float(se[0]), ds, dv, dm, dvt, tm)
+
+
+# ---- calibration upstream of the path -------------------------------------------------
+@dataclass
+class Calibration:
+    eigenvalues: np.ndarray    # [windows][51] annualised covariance eigenvalues, descending
+    volatilities: np.ndarray   # [windows][153] loadings (factor-major), fitted where asked
+    drifts: np.ndarray         # [windows][51] HJM drift of the loadings
+    raw_loadings: np.ndarray   # [windows][153] sqrt(lambda_k) v_k before fitting
+    timings_ms: np.ndarray     # [1] calibration kernel
+
+    def market(self, spot_rates, accrual, window: int = -1) -> Market:
+        """A Market from one window's loadings and drift and the given curve/accrual."""
+        return Market(np.asarray(spot_rates, np.float64), self.drifts[window],
+                      self.volatilities[window], np.asarray(accrual, np.float64))
+
+
+def calibrate_pca(curves, window: Optional[int] = None, step: int = 1, nwindows: int = 1,
+                  obs_per_year: float = 252.0, fit_degrees=None, dtau: float = 0.5,
+                  device: int = 0) -> Calibration:
+    """Rolling-window PCA calibration of the three HJM loadings from historical forward
+    curves [ncurves][51] (oldest first) on the GPU; ``window`` defaults to all curves.
+    ``fit_degrees`` (3 ints, < 0 = raw) smooths each loading by a least-squares polynomial
+    in the tenor, e.g. (0, 2, 3) for the benchmark market's parametric forms."""
+    c = np.ascontiguousarray(curves, np.float64)
+    if c.ndim != 2 or c.shape[1] != TENORS:
+        raise ValueError("curves must have shape (ncurves, 51)")
+    window = c.shape[0] if window is None else int(window)
+    deg = None
+    if fit_degrees is not None:
+        deg = np.ascontiguousarray(fit_degrees, np.int32)
+        if deg.shape != (3,):
+            raise ValueError("fit_degrees needs one degree per factor (3)")
+    W = int(nwindows)
+    ev, vols, dr, raw, tm = (np.zeros((W, TENORS)), np.zeros((W, 3 * TENORS)),
+                             np.zeros((W, TENORS)), np.zeros((W, 3 * TENORS)), np.zeros(1))
+    cal = _Calibration(_ptr(c), c.shape[0], window, int(step), W, 3, float(obs_per_year),
+                       float(dtau),
+                       deg.ctypes.data_as(C.POINTER(C.c_int32)) if deg is not None else None,
+                       int(device))
+    res = _CalibResults(_ptr(ev), _ptr(vols), _ptr(dr), _ptr(raw), _ptr(tm))
+    _check(library().hjm_calibrate_pca(C.byref(cal), C.byref(res)), "hjm_calibrate_pca")
+    return Calibration(ev, vols, dr, raw, tm)

@@ -238,6 +238,36 @@ int hjm_calculate_exposure_multi_gpu_f32(float* expected_exposure, float notiona
 int hjm_block_map(const double* drifts, const double* volatilities, double dt, double dtau,
                   int32_t portfolio, float* out);
 
+/* ---- calibration upstream of the path (no code in the reference; its roadmap,
+ * doc/TODO:16, names the volatility and drift calibration of the HJM model) ----------
+ * Batched rolling-window PCA of historical forward curves: per window the annualised
+ * sample covariance of the daily curve differences, its eigendecomposition (GPU, FP64,
+ * cyclic Jacobi), loadings sigma_k = sqrt(lambda_k) v_k of the 3 leading factors (sign:
+ * components sum > 0), optionally least-squares polynomials in the tenor, and their HJM
+ * drift (hjm_hjm_drift). The outputs feed hjm_market directly. */
+typedef struct {
+    const double* curves;       /* [ncurves][51] forward curves f(t_d, tau_j), oldest first */
+    int64_t ncurves;
+    int64_t window;             /* curves per window (>= 3; window - 1 differences) */
+    int64_t step;               /* curves between consecutive window starts (>= 1) */
+    int32_t nwindows;           /* (nwindows - 1) * step + window <= ncurves */
+    int32_t nfactors;           /* 3: the path kernels' factor count */
+    double obs_per_year;        /* observations per year, e.g. 252 */
+    double dtau;                /* tenor spacing of the curves (drift quadrature) */
+    const int32_t* fit_degrees; /* [nfactors] polynomial degree (0..5) in the tenor per
+                                   loading, < 0 for the raw loading; NULL: all raw */
+    int32_t device;
+} hjm_calibration;
+
+typedef struct {
+    double* eigenvalues;   /* [nwindows][51] covariance eigenvalues, descending */
+    double* volatilities;  /* [nwindows][3*51] loadings, factor-major, fitted where asked */
+    double* drifts;        /* [nwindows][51] HJM drift of those loadings */
+    double* raw_loadings;  /* [nwindows][3*51] sqrt(lambda_k) v_k before fitting, or NULL */
+    double* timings_ms;    /* [1] calibration kernel time, or NULL */
+} hjm_calibration_results;
+int hjm_calibrate_pca(const hjm_calibration* calibration, hjm_calibration_results* results);
+
 /* ---- normals (for tests and host-supplied runs): the in-kernel generator's output
  * for paths [path_offset, path_offset + paths) and steps [0, nsteps), written as
  * [paths][nsteps][3] doubles. Runs on the GPU.                                       */
