@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 FLOPS_PER_PATH_STEP = 561          # 51 tenors x (1 sub + 4 FMA + 1 add + 1 accumulate)
 PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}   # MI355X published vector peaks
 PATHS, EXPIRY, DT, SEED = 131072, 10.0, 0.01, 7
+TENORS_N = 51
 KERNEL_NAMES = {"matrix": "ref_paths_mfma_kernel", "vector": "ref_paths_kernel<float,compensated,philox>"}
 
 
@@ -111,6 +112,47 @@ def run_portfolio(hjm, args, nswaps, nsets, paths):
     }
 
 
+def run_calibration(hjm, args, ncurves=2600, window=500):
+    """Rolling-window PCA calibration: every window of 500 daily curves (step 1) of a
+    2600-day synthetic history (rank-3 benchmark-loading differences plus noise)."""
+    market = hjm.benchmark_market()
+    sigma = market.volatilities.reshape(3, TENORS_N)
+    curves = hjm.synthetic_history(sigma, ncurves, seed=21, noise=2e-5, base=market.spot_rates)
+    nw = ncurves - window + 1
+    deg = (0, 2, 3)
+    for _ in range(args.warmup):
+        hjm.calibrate_pca(curves, window=window, step=1, nwindows=nw, fit_degrees=deg)
+    t0 = time.perf_counter()
+    ks = []
+    for _ in range(args.steps):
+        ks.append(hjm.calibrate_pca(curves, window=window, step=1, nwindows=nw,
+                                    fit_degrees=deg).timings_ms[0])
+    wall = time.perf_counter() - t0
+    kms = float(np.mean(ks))
+    # CPU baseline: the numpy oracle (LAPACK eigh), one window at a time, ~10 s sample
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import calibration as cal
+    done, t_cpu = 0, 0.0
+    while t_cpu < 10.0 and done < nw:
+        t1 = time.perf_counter()
+        cal.pca_window(curves[done:done + window])
+        t_cpu += time.perf_counter() - t1
+        done += 1
+    return {
+        "metric": "calibration windows/s (rolling PCA of 500 daily curves, 51 tenors)",
+        "value": nw * args.steps / wall, "unit": "windows/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic history (rank-3 benchmark loadings + noise, seed 21)",
+        "config": {"workload": "calibration: %d windows x %d curves, step 1, fits (0, 2, 3)"
+                               % (nw, window), "windows": nw, "window": window},
+        "kernel_ms": kms, "kernel_windows_per_s": nw / (kms * 1e-3),
+        "cpu_baseline": {"value": done / t_cpu, "unit": "windows/s", "cores": 1,
+                         "kind": "port",
+                         "sample": "%d windows, numpy cov + LAPACK eigh per window" % done},
+    }
+
+
 def cpu_baseline(market, target_s=10.0, nthreads=1):
     """The oracle's reference loop (simulation_cpu.h:413-436) in FP64 on `nthreads` threads
     (OpenMP over paths), timed around the path loop only, on chunks of the same workload
@@ -138,7 +180,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--config", default="precision", choices=["precision", "batched", "full"])
+    ap.add_argument("--config", default="precision",
+                    choices=["precision", "batched", "full", "calibration"])
     ap.add_argument("--no-baseline-form", action="store_true",
                     help="skip the reference-design (plain FP64) GPU run")
     ap.add_argument("--path-kernel", default="matrix", choices=["matrix", "vector"],
@@ -174,6 +217,9 @@ def main():
         print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 3}), flush=True)
         return
 
+    if args.config == "calibration":
+        print(json.dumps(run_calibration(hjm, args)), flush=True)
+        return
     if args.config != "precision":
         n = {"batched": (256, 32, 262144), "full": (1024, 128, 1048576)}[args.config]
         print(json.dumps(run_portfolio(hjm, args, *n)), flush=True)

@@ -57,20 +57,3 @@ def calibrate(curves, window, step, nwindows, obs_per_year, fit_degrees, dtau, d
         vols.append(v.reshape(-1))
         drifts.append(drift_fn(v.reshape(-1), dtau))
     return np.array(ev), np.array(vols), np.array(drifts), np.array(raws)
-
-
-def rank3_history(sigma, n, obs_per_year=252.0, base=None, seed=0, noise=0.0):
-    """Curves whose daily differences have sample covariance exactly
-    sigma^T sigma / obs_per_year (sigma: [3][51]) when noise = 0: differences
-    Z sigma with Z centred and Z^T Z = (n - 2) / obs_per_year * I over n - 1 days."""
-    rng = np.random.default_rng(seed)
-    m = n - 1
-    Z = rng.standard_normal((m, 3))
-    Z -= Z.mean(axis=0)
-    q, _ = np.linalg.qr(Z)  # orthonormal columns in the span of centred ones: centred
-    Z = q * np.sqrt((m - 1) / obs_per_year)
-    d = Z @ np.asarray(sigma).reshape(3, TENORS)
-    if noise:
-        d = d + noise * rng.standard_normal(d.shape)
-    f0 = np.full(TENORS, 0.03) if base is None else np.asarray(base, np.float64)
-    return np.vstack([f0, f0 + np.cumsum(d, axis=0)])

@@ -28,7 +28,7 @@ def _flush(msg):
 @pytest.fixture(scope="module")
 def history():
     sigma = hjm.benchmark_market().volatilities.reshape(3, 51)
-    return sigma, cal.rank3_history(sigma, 2600, seed=21, noise=2e-5,
+    return sigma, hjm.synthetic_history(sigma, 2600, seed=21, noise=2e-5,
                                     base=hjm.benchmark_market().spot_rates)
 
 
@@ -50,7 +50,7 @@ def test_exact_rank3_history_recovers_generating_drift(oracle):
     _flush("[gpu] PCA calibration of an exactly rank-3 history")
     m = hjm.benchmark_market()
     sigma = m.volatilities.reshape(3, 51)
-    curves = cal.rank3_history(sigma, 750, seed=4)
+    curves = hjm.synthetic_history(sigma, 750, seed=4)
     g = hjm.calibrate_pca(curves)
     gram = np.sort(np.linalg.eigvalsh(sigma @ sigma.T))[::-1]
     np.testing.assert_allclose(g.eigenvalues[0, :3], gram, rtol=1e-9)

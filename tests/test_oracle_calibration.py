@@ -31,7 +31,7 @@ def sigma():
 
 
 def test_rank3_history_has_the_generating_covariance(sigma):
-    curves = cal.rank3_history(sigma, 600, seed=3)
+    curves = hjm.synthetic_history(sigma, 600, seed=3)
     lam, raw, C = cal.pca_window(curves)
     G = sigma.T @ sigma
     assert np.max(np.abs(C - G)) < 1e-12 * np.max(np.abs(G))
@@ -46,7 +46,7 @@ def test_rank3_history_has_the_generating_covariance(sigma):
 
 
 def test_drift_of_pca_loadings_equals_generating_drift(sigma, oracle):
-    curves = cal.rank3_history(sigma, 500, seed=5)
+    curves = hjm.synthetic_history(sigma, 500, seed=5)
     _, raw, _ = cal.pca_window(curves)
     m_gen = oracle.hjm_drift(sigma.reshape(-1))
     m_pca = oracle.hjm_drift(raw.reshape(-1))
@@ -58,7 +58,7 @@ def test_drift_of_pca_loadings_equals_generating_drift(sigma, oracle):
 
 
 def test_cubic_fit_is_exact_quadratic_is_not(sigma):
-    curves = cal.rank3_history(sigma, 400, seed=7)
+    curves = hjm.synthetic_history(sigma, 400, seed=7)
     _, raw, _ = cal.pca_window(curves)
     np.testing.assert_allclose(cal.fit(raw, (3, 3, 3)), raw, rtol=0, atol=1e-12)
     assert np.max(np.abs(cal.fit(raw, (2, 2, 2)) - raw)) > 1e-6
@@ -66,7 +66,7 @@ def test_cubic_fit_is_exact_quadratic_is_not(sigma):
 
 
 def test_sign_convention_and_windows(sigma, oracle):
-    curves = cal.rank3_history(sigma, 300, seed=11, noise=1e-5)
+    curves = hjm.synthetic_history(sigma, 300, seed=11, noise=1e-5)
     ev, vols, drifts, raws = cal.calibrate(curves, 120, 30, 6, 252.0, None, 0.5,
                                            lambda v, h: oracle.hjm_drift(v, h))
     for w in range(6):
