@@ -152,6 +152,7 @@ def _declare(lib):
     lib.hjm_calculate_exposure_multi_gpu_f32.argtypes = [fp, f, f, f, f, fp, fp, fp, fp, f, i, i,
                                                          f]
     lib.hjm_block_map.argtypes = [_dp, _dp, d, d, C.c_int32, fp]
+    lib.hjm_select_kth.argtypes = [_dp, C.c_int32, C.c_int64, C.c_int64, C.c_int32, _dp]
     lib.hjm_calibrate_pca.argtypes = [C.POINTER(_Calibration), C.POINTER(_CalibResults)]
     lib.hjm_generate_normals.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_int32, C.c_int32,
                                          _dp]
@@ -377,6 +378,16 @@ def block_map(market: Market, dt: float = 0.01, dtau: float = 0.5,
     _check(library().hjm_block_map(_ptr(market.drifts), _ptr(market.volatilities), dt, dtau,
                                    int(bool(portfolio)),
                                    out.ctypes.data_as(C.POINTER(C.c_float))), "hjm_block_map")
+    return out
+
+
+def select_kth(values, k: int, device: int = 0) -> np.ndarray:
+    """k-th smallest (1-based) of each row of non-negative values [rows][n] on the GPU
+    (the exact radix selection behind the PFE)."""
+    v = np.ascontiguousarray(np.atleast_2d(values), np.float64)
+    out = np.zeros(v.shape[0])
+    _check(library().hjm_select_kth(_ptr(v), v.shape[0], v.shape[1], int(k), int(device),
+                                    _ptr(out)), "hjm_select_kth")
     return out
 
 

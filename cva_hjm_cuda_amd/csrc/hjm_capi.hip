@@ -634,6 +634,31 @@ int hjm_generate_normals(uint64_t seed, int64_t path_offset, int64_t paths, int3
     return rc;
 }
 
+int hjm_select_kth(const double* values, int32_t rows, int64_t n, int64_t k, int32_t device,
+                   double* out)
+{
+    if (!values || !out) return HJM_ERR_NULL;
+    if (rows <= 0 || n <= 0 || k < 1 || k > n) return HJM_ERR_ARG;
+    const size_t cells = (size_t)rows * (size_t)n;
+    for (size_t i = 0; i < cells; ++i)
+        if (!(values[i] >= 0.0) || !isfinite(values[i])) return HJM_ERR_ARG;
+    DeviceGuard guard;
+    HJM_TRY(guard.set(device));
+    double *d = nullptr, *d_out = nullptr;
+    int rc = dev_alloc(&d, cells);
+    if (rc == HJM_OK) rc = dev_alloc(&d_out, (size_t)rows);
+    if (rc == HJM_OK && hipMemcpy(d, values, cells * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        rc = HJM_ERR_DEVICE;
+    if (rc == HJM_OK && launch_radix_select(d, rows, n, k, d_out, nullptr) != hipSuccess)
+        rc = HJM_ERR_LAUNCH;
+    if (rc == HJM_OK && hipDeviceSynchronize() != hipSuccess) rc = HJM_ERR_LAUNCH;
+    if (rc == HJM_OK && hipMemcpy(out, d_out, rows * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = HJM_ERR_DEVICE;
+    dev_free(d);
+    dev_free(d_out);
+    return rc;
+}
+
 // simulation_gpu.cu:955-1241 semantics: scenarios split into num_gpus contiguous shards
 // (the remainder is spread over the first shards instead of being dropped, :960), one
 // shard per device g % device_count, shards enqueued before any is awaited, partial

@@ -344,12 +344,17 @@ int pplan_collect(hjm_portfolio_plan p, hjm_portfolio_results* r)
             r->cva_stderr[g] = sqrt((var > 0.0 ? var : 0.0) / P);
         }
     }
+    // device rows are [date][set]; the results are [set][date]
     if (r->pfe) {
-        if (p->do_pfe)
-            HJM_HIP(hipMemcpy(r->pfe, p->d_pfe, sizeof(double) * G * nd, hipMemcpyDeviceToHost),
+        if (p->do_pfe) {
+            std::vector<double> pf((size_t)G * nd);
+            HJM_HIP(hipMemcpy(pf.data(), p->d_pfe, sizeof(double) * G * nd, hipMemcpyDeviceToHost),
                     HJM_ERR_DEVICE);
-        else
+            for (int e = 0; e < nd; ++e)
+                for (int g = 0; g < G; ++g) r->pfe[(size_t)g * nd + e] = pf[(size_t)e * G + g];
+        } else {
             for (int i = 0; i < G * nd; ++i) r->pfe[i] = 0.0;
+        }
     }
     if (r->exposures) {
         const size_t cells = (size_t)G * nd * p->paths;
@@ -363,9 +368,12 @@ int pplan_collect(hjm_portfolio_plan p, hjm_portfolio_results* r)
                     HJM_ERR_DEVICE);
             for (size_t i = 0; i < cells; ++i) tmp[i] = tf[i];
         }
-        for (int64_t i = 0; i < p->paths; ++i)
-            for (int row = 0; row < G * nd; ++row)
-                r->exposures[(size_t)i * G * nd + row] = tmp[(size_t)row * p->paths + i];
+        for (int e = 0; e < nd; ++e)
+            for (int g = 0; g < G; ++g) {
+                const double* src = &tmp[((size_t)e * G + g) * p->paths];
+                for (int64_t i = 0; i < p->paths; ++i)
+                    r->exposures[((size_t)i * G + g) * nd + e] = src[i];
+            }
     }
     if (r->timings_ms) {
         float ms = 0.f;

@@ -30,9 +30,12 @@ constexpr int kNT = 2;                       // N-tiles (16 paths each) per wave
 constexpr int kMfmaBlock = 512;              // 8 waves
 constexpr int kMfmaPathsPerBlock = kMfmaBlock / kWave * kNT * 16;
 
-// One 50-step block: acc = W x [fin; shocks of steps b*50 .. b*50+49; 1].
+// One 50-step block: acc = W x [fin; shocks of steps b*50 .. b*50+49; 1]. fin is consumed
+// (rotated in place). The RNG of step quad q+1 runs on the VALU while the matrix cores
+// take quad q's shocks and one chunk of the previous curve; scheduling groups interleave
+// the two instruction streams within each wave.
 template <int MT>
-__device__ __forceinline__ void mfma_block(const float* __restrict__ lds_w, const float (&fin)[kNT][16],
+__device__ __forceinline__ void mfma_block(const float* __restrict__ lds_w, float (&fin)[kNT][16],
                                            f4_t (&acc)[MT][kNT], const MfmaGrid& g,
                                            const int64_t (&path)[kNT], int b, int grp, int lane)
 {
@@ -40,21 +43,8 @@ __device__ __forceinline__ void mfma_block(const float* __restrict__ lds_w, cons
     for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int n = 0; n < kNT; ++n) acc[m][n] = f4_t{0.f, 0.f, 0.f, 0.f};
-    // the previous block's curve
-#pragma unroll
-    for (int kc = 0; kc < kMfmaFChunks; ++kc) {
-#pragma unroll
-        for (int m = 0; m < MT; ++m) {
-            const float a = lds_w[(m * kMfmaKChunks + kc) * kWave + lane];
-#pragma unroll
-            for (int n = 0; n < kNT; ++n) acc[m][n] = mfma16(a, fin[n][kc], acc[m][n]);
-        }
-        // keep the fragment loads of later chunks from being hoisted (register pressure)
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    // shocks: step 4 q + grp of this block, three factors per step; generated one quad
-    // ahead so the VALU works on quad q+1 while the matrix cores consume quad q. Lanes
-    // past the block's last step hold 0, except the constant 1 of the drift.
+    // shocks: step 4 q + grp of this block, three factors per step. Lanes past the
+    // block's last step hold 0, except the constant 1 of the drift.
     float zc[kNT][3], zn[kNT][3];
     auto gen = [&](int q, float (&z)[kNT][3]) {
         const int t = 4 * q + grp;
@@ -69,9 +59,30 @@ __device__ __forceinline__ void mfma_block(const float* __restrict__ lds_w, cons
             z[n][2] = in ? v.z2 : 0.f;
         }
     };
+    auto f_chunk = [&](int kc, int reg) {
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const float a = lds_w[(m * kMfmaKChunks + kc) * kWave + lane];
+#pragma unroll
+            for (int n = 0; n < kNT; ++n) acc[m][n] = mfma16(a, fin[n][reg], acc[m][n]);
+        }
+    };
+    constexpr int kHead = kMfmaFChunks - kMfmaQuads;  // 3 curve chunks before the loop
     gen(0, zc);
+#pragma unroll
+    for (int kc = 0; kc < kHead; ++kc) f_chunk(kc, kc);
+#pragma unroll
+    for (int n = 0; n < kNT; ++n)
+#pragma unroll
+        for (int i = 0; i < kMfmaQuads; ++i) fin[n][i] = fin[n][i + kHead];
+#pragma unroll
+    for (int i = 0; i < kHead * MT * kNT; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, MT == 7 ? 9 : 16, 0);
+    }
     for (int q = 0; q < kMfmaQuads; ++q) {
         gen(q + 1 < kMfmaQuads ? q + 1 : q, zn);
+        f_chunk(kHead + q, 0);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const int kc = kMfmaFChunks + 3 * q + k;
@@ -83,9 +94,18 @@ __device__ __forceinline__ void mfma_block(const float* __restrict__ lds_w, cons
             }
         }
 #pragma unroll
-        for (int n = 0; n < kNT; ++n)
+        for (int n = 0; n < kNT; ++n) {
+#pragma unroll
+            for (int i = 0; i + 1 < kMfmaQuads; ++i) fin[n][i] = fin[n][i + 1];
 #pragma unroll
             for (int k = 0; k < 3; ++k) zc[n][k] = zn[n][k];
+        }
+        // one MFMA, then a share of the VALU work, 8 MT times
+#pragma unroll
+        for (int i = 0; i < 8 * MT; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, MT == 7 ? 7 : 12, 0);
+        }
     }
 }
 

@@ -10,11 +10,12 @@
 //      checkpoints the curve and the log bank account (coalesced [date][tenor][path]).
 //   2. pf_value_kernel: per path and date, 60 bond prices and every netting set's value
 //      V_g = a_ge - sum_k gamma_gek P_k (host-aggregated coefficients, read through
-//      scalar loads), exposure max(V_g, 0) / B stored [set][date][path].
+//      scalar loads), exposure max(V_g, 0) / B stored [date][set][path].
 //   3. pf_stats_kernel: per set, EE numerators and per-path CVA moments (FP64).
 #include "hjm_common.h"
 #include "hjm_kernels.h"
 #include "hjm_rng.h"
+#include "hjm_select.h"
 #include "hjm_valu.h"
 
 namespace hjm {
@@ -214,50 +215,6 @@ __global__ __launch_bounds__(kBlock) void grouped_column_sum_kernel(const double
     }
 }
 
-// k-th smallest of each row of [rows][n] non-negative FP32 values: MSD radix selection
-// on the bit patterns (order-preserving for x >= 0), passes of 11/11/10 bits.
-__global__ __launch_bounds__(1024) void radix_select_f32_kernel(const float* __restrict__ x,
-                                                                int64_t n, int64_t k,
-                                                                double* __restrict__ out)
-{
-    constexpr int kBins = 2048;
-    __shared__ uint32_t hist[kBins];
-    __shared__ uint32_t s_prefix;
-    __shared__ int64_t s_k;
-    const float* row = x + (int64_t)blockIdx.x * n;
-    if (threadIdx.x == 0) {
-        s_prefix = 0;
-        s_k = k;
-    }
-    int shift = 32;
-    for (int pass = 0; pass < 3; ++pass) {
-        const int width = (pass < 2) ? 11 : 10;
-        shift -= width;
-        for (int b = threadIdx.x; b < kBins; b += blockDim.x) hist[b] = 0;
-        __syncthreads();
-        const uint32_t prefix = s_prefix;
-        const int hi = shift + width;
-        for (int64_t t = threadIdx.x; t < n; t += blockDim.x) {
-            const uint32_t bits = __builtin_bit_cast(uint32_t, row[t]);
-            const bool match = (hi >= 32) ? true : ((bits >> hi) == (prefix >> hi));
-            if (match) atomicAdd(&hist[(bits >> shift) & ((1u << width) - 1u)], 1u);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int64_t kk = s_k;
-            int d = 0;
-            for (; d < (1 << width); ++d) {
-                if ((int64_t)hist[d] >= kk) break;
-                kk -= hist[d];
-            }
-            s_prefix = prefix | ((uint32_t)d << shift);
-            s_k = kk;
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[blockIdx.x] = (double)__builtin_bit_cast(float, s_prefix);
-}
-
 // ---- launchers -------------------------------------------------------------------
 template <typename Real>
 static hipError_t pf_paths_launch(const RefStepConsts<Real>& k, const PfGrid& g, const double* normals,
@@ -337,7 +294,7 @@ hipError_t launch_radix_select_f32(const float* x, int rows, int64_t n, int64_t 
                                    hipStream_t st)
 {
     if (rows <= 0) return hipSuccess;
-    radix_select_f32_kernel<<<rows, 1024, 0, st>>>(x, n, k, out);
+    select_kth_kernel<float><<<rows, kSelBlock, 0, st>>>(x, n, k, out);
     return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@
 // one coalesced store per numeraire slot, slot-major [slot][path].
 #include "hjm_common.h"
 #include "hjm_rng.h"
+#include "hjm_select.h"
 #include "hjm_kernels.h"
 #include "hjm_valu.h"
 
@@ -225,51 +226,6 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(const double* __rest
     }
 }
 
-// k-th smallest (1-based) of each row of a [rows][n] array of non-negative doubles by
-// MSD radix selection on the IEEE bit patterns (order-preserving for x >= 0): six
-// passes of 11/11/11/11/11/9 bits, LDS histograms, one workgroup per row. Exact.
-__global__ __launch_bounds__(1024) void radix_select_kernel(const double* __restrict__ x,
-                                                            int64_t n, int64_t k,
-                                                            double* __restrict__ out)
-{
-    constexpr int kBins = 2048;
-    __shared__ uint32_t hist[kBins];
-    __shared__ uint64_t s_prefix;
-    __shared__ int64_t s_k;
-    const double* row = x + (int64_t)blockIdx.x * n;
-    if (threadIdx.x == 0) {
-        s_prefix = 0;
-        s_k = k;
-    }
-    int shift = 64;
-    for (int pass = 0; pass < 6; ++pass) {
-        const int width = (pass < 5) ? 11 : 9;
-        shift -= width;
-        for (int b = threadIdx.x; b < kBins; b += blockDim.x) hist[b] = 0;
-        __syncthreads();
-        const uint64_t prefix = s_prefix;
-        const int hi_shift = shift + width;
-        for (int64_t t = threadIdx.x; t < n; t += blockDim.x) {
-            const uint64_t bits = __builtin_bit_cast(uint64_t, row[t]);
-            const bool match = (hi_shift >= 64) ? true : ((bits >> hi_shift) == (prefix >> hi_shift));
-            if (match) atomicAdd(&hist[(bits >> shift) & ((1u << width) - 1u)], 1u);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int64_t kk = s_k;
-            int d = 0;
-            for (; d < (1 << width); ++d) {
-                if ((int64_t)hist[d] >= kk) break;
-                kk -= hist[d];
-            }
-            s_prefix = prefix | ((uint64_t)d << shift);
-            s_k = kk;
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[blockIdx.x] = __builtin_bit_cast(double, s_prefix);
-}
-
 // The in-kernel generator's normals for [path0, path0 + paths) x [0, nsteps).
 __global__ __launch_bounds__(kBlock) void normals_kernel(uint32_t seed_lo, uint32_t seed_hi,
                                                          int64_t path0, int64_t paths,
@@ -329,7 +285,7 @@ hipError_t launch_radix_select(const double* x, int rows, int64_t n, int64_t k, 
                                hipStream_t st)
 {
     if (rows <= 0) return hipSuccess;
-    radix_select_kernel<<<rows, 1024, 0, st>>>(x, n, k, out);
+    select_kth_kernel<double><<<rows, kSelBlock, 0, st>>>(x, n, k, out);
     return hipGetLastError();
 }
 

@@ -274,6 +274,12 @@ int hjm_calibrate_pca(const hjm_calibration* calibration, hjm_calibration_result
 int hjm_generate_normals(uint64_t seed, int64_t path_offset, int64_t paths, int32_t nsteps,
                          int32_t device, double* out);
 
+/* ---- the PFE selection on caller data (no counterpart in the reference): out[r] = the
+ * k-th smallest (1-based) of row r of values [rows][n], all >= 0 (exact, by the GPU
+ * radix selection the PFE uses). HJM_ERR_ARG for negative or non-finite values.       */
+int hjm_select_kth(const double* values, int32_t rows, int64_t n, int64_t k, int32_t device,
+                   double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,7 +68,7 @@ def test_exact_rank3_history_recovers_generating_drift(oracle):
 def test_calibration_argument_errors(history):
     _, curves = history
     with pytest.raises(hjm.HJMError):
-        hjm.calibrate_pca(curves, window=2000, step=100, nwindows=7)
+        hjm.calibrate_pca(curves, window=2000, step=100, nwindows=8)
     with pytest.raises(hjm.HJMError):
         hjm.calibrate_pca(curves, window=2)
     with pytest.raises(hjm.HJMError):

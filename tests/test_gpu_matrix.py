@@ -1,7 +1,8 @@
 """GPU parity of the matrix-core (MFMA) FP32 path kernel.
 
 Per path against the FP64 oracle on the same (bit-identical) Philox normals: forward
-rates num_x within 2e-6 absolute and discount factors num_y within 1e-6 relative; EE and
+rates num_x within 2e-6 absolute and discount factors num_y within 1e-6 relative up to
+10y, and no further from FP64 than the VALU FP32 kernel at 25y; EE and
 CVA of the precision configuration (131072 paths x 10y) within 1e-4 relative of the FP64
 kernel; shard additivity; the kernel choice rules (AUTO, VECTOR, MATRIX).
 """
@@ -20,6 +21,11 @@ def _flush(msg):
 
 
 def test_matrix_paths_match_oracle_per_path(oracle):
+    """Up to 10y the FP32 curve stays within 2e-6 of FP64. Over 25y the scheme's last two
+    tenors form a Jordan block of eigenvalue 1 (the downwind extrapolation keeps
+    f_50 - f_49 fixed), so rounding errors grow linearly with the step count and advect to
+    every tenor: there the matrix kernel must be at least as close to FP64 as the VALU FP32
+    kernel (CPU emulation of plain FP32 Euler: 1.1e-4 max error at 25y)."""
     m = hjm.benchmark_market()
     for P, expiry, seed in ((777, 10.0, 5), (256, 25.0, 11), (33, 0.5, 2)):
         _flush("[gpu] matrix kernel vs oracle: %d paths x %gy seed %d" % (P, expiry, seed))
@@ -30,11 +36,22 @@ def test_matrix_paths_match_oracle_per_path(oracle):
         nx, ny = oracle.reference_paths(m.spot_rates, m.drifts, m.volatilities, expiry, 0.5,
                                         0.01, z)
         D = int(round(expiry / 0.5))
-        assert np.max(np.abs(r.num_x - nx)) < 2e-6
         live = ny != 0
         assert np.array_equal(r.num_y != 0, live)
-        assert np.max(np.abs(r.num_y[live] / ny[live] - 1)) < 1e-6
         assert np.all(r.num_x[:, D + 1:] == 0) and np.all(r.num_x[:, 0] == 0)
+        ex = np.max(np.abs(r.num_x - nx))
+        ey = np.max(np.abs(r.num_y[live] / ny[live] - 1))
+        if expiry <= 10.0:
+            assert ex < 2e-6 and ey < 1e-6, (ex, ey)
+        else:
+            v = hjm.simulate_reference(m, swap, P, seed=seed, precision=hjm.FP32_KAHAN,
+                                       keep_paths=True, path_kernel=hjm.KERNEL_VECTOR)
+            vx = np.max(np.abs(v.num_x - nx))
+            vy = np.max(np.abs(v.num_y[live] / ny[live] - 1))
+            _flush("  25y max errors: matrix x %.2e y %.2e, vector x %.2e y %.2e"
+                   % (ex, ey, vx, vy))
+            assert ex <= max(vx, 2e-6) and ey <= max(vy, 1e-6)
+            assert ex < 1e-4 and ey < 1e-3
 
 
 def test_matrix_precision_config_vs_fp64():

@@ -94,8 +94,8 @@ def test_graph_replays_and_tuned_grids_are_bit_identical():
     _flush("[gpu] hipGraph replays vs eager, tuned grids")
     m = hjm.benchmark_market()
     pf = hjm.synthetic_portfolio(96, 12, seed=3)
-    eager = hjm.simulate_portfolio(m, pf, 20000, horizon=15.0, seed=9)
-    plan = hjm.PortfolioPlan(m, pf, 20000, horizon=15.0, seed=9,
+    eager = hjm.simulate_portfolio(m, pf, 20000, horizon=30.0, seed=9)
+    plan = hjm.PortfolioPlan(m, pf, 20000, horizon=30.0, seed=9,
                              tuning=hjm.Tuning(256, 0, use_graph=True))
     for rep in range(4):
         r = plan.execute()
@@ -105,14 +105,14 @@ def test_graph_replays_and_tuned_grids_are_bit_identical():
         hjm.simulate_reference(m, hjm.InterestRateSwap(1e6, 0.02, 5.0), 512, seed=rep)
     plan.close()
     # persistent grids of the VALU path kernel
-    eager_v = hjm.simulate_portfolio(m, pf, 20000, horizon=15.0, seed=9,
+    eager_v = hjm.simulate_portfolio(m, pf, 20000, horizon=30.0, seed=9,
                                      path_kernel=hjm.KERNEL_VECTOR)
     for t in (hjm.Tuning(64, 1), hjm.Tuning(128, 4), hjm.Tuning(256, 2, use_graph=True)):
-        r = hjm.simulate_portfolio(m, pf, 20000, horizon=15.0, seed=9, tuning=t,
+        r = hjm.simulate_portfolio(m, pf, 20000, horizon=30.0, seed=9, tuning=t,
                                    path_kernel=hjm.KERNEL_VECTOR)
         assert np.array_equal(r.expected_exposure, eager_v.expected_exposure), t
     assert np.max(np.abs(eager_v.cva / eager.cva - 1)) < 1e-4
-    tune, ms = hjm.autotune_portfolio(m, pf, 20000, horizon=15.0, seed=9)
+    tune, ms = hjm.autotune_portfolio(m, pf, 20000, horizon=30.0, seed=9)
     assert tune.block_size in (64, 128, 256) and ms > 0
 
 

@@ -206,3 +206,26 @@ def test_gpu_argument_errors(c1):
     with pytest.raises(hjm.HJMError) as ei:
         hjm.simulate_reference(c1["m"], hjm.InterestRateSwap(1e6, 0.025, 30.0), 64)
     assert ei.value.code == 3
+
+
+def test_select_kth_exact_on_hard_rows():
+    """The PFE selection on crafted rows: ties, all zeros, a rank inside a bin too large
+    for the LDS candidate buffer, extreme ranks, values spanning many binades."""
+    _flush("[gpu] radix selection on crafted rows")
+    rng = np.random.default_rng(12)
+    n = 200003
+    rows = [rng.lognormal(0.0, 3.0, n),                 # many binades
+            np.zeros(n),                                # all equal (zero)
+            np.where(rng.random(n) < 0.9, 0.0, rng.random(n)),
+            np.full(n, 1234.5),                         # one value, > 4096 copies per bin
+            np.round(rng.random(n) * 50.0) / 8.0,       # heavy ties
+            1.0 + rng.random(n) * 1e-12]                # one binade, narrow mantissas
+    X = np.stack(rows)
+    for k in (1, 2, 4097, math.ceil(0.975 * n), n - 1, n):
+        got = hjm.select_kth(X, k)
+        want = np.partition(X, k - 1, axis=1)[:, k - 1]
+        assert np.array_equal(got, want), k
+    with pytest.raises(hjm.HJMError):
+        hjm.select_kth(-X[:1, :10], 3)
+    with pytest.raises(hjm.HJMError):
+        hjm.select_kth(X[:1, :10], 11)
