@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOPS_PER_PATH_STEP = 561          # 51 tenors x (1 sub + 4 FMA + 1 add + 1 accumulate)
+PF_FLOPS_PER_PATH_STEP = 512       # 51 tenors x 5 FMA + the short-rate sum (portfolio)
 PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}   # MI355X published vector peaks
 PATHS, EXPIRY, DT, SEED = 131072, 10.0, 0.01, 7
 TENORS_N = 51
@@ -70,28 +71,32 @@ def adjoint_ratio(hjm, market, reps=3):
 
 
 def run_portfolio(hjm, args, nswaps, nsets, paths):
+    """Portfolio workload: the matrix-core FP32 path kernel, valuation, statistics and PFE
+    of every netting set, replayed as one hipGraph per step (timed on the wall clock);
+    the per-kernel breakdown comes from eager runs of the same plan."""
     market = hjm.benchmark_market()
     pf = hjm.synthetic_portfolio(nswaps, nsets, seed=2024)
-    log("[bench] autotuning the portfolio path kernel")
-    tune, tune_ms = hjm.autotune_portfolio(market, pf, paths, 30.0, DT, seed=2024)
-    plan = hjm.PortfolioPlan(market, pf, paths, 30.0, DT, precision=hjm.FP32_KAHAN, seed=2024,
-                             tuning=tune)
-    for _ in range(args.warmup):
+    S = 3000
+    mk = dict(precision=hjm.FP32_KAHAN, seed=2024)
+    eager = hjm.PortfolioPlan(market, pf, paths, 30.0, DT, tuning=hjm.Tuning(256, 0, False),
+                              **mk)
+    eager.execute()
+    times = np.array([eager.execute().timings_ms.copy() for _ in range(3)])
+    eager.close()
+    plan = hjm.PortfolioPlan(market, pf, paths, 30.0, DT, tuning=hjm.Tuning(256, 0, True), **mk)
+    for _ in range(max(1, args.warmup)):
         plan.execute()
-    times = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        times.append(plan.execute().timings_ms.copy())
+        plan.execute()
     wall = time.perf_counter() - t0
     plan.close()
-    S = 3000
-    times = np.array(times)
     kp = float(np.mean(times[:, 0]))
-    flops = 51 * 10 * paths * S   # 5 FMAs per tenor, no discount integrals
+    flops = PF_FLOPS_PER_PATH_STEP * paths * S
     achieved = flops / (kp * 1e-3) / 1e12
     value = paths * S * args.steps / wall
     return {
-        "metric": "path-steps/s (51 tenors) FP32 vs FP64; CVA+204 Greeks time vs forward-only",
+        "metric": "path-steps/s (51 tenors), portfolio exposure + CVA per netting set",
         "value": value, "unit": "path-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
@@ -99,15 +104,15 @@ def run_portfolio(hjm, args, nswaps, nsets, paths):
         "config": {"workload": "portfolio: %d swaps in %d netting sets, %d paths x %d steps "
                                "(30y, dt=0.01), EE/PFE97.5/CVA per set" % (nswaps, nsets, paths, S),
                    "paths": paths, "steps_per_path": S, "swaps": nswaps, "netting_sets": nsets,
-                   "tuning": {"block_size": tune.block_size,
-                              "blocks_per_cu": tune.blocks_per_cu, "hip_graph": True}},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["f32"],
+                   "hip_graph": True},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["f32"],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f32"], "traffic": None,
-                     "kernel": "pf_paths_kernel<float>", "kernel_ms": kp},
+                     "kernel": "pf_paths_mfma_kernel", "kernel_ms": kp,
+                     "flops_per_launch": flops},
         "kernel_ms_breakdown": {"paths": kp, "valuation": float(np.mean(times[:, 1])),
                                 "statistics": float(np.mean(times[:, 2])),
                                 "pfe_select": float(np.mean(times[:, 3])),
-                                "device_total": float(np.mean(times[:, 4]))},
+                                "device_total_eager": float(np.mean(times[:, 4]))},
         "cpu_baseline": None,
     }
 

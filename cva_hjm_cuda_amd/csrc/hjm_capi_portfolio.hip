@@ -261,13 +261,15 @@ int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfol
     return HJM_OK;
 }
 
-// Enqueue the four passes (with timing events) on the plan's stream.
-int pplan_enqueue(hjm_portfolio_plan p)
+// Enqueue the four passes on the plan's stream, with timing events between them unless
+// the sequence is being captured into a graph (a captured replay is timed as a whole).
+int pplan_enqueue(hjm_portfolio_plan p, bool events = true)
 {
     hipStream_t st = p->stream;
     const bool f64 = p->precision == HJM_FP64;
     const int rows = p->nsets * p->ndates;
-    HJM_HIP(hipEventRecord(p->ev[0], st), HJM_ERR_DEVICE);
+    auto mark = [&](int i) { return events ? hipEventRecord(p->ev[i], st) : hipSuccess; };
+    HJM_HIP(mark(0), HJM_ERR_DEVICE);
     hipError_t err =
         p->use_mfma ? launch_pf_paths_mfma(p->mg, p->d_wfrag, (float*)p->d_curves, p->d_logb, st)
         : f64 ? launch_pf_paths_f64(p->kd, p->grid, p->d_normals, (double*)p->d_curves, p->d_logb,
@@ -275,7 +277,7 @@ int pplan_enqueue(hjm_portfolio_plan p)
               : launch_pf_paths_f32(p->kf, p->grid, p->d_normals, (float*)p->d_curves, p->d_logb,
                                     p->tuning.block_size, p->grid_blocks, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
-    HJM_HIP(hipEventRecord(p->ev[1], st), HJM_ERR_DEVICE);
+    HJM_HIP(mark(1), HJM_ERR_DEVICE);
     err = f64 ? launch_pf_value_f64(p->vp, (const double*)p->d_curves, p->d_logb,
                                     (const double*)p->d_coef, (const double*)p->d_a, p->d_kmax,
                                     (double*)p->d_ex, st)
@@ -283,13 +285,13 @@ int pplan_enqueue(hjm_portfolio_plan p)
                                     (const float*)p->d_coef, (const float*)p->d_a, p->d_kmax,
                                     (float*)p->d_ex, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
-    HJM_HIP(hipEventRecord(p->ev[2], st), HJM_ERR_DEVICE);
+    HJM_HIP(mark(2), HJM_ERR_DEVICE);
     err = f64 ? launch_pf_stats_f64(p->vp, (const double*)p->d_ex, p->d_w, p->d_partials,
                                     p->d_sums, st)
               : launch_pf_stats_f32(p->vp, (const float*)p->d_ex, p->d_w, p->d_partials,
                                     p->d_sums, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
-    HJM_HIP(hipEventRecord(p->ev[3], st), HJM_ERR_DEVICE);
+    HJM_HIP(mark(3), HJM_ERR_DEVICE);
     if (p->do_pfe) {
         err = f64 ? launch_radix_select((const double*)p->d_ex, rows, p->paths, p->pfe_rank,
                                         p->d_pfe, st)
@@ -297,7 +299,7 @@ int pplan_enqueue(hjm_portfolio_plan p)
                                             p->d_pfe, st);
         HJM_HIP(err, HJM_ERR_LAUNCH);
     }
-    HJM_HIP(hipEventRecord(p->ev[4], st), HJM_ERR_DEVICE);
+    HJM_HIP(mark(4), HJM_ERR_DEVICE);
     return HJM_OK;
 }
 
@@ -306,7 +308,7 @@ int pplan_run(hjm_portfolio_plan p)
     if (!p->tuning.use_graph) return pplan_enqueue(p);
     if (!p->exec) {
         HJM_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), HJM_ERR_DEVICE);
-        const int rc = pplan_enqueue(p);
+        const int rc = pplan_enqueue(p, false);
         hipGraph_t graph = nullptr;
         const hipError_t end = hipStreamEndCapture(p->stream, &graph);
         if (rc != HJM_OK) {
@@ -317,7 +319,9 @@ int pplan_run(hjm_portfolio_plan p)
         p->graph = graph;
         HJM_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0), HJM_ERR_DEVICE);
     }
+    HJM_HIP(hipEventRecord(p->ev[0], p->stream), HJM_ERR_DEVICE);
     HJM_HIP(hipGraphLaunch(p->exec, p->stream), HJM_ERR_LAUNCH);
+    HJM_HIP(hipEventRecord(p->ev[4], p->stream), HJM_ERR_DEVICE);
     return HJM_OK;
 }
 
@@ -378,6 +382,10 @@ int pplan_collect(hjm_portfolio_plan p, hjm_portfolio_results* r)
     if (r->timings_ms) {
         float ms = 0.f;
         for (int i = 0; i < 4; ++i) {
+            if (p->tuning.use_graph) {
+                r->timings_ms[i] = NAN;  // one graph replay: only its total is timed
+                continue;
+            }
             HJM_HIP(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]), HJM_ERR_DEVICE);
             r->timings_ms[i] = ms;
         }

@@ -29,6 +29,9 @@ struct DeviceGuard {
         if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return HJM_ERR_DEVICE;
         if (dev < 0 || dev >= n) return HJM_ERR_ARG;
         if (hipSetDevice(dev) != hipSuccess) return HJM_ERR_DEVICE;
+        // a non-sticky error left by an earlier, unrelated call must not be reported by
+        // this call's launch checks
+        (void)hipGetLastError();
         return HJM_OK;
     }
     ~DeviceGuard()
