@@ -372,9 +372,10 @@ class ReferencePlan:
 
 def block_map(market: Market, dt: float = 0.01, dtau: float = 0.5,
               portfolio: bool = False) -> np.ndarray:
-    """A matrix-core path kernel's affine 50-step block map, float32 with 220 columns:
-    [102][220] for the reference semantics, [52][220] for the portfolio semantics."""
-    out = np.zeros((52 if portfolio else 102, 220), np.float32)
+    """A matrix-core path kernel's affine 50-step block map, float32 with 224 columns in
+    the kernel's K order: [102][224] for the reference semantics, [52][224] for the
+    portfolio semantics."""
+    out = np.zeros((52 if portfolio else 102, 224), np.float32)
     _check(library().hjm_block_map(_ptr(market.drifts), _ptr(market.volatilities), dt, dtau,
                                    int(bool(portfolio)),
                                    out.ctypes.data_as(C.POINTER(C.c_float))), "hjm_block_map")

@@ -62,7 +62,7 @@ struct hjm_plan_s {
     RefGrid grid{};
     ExposureParams ep{};
     MfmaGrid mg{};
-    float* d_wfrag = nullptr;
+    uint16_t* d_wfrag = nullptr;  // bf16 parts of the block map
     double* d_normals = nullptr;
     double* d_nx = nullptr;
     double* d_ny = nullptr;
@@ -166,10 +166,10 @@ int plan_init(hjm_plan p, const hjm_market* market, const hjm_swap* swap, const 
     HJM_TRY(dev_alloc(&p->d_pfe, kTenors));
     if (p->use_mfma) {
         HJM_HIP(mfma_kernels_prepare(), HJM_ERR_DEVICE);
-        std::vector<float> wf;
+        std::vector<uint16_t> wf;
         build_mfma_block_map(market->drifts, market->volatilities, cfg->dt, swap->dtau, wf);
         HJM_TRY(dev_alloc(&p->d_wfrag, wf.size()));
-        HJM_HIP(hipMemcpyAsync(p->d_wfrag, wf.data(), wf.size() * sizeof(float),
+        HJM_HIP(hipMemcpyAsync(p->d_wfrag, wf.data(), wf.size() * sizeof(uint16_t),
                                hipMemcpyHostToDevice, p->stream),
                 HJM_ERR_DEVICE);
         MfmaGrid& mg = p->mg;
@@ -596,19 +596,9 @@ int hjm_block_map(const double* drifts, const double* vols, double dt, double dt
     if (portfolio != 0 && portfolio != 1) return HJM_ERR_ARG;
     if (!(dt > 0.0) || !(dtau > 0.0) || fabs(dtau / dt - kMfmaStride) > 1e-6 * kMfmaStride)
         return HJM_ERR_GRID;
-    std::vector<float> wf;
-    build_mfma_block_map(drifts, vols, dt, dtau, wf,
-                         portfolio ? MfmaMap::kPortfolio : MfmaMap::kReference);
-    const int K = 4 * kMfmaKChunks;
-    const int mtiles = portfolio ? kMfmaPfMTiles : kMfmaMTiles;
-    const int rows = portfolio ? kTenors + 1 : 2 * kTenors;
-    for (int m = 0; m < mtiles; ++m)
-        for (int kc = 0; kc < kMfmaKChunks; ++kc)
-            for (int l = 0; l < kWave; ++l) {
-                const int row = 16 * m + (l % 16), kidx = 4 * kc + l / 16;
-                if (row < rows)
-                    out[(size_t)row * K + kidx] = wf[((size_t)m * kMfmaKChunks + kc) * kWave + l];
-            }
+    const std::vector<double> W = mfma_block_map_dense(
+        drifts, vols, dt, dtau, portfolio ? MfmaMap::kPortfolio : MfmaMap::kReference);
+    for (size_t i = 0; i < W.size(); ++i) out[i] = (float)W[i];
     return HJM_OK;
 }
 

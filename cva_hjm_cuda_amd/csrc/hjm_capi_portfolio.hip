@@ -28,7 +28,7 @@ struct hjm_pplan_s {
     int grid_blocks = 0;
     bool use_mfma = false;
     MfmaGrid mg{};
-    float* d_wfrag = nullptr;
+    uint16_t* d_wfrag = nullptr;  // bf16 parts of the block map
     RefStepConsts<double> kd{};
     RefStepConsts<float> kf{};
     PfGrid grid{};
@@ -243,11 +243,11 @@ int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfol
     }
     if (p->use_mfma) {
         HJM_HIP(mfma_kernels_prepare(), HJM_ERR_DEVICE);
-        std::vector<float> wf;
+        std::vector<uint16_t> wf;
         build_mfma_block_map(market->drifts, market->volatilities, cfg->dt, pf->dtau, wf,
                              MfmaMap::kPortfolio);
         HJM_TRY(dev_alloc(&p->d_wfrag, wf.size()));
-        HJM_HIP(hipMemcpy(p->d_wfrag, wf.data(), wf.size() * sizeof(float), hipMemcpyHostToDevice),
+        HJM_HIP(hipMemcpy(p->d_wfrag, wf.data(), wf.size() * sizeof(uint16_t), hipMemcpyHostToDevice),
                 HJM_ERR_DEVICE);
         MfmaGrid& mg = p->mg;
         mg.ndates = nd;

@@ -30,10 +30,10 @@ hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nstep
 // Matrix-core FP32 path kernels (50 steps per date); prepare once per plan (sets the
 // kernels' dynamic LDS limits) before launching or capturing them.
 hipError_t mfma_kernels_prepare();
-hipError_t launch_ref_paths_mfma(const MfmaGrid& g, const float* wfrag, double* nx, double* ny,
+hipError_t launch_ref_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, double* nx, double* ny,
                                  hipStream_t st);
-hipError_t launch_pf_paths_mfma(const MfmaGrid& g, const float* wfrag, float* curves, double* logb,
-                                hipStream_t st);
+hipError_t launch_pf_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, float* curves,
+                                double* logb, hipStream_t st);
 
 // Pathwise adjoint of the reference-semantics CVA.
 hipError_t launch_ref_adjoint_seed(const ExposureParams& e, double dt, const double* nx,

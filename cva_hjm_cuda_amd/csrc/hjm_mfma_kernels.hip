@@ -1,13 +1,16 @@
 // hjm_mfma_kernels.hip -- FP32 path kernels on the matrix cores.
 //
 // Each 50-step block is the affine map [f_out; integral rows] = W [f_in; phi; 1]
-// (hjm_mfma_plan.h). A wave owns 32 paths = 2 N-tiles of 16; per block it issues
-// MT x 2 x 55 v_mfma_f32_16x16x4f32 (A = W fragments from LDS, B = the previous block's
-// accumulators or freshly generated shocks) while the VALU generates the next shocks. The
-// accumulator layout of a 16x16 tile -- lane l holds rows 4 (l / 16) + r, r = 0..3, of
-// column l % 16 -- is used directly as the next block's B operand (W's columns are
-// permuted to match), so the curve never leaves registers. One workgroup of 8 waves
-// shares W in LDS.
+// (hjm_mfma_plan.h), computed with v_mfma_f32_16x16x32_bf16 as six products of three-way
+// bf16 splits (W's parts precomputed, the operand split exactly in registers), which
+// keeps FP32 accuracy while running on the matrix cores concurrently with the VALU
+// (FP32 MFMA on gfx950 shares the vector pipeline and gains nothing over it). A wave owns
+// 32 paths = 2 N-tiles of 16. The accumulator layout of a 16x16 tile -- lane l holds rows
+// 4 (l / 16) + r, r = 0..3, of column l % 16 -- is used directly as the next block's B
+// operand (W's columns are ordered to match), so the curve never leaves registers. The
+// shocks of the block are generated on the VALU, one step per lane group per quad, and
+// each 32-entry K chunk enters the matrix cores as soon as it is complete. One
+// workgroup of 8 waves shares W's fragments in LDS.
 //   ref_paths_mfma_kernel: reference semantics (7 M-tiles): the block integrals dA are
 //     Kahan-added into (A, C); numeraire slot b+1 is recorded after block b.
 //   pf_paths_mfma_kernel: portfolio semantics (4 M-tiles): the curve is checkpointed at
@@ -17,25 +20,97 @@
 #include "hjm_mfma_plan.h"
 #include "hjm_rng.h"
 
+#include <utility>
+
 namespace hjm {
 
 typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef uint32_t u4_t __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ f4_t mfma16(float a, float b, f4_t c)
+__device__ __forceinline__ f4_t mfma_bf16(u4_t a, u4_t b, f4_t c)
 {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8_t, a),
+                                                   __builtin_bit_cast(bf8_t, b), c, 0, 0, 0);
+}
+
+// Exact three-way split of 8 FP32 values into packed bf16 parts (h + m + l = x): h keeps
+// the top 8 significant bits, m the next 8, l the last 8 (each difference is exact).
+struct Split8 {
+    u4_t h, m, l;
+};
+
+__device__ __forceinline__ Split8 split8(const float* x)
+{
+#pragma clang fp contract(off)
+    uint32_t hb[8], mb[8], lb[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint32_t u = __builtin_bit_cast(uint32_t, x[i]);
+        hb[i] = u & 0xffff0000u;
+        const float r1 = x[i] - __builtin_bit_cast(float, hb[i]);
+        mb[i] = __builtin_bit_cast(uint32_t, r1) & 0xffff0000u;
+        const float r2 = r1 - __builtin_bit_cast(float, mb[i]);
+        lb[i] = __builtin_bit_cast(uint32_t, r2);
+    }
+    Split8 s;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        // dword j = element 2j (low half) | element 2j+1 (high half)
+        s.h[j] = __builtin_amdgcn_perm(hb[2 * j + 1], hb[2 * j], 0x07060302u);
+        s.m[j] = __builtin_amdgcn_perm(mb[2 * j + 1], mb[2 * j], 0x07060302u);
+        s.l[j] = __builtin_amdgcn_perm(lb[2 * j + 1], lb[2 * j], 0x07060302u);
+    }
+    return s;
 }
 
 constexpr int kNT = 2;                       // N-tiles (16 paths each) per wave
 constexpr int kMfmaBlock = 512;              // 8 waves
 constexpr int kMfmaPathsPerBlock = kMfmaBlock / kWave * kNT * 16;
 
-// One 50-step block: acc = W x [fin; shocks of steps b*50 .. b*50+49; 1]. fin is consumed
-// (rotated in place). The RNG of step quad q+1 runs on the VALU while the matrix cores
-// take quad q's shocks and one chunk of the previous curve; scheduling groups interleave
-// the two instruction streams within each wave.
+// last quad whose values complete shock chunk s (values 8 s .. 8 s + 7, three per quad)
+__host__ __device__ constexpr int shock_chunk_last_quad(int s)
+{
+    return (8 * s + 7) / 3 < kMfmaQuads - 1 ? (8 * s + 7) / 3 : kMfmaQuads - 1;
+}
+
+// Issue plan: the matrix-core work of the block is spread over the 13 quads of shock
+// generation so that every quad's RNG (VALU) runs beside a share of the MFMAs: chunk c's
+// M-tiles [m0, m1) go to quad issue_quad. Curve chunks 0, 1 are ready at the start;
+// shock chunk s is ready one quad after its last value. Shock chunk 4 completes with the
+// last quad and runs after it.
+struct IssueSlot {
+    int chunk, m0, m1;
+};
 template <int MT>
-__device__ __forceinline__ void mfma_block(const float* __restrict__ lds_w, float (&fin)[kNT][16],
+__host__ __device__ constexpr IssueSlot issue_slot(int q, int half)
+{
+    // quad -> chunk: 0,1: curve 0 | 2,3: curve 1 | 4,5: shock 0 | 6,7: shock 1 | 8,9: shock 2
+    //                11,12: shock 3 | 10: none
+    const int split = (MT + 1) / 2;
+    int c = -1, part = 0;
+    if (q <= 9) {
+        c = q / 2 < 2 ? q / 2 : 2 + (q / 2 - 2);
+        part = q % 2;
+    } else if (q >= 11) {
+        c = 5;
+        part = q - 11;
+    }
+    (void)half;
+    if (c < 0) return {-1, 0, 0};
+    return part == 0 ? IssueSlot{c, 0, split} : IssueSlot{c, split, MT};
+}
+
+// f(integral_constant<int, Q>) for Q in the sequence, in order (compile-time indices).
+template <class F, int... Q>
+__device__ __forceinline__ void for_each_index(F& f, std::integer_sequence<int, Q...>)
+{
+    (f(std::integral_constant<int, Q>{}), ...);
+}
+
+// One 50-step block: acc = W x [fin; shocks of steps b*50 .. b*50+49; 1].
+template <int MT>
+__device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const float (&fin)[kNT][16],
                                            f4_t (&acc)[MT][kNT], const MfmaGrid& g,
                                            const int64_t (&path)[kNT], int b, int grp, int lane)
 {
@@ -43,79 +118,74 @@ __device__ __forceinline__ void mfma_block(const float* __restrict__ lds_w, floa
     for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int n = 0; n < kNT; ++n) acc[m][n] = f4_t{0.f, 0.f, 0.f, 0.f};
-    // shocks: step 4 q + grp of this block, three factors per step. Lanes past the
-    // block's last step hold 0, except the constant 1 of the drift.
-    float zc[kNT][3], zn[kNT][3];
-    auto gen = [&](int q, float (&z)[kNT][3]) {
+    // W's fragments are the same in every block: an opaque offset keeps their LDS reads
+    // inside the block instead of hoisted (hundreds of registers) out of the block loop
+    uint32_t off = (uint32_t)lane;
+    asm volatile("" : "+v"(off));
+    auto tiles = [&](int c, int m0, int m1, const Split8 (&xs)[kNT]) {
+#pragma unroll
+        for (int m = m0; m < m1; ++m) {
+            const u4_t* a = lds_w + ((m * kMfmaChunks + c) * kMfmaParts) * kWave + off;
+            const u4_t ah = a[0], am = a[kWave], al = a[2 * kWave];
+#pragma unroll
+            for (int n = 0; n < kNT; ++n) {
+                f4_t v = acc[m][n];
+                v = mfma_bf16(ah, xs[n].h, v);
+                v = mfma_bf16(ah, xs[n].m, v);
+                v = mfma_bf16(am, xs[n].h, v);
+                v = mfma_bf16(ah, xs[n].l, v);
+                v = mfma_bf16(al, xs[n].h, v);
+                v = mfma_bf16(am, xs[n].m, v);
+                acc[m][n] = v;
+            }
+        }
+    };
+    // split operands of the 7 chunks: curve chunks from fin (registers 8 c .. 8 c + 7),
+    // shock chunk s from values 8 s .. 8 s + 7 of z
+    Split8 xs[kMfmaChunks][kNT];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int n = 0; n < kNT; ++n) xs[c][n] = split8(&fin[n][8 * c]);
+    // shocks: value v = 3 q + k of lane group grp is factor k of step 4 q + grp; step 50
+    // (lane group 2, q = 12) carries the constant 1 of the drift in factor 0.
+    float z[kNT][8 * (kMfmaChunks - 2)];
+#pragma unroll
+    for (int n = 0; n < kNT; ++n) z[n][3 * kMfmaQuads] = 0.f;
+    auto quad = [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
         const int t = 4 * q + grp;
         const int tc = t < kMfmaStride ? t : kMfmaStride - 1;
+        const bool in = t < kMfmaStride;
 #pragma unroll
         for (int n = 0; n < kNT; ++n) {
             const Normals3 v = step_normals(g.seed_lo, g.seed_hi, (uint32_t)(g.path0 + path[n]),
                                             (uint32_t)(b * kMfmaStride + tc));
-            const bool in = t < kMfmaStride;
-            z[n][0] = in ? v.z0 : (t == kMfmaStride ? 1.f : 0.f);
-            z[n][1] = in ? v.z1 : 0.f;
-            z[n][2] = in ? v.z2 : 0.f;
+            z[n][3 * q + 0] = in ? v.z0 : (t == kMfmaStride ? 1.f : 0.f);
+            z[n][3 * q + 1] = in ? v.z1 : 0.f;
+            z[n][3 * q + 2] = in ? v.z2 : 0.f;
         }
+#pragma unroll
+        for (int s = 0; s < kMfmaChunks - 2; ++s)
+            if (q == shock_chunk_last_quad(s))
+#pragma unroll
+                for (int n = 0; n < kNT; ++n) xs[2 + s][n] = split8(&z[n][8 * s]);
+        // this quad's share of the matrix work (its operands are complete); it runs on
+        // the matrix cores beside the RNG of this and the other waves
+        constexpr IssueSlot is = issue_slot<MT>(q, 0);
+        if constexpr (is.chunk >= 0) tiles(is.chunk, is.m0, is.m1, xs[is.chunk]);
     };
-    auto f_chunk = [&](int kc, int reg) {
-#pragma unroll
-        for (int m = 0; m < MT; ++m) {
-            const float a = lds_w[(m * kMfmaKChunks + kc) * kWave + lane];
-#pragma unroll
-            for (int n = 0; n < kNT; ++n) acc[m][n] = mfma16(a, fin[n][reg], acc[m][n]);
-        }
-    };
-    constexpr int kHead = kMfmaFChunks - kMfmaQuads;  // 3 curve chunks before the loop
-    gen(0, zc);
-#pragma unroll
-    for (int kc = 0; kc < kHead; ++kc) f_chunk(kc, kc);
-#pragma unroll
-    for (int n = 0; n < kNT; ++n)
-#pragma unroll
-        for (int i = 0; i < kMfmaQuads; ++i) fin[n][i] = fin[n][i + kHead];
-#pragma unroll
-    for (int i = 0; i < kHead * MT * kNT; ++i) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-        __builtin_amdgcn_sched_group_barrier(0x002, MT == 7 ? 9 : 16, 0);
-    }
-    for (int q = 0; q < kMfmaQuads; ++q) {
-        gen(q + 1 < kMfmaQuads ? q + 1 : q, zn);
-        f_chunk(kHead + q, 0);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const int kc = kMfmaFChunks + 3 * q + k;
-#pragma unroll
-            for (int m = 0; m < MT; ++m) {
-                const float a = lds_w[(m * kMfmaKChunks + kc) * kWave + lane];
-#pragma unroll
-                for (int n = 0; n < kNT; ++n) acc[m][n] = mfma16(a, zc[n][k], acc[m][n]);
-            }
-        }
-#pragma unroll
-        for (int n = 0; n < kNT; ++n) {
-#pragma unroll
-            for (int i = 0; i + 1 < kMfmaQuads; ++i) fin[n][i] = fin[n][i + 1];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) zc[n][k] = zn[n][k];
-        }
-        // one MFMA, then a share of the VALU work, 8 MT times
-#pragma unroll
-        for (int i = 0; i < 8 * MT; ++i) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, MT == 7 ? 7 : 12, 0);
-        }
-    }
+    for_each_index(quad, std::make_integer_sequence<int, kMfmaQuads>{});
+    tiles(kMfmaChunks - 1, 0, MT, xs[kMfmaChunks - 1]);
 }
 
 __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGrid g,
-                                                                const float* __restrict__ wfrag,
+                                                                const u4_t* __restrict__ wfrag,
                                                                 double* __restrict__ num_x,
                                                                 double* __restrict__ num_y)
 {
-    extern __shared__ float lds_w[];
-    for (int t = threadIdx.x; t < kMfmaFragFloats; t += kMfmaBlock) lds_w[t] = wfrag[t];
+    extern __shared__ u4_t lds_w[];
+    for (int t = threadIdx.x; t < kMfmaFragElems / 8; t += kMfmaBlock) lds_w[t] = wfrag[t];
     __syncthreads();
 
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
@@ -213,12 +283,12 @@ __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGr
 // curve goes to curves[e][tenor][path] and the log bank account dt sum_{n < 50 e} f_n(0)
 // to logb[e][path]; the block's row 51 (left-point sum of the short rate) extends it.
 __global__ __launch_bounds__(kMfmaBlock) void pf_paths_mfma_kernel(const MfmaGrid g,
-                                                               const float* __restrict__ wfrag,
+                                                               const u4_t* __restrict__ wfrag,
                                                                float* __restrict__ curves,
                                                                double* __restrict__ logb)
 {
-    extern __shared__ float lds_w[];
-    for (int t = threadIdx.x; t < kMfmaPfFragFloats; t += kMfmaBlock) lds_w[t] = wfrag[t];
+    extern __shared__ u4_t lds_w[];
+    for (int t = threadIdx.x; t < kMfmaPfFragElems / 8; t += kMfmaBlock) lds_w[t] = wfrag[t];
     __syncthreads();
 
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
@@ -276,27 +346,27 @@ hipError_t mfma_kernels_prepare()
 {
     hipError_t e = hipFuncSetAttribute((const void*)ref_paths_mfma_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)(sizeof(float) * kMfmaFragFloats));
+                                       (int)(2 * kMfmaFragElems));
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void*)pf_paths_mfma_kernel,
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)(sizeof(float) * kMfmaPfFragFloats));
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)(2 * kMfmaPfFragElems));
 }
 
-hipError_t launch_ref_paths_mfma(const MfmaGrid& g, const float* wfrag, double* nx, double* ny,
+hipError_t launch_ref_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, double* nx, double* ny,
                                  hipStream_t st)
 {
     const dim3 grid((unsigned)((g.paths + kMfmaPathsPerBlock - 1) / kMfmaPathsPerBlock));
-    ref_paths_mfma_kernel<<<grid, kMfmaBlock, sizeof(float) * kMfmaFragFloats, st>>>(g, wfrag, nx, ny);
+    ref_paths_mfma_kernel<<<grid, kMfmaBlock, 2 * kMfmaFragElems, st>>>(
+        g, reinterpret_cast<const u4_t*>(wfrag), nx, ny);
     return hipGetLastError();
 }
 
-hipError_t launch_pf_paths_mfma(const MfmaGrid& g, const float* wfrag, float* curves, double* logb,
-                                hipStream_t st)
+hipError_t launch_pf_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, float* curves,
+                                double* logb, hipStream_t st)
 {
     const dim3 grid((unsigned)((g.paths + kMfmaPathsPerBlock - 1) / kMfmaPathsPerBlock));
-    pf_paths_mfma_kernel<<<grid, kMfmaBlock, sizeof(float) * kMfmaPfFragFloats, st>>>(g, wfrag, curves,
-                                                                                      logb);
+    pf_paths_mfma_kernel<<<grid, kMfmaBlock, 2 * kMfmaPfFragElems, st>>>(
+        g, reinterpret_cast<const u4_t*>(wfrag), curves, logb);
     return hipGetLastError();
 }
 

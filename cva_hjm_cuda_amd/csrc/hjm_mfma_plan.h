@@ -1,24 +1,34 @@
 // hjm_mfma_plan.h -- host construction of the 50-step block maps for the matrix-core
 // FP32 path kernels.
 //
-// One block of `stride` Euler steps is affine in the curve f, the scaled shocks
+// One block of S = 50 Euler steps is affine in the curve f, the scaled shocks
 // s_{t,k} = sqrt(dt) phi_k(t) and 1:
 //   f_out  = T^S f_in + sum_t T^{S-1-t} (m dt + V s_t)
 // with T = I + (dt/dtau) Dfwd (backward difference on the last tenor). Two maps:
-//   reference semantics: rows 51..101 hold the block's discount-integral increment
-//     dA = sum_{r=1}^{S} f_r = (sum_{r=1}^{S} T^r) f_in + sum_t (sum_{q=0}^{S-1-t} T^q)(m dt + V s_t)
-//     (7 M-tiles of 16 rows);
-//   portfolio semantics: row 51 holds the short rate's left-point sum
-//     sum_{r=0}^{S-1} f_r[0] = (sum_{r=0}^{S-1} T^r f_in)[0] + sum_t (sum_{q=0}^{S-2-t} T^q (m dt + V s_t))[0]
-//     (4 M-tiles; rows 52..63 are zero).
-// The kernel computes W x [f_in; phi (3 per step); 1] with MFMA 16x16x4 f32 tiles. K is
-// laid out in chunks of 4 so that each lane group g = lane/16 of a chunk is one K index:
-//   chunk kc < 16      : the previous block's output registers, kc = 4 m + r holds rows
-//                        16 m + 4 g + r (zero columns where that row is not a curve row)
-//   chunk 16 + 3 q + k : factor k of step t = 4 q + g (t < S); step S (q = 12, g = 2) k = 0
-//                        is the constant 1 carrying the drift; other padding is zero.
-// Wfrag[m][kc][lane] = W[16 m + lane % 16][K(kc, lane / 16)] is the A-operand fragment.
+//   reference semantics (102 rows, 7 M-tiles of 16): rows 51..101 hold the block's
+//     discount-integral increments dA = sum_{r=1}^{S} f_r
+//       = (sum_{r=1}^{S} T^r) f_in + sum_t (sum_{q=0}^{S-1-t} T^q)(m dt + V s_t);
+//   portfolio semantics (52 rows, 4 M-tiles): row 51 holds the short rate's left-point sum
+//       sum_{r=0}^{S-1} f_r[0] = (sum_{r=0}^{S-1} T^r f_in)[0]
+//                               + sum_t (sum_{q=0}^{S-2-t} T^q (m dt + V s_t))[0].
+// The kernel computes W x [f_in; phi; 1] with v_mfma_f32_16x16x32_bf16. K has 224 entries
+// in 7 chunks of 32; in chunk c lane group g = lane / 16 supplies slots i = 0..7, the
+// K index being 32 c + 8 g + i:
+//   chunks 0, 1: curve row R = 16 (2c + i/4) + 4 g + i%4 (zero column where R > 50): the
+//     previous block's accumulator registers 8c + i of the lane, unchanged;
+//   chunks 2..6 (s = c - 2): value v = 8 s + i of lane group g, which generates the steps
+//     t = 4 q + g, q = 0..12: v = 3 q + k is factor k of step t (t < 50); at t = 50
+//     (g = 2, q = 12) factor 0 is the constant 1 carrying the drift; other slots are zero.
+// W (FP64) is split into three bf16 parts W = Wh + Wm + Wl (round to nearest); the kernel
+// splits its FP32 operand x exactly (truncation: xh + xm + xl = x) and accumulates the six
+// products Wh xh + Wh xm + Wm xh + Wh xl + Wl xh + Wm xm in FP32, which leaves an error of
+// the order of FP32 rounding.
+// Fragment [m][c][part][lane][i] = part of W[16 m + lane % 16][32 c + 8 (lane / 16) + i].
 #pragma once
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
 #include <vector>
 
 #include "hjm_common.h"
@@ -28,20 +38,48 @@ namespace hjm {
 constexpr int kMfmaStride = 50;   // steps per block handled by the matrix-core kernels
 constexpr int kMfmaMTiles = 7;    // reference map: 102 output rows -> 7 x 16
 constexpr int kMfmaPfMTiles = 4;  // portfolio map: 52 output rows -> 4 x 16
-constexpr int kMfmaFChunks = 16;  // previous block's rows 0..63 as B operand
-constexpr int kMfmaQuads = 13;    // ceil(50 / 4) step quads
-constexpr int kMfmaKChunks = kMfmaFChunks + 3 * kMfmaQuads;  // 55
-constexpr int kMfmaFragFloats = kMfmaMTiles * kMfmaKChunks * kWave;
-constexpr int kMfmaPfFragFloats = kMfmaPfMTiles * kMfmaKChunks * kWave;
+constexpr int kMfmaQuads = 13;    // steps 4 q + g, q = 0..12, per lane group
+constexpr int kMfmaChunks = 7;    // 2 curve chunks + 5 shock chunks of 32 K entries
+constexpr int kMfmaK = 32 * kMfmaChunks;  // 224
+constexpr int kMfmaParts = 3;     // bf16 parts of W
+// bf16 elements of one kernel's fragments
+constexpr int kMfmaFragElems = kMfmaMTiles * kMfmaChunks * kMfmaParts * kWave * 8;
+constexpr int kMfmaPfFragElems = kMfmaPfMTiles * kMfmaChunks * kMfmaParts * kWave * 8;
 
 enum class MfmaMap { kReference, kPortfolio };
 
-inline void build_mfma_block_map(const double* drift, const double* vols, double dt, double dtau,
-                                 std::vector<float>& wfrag, MfmaMap map = MfmaMap::kReference)
+// What K index k of the block map multiplies: a curve row, a shock (step, factor), the
+// constant 1, or nothing.
+struct MfmaColumn {
+    enum Kind { kZero, kCurve, kShock, kConst } kind;
+    int row;     // kCurve
+    int step;    // kShock
+    int factor;  // kShock
+};
+
+inline MfmaColumn mfma_column(int k)
+{
+    const int c = k / 32, g = (k % 32) / 8, i = k % 8;
+    if (c < 2) {
+        const int R = 16 * (2 * c + i / 4) + 4 * g + i % 4;
+        if (R < kTenors) return {MfmaColumn::kCurve, R, 0, 0};
+        return {MfmaColumn::kZero, 0, 0, 0};
+    }
+    const int v = 8 * (c - 2) + i;
+    if (v >= 3 * kMfmaQuads) return {MfmaColumn::kZero, 0, 0, 0};
+    const int q = v / 3, f = v % 3, t = 4 * q + g;
+    if (t < kMfmaStride) return {MfmaColumn::kShock, 0, t, f};
+    if (t == kMfmaStride && f == 0) return {MfmaColumn::kConst, 0, 0, 0};
+    return {MfmaColumn::kZero, 0, 0, 0};
+}
+
+// Dense FP64 block map [rows][224]: rows = 102 (reference) or 52 (portfolio).
+inline std::vector<double> mfma_block_map_dense(const double* drift, const double* vols, double dt,
+                                                double dtau, MfmaMap map)
 {
     const bool pf = map == MfmaMap::kPortfolio;
-    const int mtiles = pf ? kMfmaPfMTiles : kMfmaMTiles;
     const int n = kTenors, S = kMfmaStride;
+    const int rows = pf ? n + 1 : 2 * n;
     const double c = dt / dtau, sq = sqrt(dt);
     auto matmul = [n](const std::vector<double>& A, const std::vector<double>& B) {
         std::vector<double> C((size_t)n * n, 0.0);
@@ -70,75 +108,97 @@ inline void build_mfma_block_map(const double* drift, const double* vols, double
         Cs[s] = Cs[s - 1];
         for (size_t i = 0; i < Cs[s].size(); ++i) Cs[s][i] += Tp[s][i];
     }
-    const int K = 4 * kMfmaKChunks;
-    std::vector<double> W((size_t)mtiles * 16 * K, 0.0);
-    auto wref = [&](int row, int kidx) -> double& { return W[(size_t)row * K + kidx]; };
     auto mv = [n](const std::vector<double>& M, const double* v, int j) {
         double s = 0.0;
         for (int i = 0; i < n; ++i) s += M[(size_t)j * n + i] * v[i];
         return s;
     };
-    // row sums of the integral part: reference row n + j, portfolio row n (tenor 0 only)
-    auto integral_cols = [&](int kidx, const std::vector<double>& M, const double* v) {
-        if (pf) {
-            wref(n, kidx) = v ? mv(M, v, 0) : 0.0;
-        } else {
-            for (int j = 0; j < n; ++j) wref(n + j, kidx) = v ? mv(M, v, j) : 0.0;
-        }
-    };
     std::vector<double> mdt(n);
     for (int i = 0; i < n; ++i) mdt[i] = drift[i] * dt;
-    for (int kc = 0; kc < kMfmaKChunks; ++kc) {
-        for (int g = 0; g < 4; ++g) {
-            const int kidx = 4 * kc + g;
-            if (kc < kMfmaFChunks) {
-                const int m = kc / 4, r = kc % 4, R = 16 * m + 4 * g + r;
-                if (R >= n) continue;  // rows 51..63 of the previous output are not curve rows
-                for (int j = 0; j < n; ++j) wref(j, kidx) = Tp[S][(size_t)j * n + R];
-                if (pf) {
-                    wref(n, kidx) = Cs[S - 1][(size_t)0 * n + R];
-                } else {
-                    for (int j = 0; j < n; ++j) {
-                        double sa = 0.0;
-                        for (int q = 1; q <= S; ++q) sa += Tp[q][(size_t)j * n + R];
-                        wref(n + j, kidx) = sa;
-                    }
-                }
+    std::vector<double> W((size_t)rows * kMfmaK, 0.0);
+    for (int k = 0; k < kMfmaK; ++k) {
+        const MfmaColumn col = mfma_column(k);
+        auto put = [&](int row, double v) { W[(size_t)row * kMfmaK + k] = v; };
+        if (col.kind == MfmaColumn::kCurve) {
+            const int R = col.row;
+            for (int j = 0; j < n; ++j) put(j, Tp[S][(size_t)j * n + R]);
+            if (pf) {
+                put(n, Cs[S - 1][R]);
             } else {
-                const int q = (kc - kMfmaFChunks) / 3, k = (kc - kMfmaFChunks) % 3;
-                const int t = 4 * q + g;
-                if (t < S) {
-                    std::vector<double> sv(n);
-                    for (int i = 0; i < n; ++i) sv[i] = sq * vols[k * n + i];
-                    for (int j = 0; j < n; ++j) wref(j, kidx) = mv(Tp[S - 1 - t], sv.data(), j);
-                    if (pf) {
-                        if (t <= S - 2) integral_cols(kidx, Cs[S - 2 - t], sv.data());
-                    } else {
-                        integral_cols(kidx, Cs[S - 1 - t], sv.data());
-                    }
-                } else if (t == S && k == 0) {
-                    for (int j = 0; j < n; ++j) {
-                        double sf = 0.0, sa = 0.0;
-                        for (int tt = 0; tt < S; ++tt) {
-                            sf += mv(Tp[S - 1 - tt], mdt.data(), j);
-                            if (!pf) sa += mv(Cs[S - 1 - tt], mdt.data(), j);
-                            else if (j == 0 && tt <= S - 2) sa += mv(Cs[S - 2 - tt], mdt.data(), 0);
-                        }
-                        wref(j, kidx) = sf;
-                        if (!pf) wref(n + j, kidx) = sa;
-                        else if (j == 0) wref(n, kidx) = sa;
-                    }
+                for (int j = 0; j < n; ++j) {
+                    double sa = 0.0;
+                    for (int q = 1; q <= S; ++q) sa += Tp[q][(size_t)j * n + R];
+                    put(n + j, sa);
                 }
+            }
+        } else if (col.kind == MfmaColumn::kShock) {
+            const int t = col.step;
+            std::vector<double> sv(n);
+            for (int i = 0; i < n; ++i) sv[i] = sq * vols[col.factor * n + i];
+            for (int j = 0; j < n; ++j) put(j, mv(Tp[S - 1 - t], sv.data(), j));
+            if (pf) {
+                if (t <= S - 2) put(n, mv(Cs[S - 2 - t], sv.data(), 0));
+            } else {
+                for (int j = 0; j < n; ++j) put(n + j, mv(Cs[S - 1 - t], sv.data(), j));
+            }
+        } else if (col.kind == MfmaColumn::kConst) {
+            for (int j = 0; j < n; ++j) {
+                double sf = 0.0, sa = 0.0;
+                for (int tt = 0; tt < S; ++tt) {
+                    sf += mv(Tp[S - 1 - tt], mdt.data(), j);
+                    if (!pf) sa += mv(Cs[S - 1 - tt], mdt.data(), j);
+                    else if (j == 0 && tt <= S - 2) sa += mv(Cs[S - 2 - tt], mdt.data(), 0);
+                }
+                put(j, sf);
+                if (!pf) put(n + j, sa);
+                else if (j == 0) put(n, sa);
             }
         }
     }
-    wfrag.assign((size_t)mtiles * kMfmaKChunks * kWave, 0.f);
+    return W;
+}
+
+// bf16 bits of x rounded to nearest (ties to even); x finite.
+inline uint16_t bf16_rne(double x)
+{
+    const float f = (float)x;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+inline double bf16_value(uint16_t b)
+{
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+// Fragments of the three bf16 parts of W in the kernel's A-operand order.
+inline void build_mfma_block_map(const double* drift, const double* vols, double dt, double dtau,
+                                 std::vector<uint16_t>& frag, MfmaMap map = MfmaMap::kReference)
+{
+    const bool pf = map == MfmaMap::kPortfolio;
+    const int mtiles = pf ? kMfmaPfMTiles : kMfmaMTiles;
+    const int rows = pf ? kTenors + 1 : 2 * kTenors;
+    const std::vector<double> W = mfma_block_map_dense(drift, vols, dt, dtau, map);
+    frag.assign((size_t)mtiles * kMfmaChunks * kMfmaParts * kWave * 8, 0);
     for (int m = 0; m < mtiles; ++m)
-        for (int kc = 0; kc < kMfmaKChunks; ++kc)
-            for (int l = 0; l < kWave; ++l) {
-                const int row = 16 * m + (l % 16), kidx = 4 * kc + l / 16;
-                wfrag[((size_t)m * kMfmaKChunks + kc) * kWave + l] = (float)W[(size_t)row * K + kidx];
-            }
+        for (int c = 0; c < kMfmaChunks; ++c)
+            for (int l = 0; l < kWave; ++l)
+                for (int i = 0; i < 8; ++i) {
+                    const int row = 16 * m + l % 16, k = 32 * c + 8 * (l / 16) + i;
+                    const double w = row < rows ? W[(size_t)row * kMfmaK + k] : 0.0;
+                    const uint16_t h = bf16_rne(w);
+                    const double r1 = w - bf16_value(h);
+                    const uint16_t mm = bf16_rne(r1);
+                    const uint16_t lo = bf16_rne(r1 - bf16_value(mm));
+                    const uint16_t parts[3] = {h, mm, lo};
+                    for (int p = 0; p < kMfmaParts; ++p)
+                        frag[((((size_t)m * kMfmaChunks + c) * kMfmaParts + p) * kWave + l) * 8 + i] = parts[p];
+                }
 }
 
 }  // namespace hjm

@@ -231,9 +231,10 @@ int hjm_calculate_exposure_multi_gpu_f32(float* expected_exposure, float notiona
                                          int num_gpus, int scenarios, float dt);
 
 /* The affine 50-step block map of a matrix-core path kernel as a dense FP32 matrix with
- * 220 columns in the kernel's K order, for inspection and tests. Host only.
- * portfolio = 0: reference semantics, [102][220] (rows: curve after the block, then the
- * block's discount-integral increments); portfolio = 1: [52][220] (rows: curve after the
+ * 224 columns in the kernel's K order (2 chunks of 32 for the previous curve, 5 for the
+ * block's shocks and the drift constant), for inspection and tests. Host only.
+ * portfolio = 0: reference semantics, [102][224] (rows: curve after the block, then the
+ * block's discount-integral increments); portfolio = 1: [52][224] (rows: curve after the
  * block, then the short rate's left-point sum over the block's 50 steps). */
 int hjm_block_map(const double* drifts, const double* volatilities, double dt, double dtau,
                   int32_t portfolio, float* out);

@@ -12,21 +12,25 @@ import pytest
 
 import cva_hjm_cuda_amd as hjm
 
-TENORS, STRIDE, KCH, FCH = 51, 50, 55, 16
+TENORS, STRIDE, QUADS, K = 51, 50, 13, 224
 
 
 def _k_vector(f_in, phi):
-    """Input column in the kernel's K order: f rows by (chunk, lane group), then shocks."""
-    x = np.zeros(4 * KCH)
-    for kc in range(KCH):
-        for g in range(4):
-            k = 4 * kc + g
-            if kc < FCH:
-                R = 16 * (kc // 4) + 4 * g + kc % 4
-                if R < TENORS:
-                    x[k] = f_in[R]
-            else:
-                q, fac = divmod(kc - FCH, 3)
+    """Input column in the kernel's K order: index 32 c + 8 g + i (chunk c, lane group g,
+    slot i); chunks 0, 1 carry curve rows 16 (2c + i/4) + 4 g + i%4, chunks 2..6 carry value
+    v = 8 (c - 2) + i of lane group g: factor v % 3 of step 4 (v // 3) + g, the constant 1 at
+    step 50 factor 0."""
+    x = np.zeros(K)
+    for k in range(K):
+        c, g, i = k // 32, (k % 32) // 8, k % 8
+        if c < 2:
+            R = 16 * (2 * c + i // 4) + 4 * g + i % 4
+            if R < TENORS:
+                x[k] = f_in[R]
+        else:
+            v = 8 * (c - 2) + i
+            if v < 3 * QUADS:
+                q, fac = divmod(v, 3)
                 t = 4 * q + g
                 if t < STRIDE:
                     x[k] = phi[t, fac]
@@ -55,7 +59,7 @@ def market():
 
 def test_block_map_equals_fifty_euler_steps(market):
     W = hjm.block_map(market).astype(np.float64)
-    assert W.shape == (102, 220)
+    assert W.shape == (102, 224)
     rng = np.random.default_rng(5)
     vols = market.volatilities.reshape(3, TENORS)
     for _ in range(3):
@@ -65,11 +69,12 @@ def test_block_map_equals_fifty_euler_steps(market):
         y = W @ _k_vector(f_in, phi)
         np.testing.assert_allclose(y[:TENORS], f_out, rtol=0, atol=2e-8)
         np.testing.assert_allclose(y[TENORS:], area, rtol=2e-7, atol=0)
-    # the columns of rows 51..63 of the previous output are zero (not curve rows)
-    for kc in range(FCH):
-        for g in range(4):
-            if 16 * (kc // 4) + 4 * g + kc % 4 >= TENORS:
-                assert not np.any(W[:, 4 * kc + g])
+    # every curve row, every (step, factor) and the constant appear exactly once in K;
+    # the remaining columns are zero
+    x = _k_vector(np.arange(TENORS) + 2.0, 1000.0 + np.arange(3 * STRIDE).reshape(STRIDE, 3))
+    used = x[x != 0]
+    assert len(used) == len(set(used)) == TENORS + 3 * STRIDE + 1
+    assert not np.any(W[:, x == 0])
 
 
 def test_block_map_chain_matches_oracle_paths(market, oracle):
@@ -91,7 +96,7 @@ def test_block_map_chain_matches_oracle_paths(market, oracle):
 
 def test_portfolio_block_map_equals_fifty_euler_steps(market):
     W = hjm.block_map(market, portfolio=True).astype(np.float64)
-    assert W.shape == (52, 220)
+    assert W.shape == (52, 224)
     rng = np.random.default_rng(9)
     vols = market.volatilities.reshape(3, TENORS)
     for _ in range(3):
