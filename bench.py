@@ -158,6 +158,18 @@ def run_calibration(hjm, args, ncurves=2600, window=500):
     }
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    passes (copy-calibrated), or None when no measurement of that kernel is recorded."""
+    path = os.path.join(ROOT, "profiles", "ref_paths_mfma_hbm_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return rec.get("traffic_bytes") if rec.get("kernel") == kernel else None
+
+
 def cpu_baseline(market, target_s=10.0, nthreads=1):
     """The oracle's reference loop (simulation_cpu.h:413-436) in FP64 on `nthreads` threads
     (OpenMP over paths), timed around the path loop only, on chunks of the same workload
@@ -325,7 +337,7 @@ def main():
         "roofline": {"bound": "mfma" if args.path_kernel == "matrix" else "valu",
                      "achieved": achieved, "peak": PEAK_TFLOPS["f32"],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f32"],
-                     "traffic": None,
+                     "traffic": measured_traffic(KERNEL_NAMES[args.path_kernel]),
                      "kernel": KERNEL_NAMES[args.path_kernel],
                      "kernel_ms": k32,
                      "flops_per_launch": flops},
