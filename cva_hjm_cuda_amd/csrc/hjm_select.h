@@ -7,7 +7,10 @@
 //     of the 2048 bins (two bins per thread: wave scans + a scan of the 16 wave totals);
 //   * as soon as the chosen bin holds at most kSelCand elements they are copied to LDS
 //     (one more pass over the row) and the remaining digits are resolved there, so a
-//     64-bit row is streamed from HBM three or four times rather than six.
+//     64-bit row is streamed from HBM three or four times rather than six;
+//   * exact zeros (exposures of out-of-the-money paths, often most of a row) are counted
+//     per thread instead of by LDS atomics on one bin, and when a row holds at least k
+//     zeros the answer is 0 after the first pass.
 #pragma once
 #include <stdint.h>
 
@@ -43,12 +46,13 @@ __device__ void select_kth_row(const T* __restrict__ row, int64_t n, int64_t k, 
     __shared__ uint32_t wtot[kWaves];
     __shared__ U s_prefix;
     __shared__ int64_t s_k;
-    __shared__ uint32_t s_count, s_ncand;
+    __shared__ uint32_t s_count, s_ncand, s_zeros;
 
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
     if (tid == 0) {
         s_prefix = 0;
         s_k = k;
+        s_zeros = 0;
     }
     bool collected = false;
     uint32_t ncand = 0;
@@ -63,10 +67,20 @@ __device__ void select_kth_row(const T* __restrict__ row, int64_t n, int64_t k, 
         const int64_t kk = s_k;
         auto matches = [&](U bits) { return known == 0 || (bits >> (NB - known)) == (prefix >> (NB - known)); };
         if (!collected) {
+            // zeros lie in bin 0 of every pass whose prefix they match: count them apart
+            const bool zero_in = matches((U)0);
+            uint32_t zc = 0;
             for (int64_t t = tid; t < n; t += kSelBlock) {
                 const U bits = __builtin_bit_cast(U, row[t]);
-                if (matches(bits)) atomicAdd(&hist[(uint32_t)((bits >> shift) & mask)], 1u);
+                if (bits == 0) {
+                    zc += zero_in ? 1u : 0u;
+                } else if (matches(bits)) {
+                    atomicAdd(&hist[(uint32_t)((bits >> shift) & mask)], 1u);
+                }
             }
+            for (int o = kWave / 2; o > 0; o >>= 1) zc += __shfl_xor(zc, o, kWave);
+            if (lane == 0 && zc) atomicAdd(&hist[0], zc);
+            if (known == 0 && lane == 0 && zc) atomicAdd(&s_zeros, zc);
         } else {
             for (uint32_t t = tid; t < ncand; t += kSelBlock) {
                 const U bits = cand[t];
@@ -74,6 +88,11 @@ __device__ void select_kth_row(const T* __restrict__ row, int64_t n, int64_t k, 
             }
         }
         __syncthreads();
+        if (known == 0 && (int64_t)s_zeros >= kk) {
+            // at least k exact zeros: the k-th smallest of a non-negative row is 0 (the
+            // prefix is still 0 after the first pass's histogram)
+            break;
+        }
         // exclusive scan of the bins, two per thread
         const uint32_t h0 = hist[2 * tid], h1 = hist[2 * tid + 1];
         uint32_t incl = h0 + h1;
