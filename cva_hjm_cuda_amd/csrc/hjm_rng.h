@@ -5,7 +5,8 @@
 // 64-bit seed, with counter (n, p, 0, 0). Box-Muller turns the four 32-bit outputs into
 // three N(0,1) floats: z0, z1 = r cos, r sin of (x0, x1); z2 = r' cos of (x2, x3). ln,
 // sin and cos are fixed FP32 polynomials evaluated with explicit FMAs (coefficients from
-// rng_poly_fit.py), so host and GPU produce the same bits; sqrtf is IEEE-rounded on both.
+// rng_poly_fit.py), so host and GPU produce the same bits; the square root is IEEE-rounded on
+// both.
 //
 // Counter addressing makes any (path, step) directly reachable: paths shard across
 // devices without stream offsets, and the adjoint sweep regenerates shocks in reverse.
@@ -23,6 +24,16 @@ struct Normals3 {
     float z0, z1, z2;
 };
 
+// a ^ b ^ c; one v_bitop3_b32 (truth table 0x96) on the GPU
+HJM_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 HJM_HD void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
                           uint32_t k1, uint32_t out[4])
 {
@@ -32,8 +43,8 @@ HJM_HD void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, ui
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)M0 * c0;
         const uint64_t p1 = (uint64_t)M1 * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
@@ -110,6 +121,26 @@ HJM_HD float rng_cos_turn24(uint32_t k24)
     return c;
 }
 
+// IEEE-rounded square root for x == 0 and x >= 2^-96 (the Box-Muller radii lie in
+// [0, 33.3]): on the GPU, v_sqrt_f32 (within 1 ulp) and a one-ulp correction from the two
+// neighbours' residuals, the hardware-rate form of the correctly rounded expansion
+// without its tiny-argument rescaling.
+HJM_HD float rng_sqrt(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma clang fp contract(off)
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float dn = bits_to_float(float_to_bits(s) - 1u);
+    const float up = bits_to_float(float_to_bits(s) + 1u);
+    const float rdn = __builtin_fmaf(-dn, s, x);
+    const float rup = __builtin_fmaf(-up, s, x);
+    const float t = (rdn <= 0.0f) ? dn : s;
+    return (rup > 0.0f) ? up : t;
+#else
+    return __builtin_sqrtf(x);
+#endif
+}
+
 HJM_HD Normals3 step_normals(uint32_t seed_lo, uint32_t seed_hi, uint32_t path, uint32_t step)
 {
 #pragma clang fp contract(off)
@@ -117,8 +148,8 @@ HJM_HD Normals3 step_normals(uint32_t seed_lo, uint32_t seed_hi, uint32_t path, 
     philox4x32_10(step, path, 0u, 0u, seed_lo, seed_hi, x);
     const float u0 = (float)((x[0] >> 8) + 1u) * 0x1p-24f;
     const float u2 = (float)((x[2] >> 8) + 1u) * 0x1p-24f;
-    const float r0 = __builtin_sqrtf(-2.0f * rng_log(u0));
-    const float r2 = __builtin_sqrtf(-2.0f * rng_log(u2));
+    const float r0 = rng_sqrt(-2.0f * rng_log(u0));
+    const float r2 = rng_sqrt(-2.0f * rng_log(u2));
     float s0, c0;
     rng_sincos_turn24(x[1] >> 8, s0, c0);
     const float c2 = rng_cos_turn24(x[3] >> 8);

@@ -56,6 +56,15 @@ def test_gpu_normals_bit_identical_to_oracle(oracle):
         assert np.array_equal(g, o), seed
 
 
+def test_gpu_normals_bit_identical_large_sample(oracle):
+    """2048 paths x 1000 steps (6.1 M normals, 2 M Box-Muller radii through the GPU's
+    square root and logarithm) against the oracle, bit for bit."""
+    _flush("[gpu] normals, large sample")
+    g = hjm.generate_normals(7, 2048, 1000, path_offset=99991)
+    o = oracle.normals(7, 2048, 1000, path0=99991)
+    assert np.array_equal(g, o)
+
+
 def test_fp64_host_normals_matches_oracle(c1):
     _flush("[gpu] config-1 FP64 host normals")
     r = hjm.simulate_reference(c1["m"], c1["swap"], C1["paths"], normals=c1["z"],
