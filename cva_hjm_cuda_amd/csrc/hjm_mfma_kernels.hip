@@ -154,16 +154,24 @@ __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const
     for (int n = 0; n < kNT; ++n) z[n][3 * kMfmaQuads] = 0.f;
     auto quad = [&](auto qc) {
         constexpr int q = decltype(qc)::value;
+        // steps 4 q + grp < 50 for every lane group up to q = 11; only the last quad pads
+        constexpr bool last = 4 * q + 3 >= kMfmaStride;
         const int t = 4 * q + grp;
-        const int tc = t < kMfmaStride ? t : kMfmaStride - 1;
-        const bool in = t < kMfmaStride;
+        const int tc = (!last || t < kMfmaStride) ? t : kMfmaStride - 1;
+        const bool in = !last || t < kMfmaStride;
 #pragma unroll
         for (int n = 0; n < kNT; ++n) {
             const Normals3 v = step_normals(g.seed_lo, g.seed_hi, (uint32_t)(g.path0 + path[n]),
                                             (uint32_t)(b * kMfmaStride + tc));
-            z[n][3 * q + 0] = in ? v.z0 : (t == kMfmaStride ? 1.f : 0.f);
-            z[n][3 * q + 1] = in ? v.z1 : 0.f;
-            z[n][3 * q + 2] = in ? v.z2 : 0.f;
+            if constexpr (last) {
+                z[n][3 * q + 0] = in ? v.z0 : (t == kMfmaStride ? 1.f : 0.f);
+                z[n][3 * q + 1] = in ? v.z1 : 0.f;
+                z[n][3 * q + 2] = in ? v.z2 : 0.f;
+            } else {
+                z[n][3 * q + 0] = v.z0;
+                z[n][3 * q + 1] = v.z1;
+                z[n][3 * q + 2] = v.z2;
+            }
         }
 #pragma unroll
         for (int s = 0; s < kMfmaChunks - 2; ++s)
@@ -173,7 +181,19 @@ __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const
         // this quad's share of the matrix work (its operands are complete); it runs on
         // the matrix cores beside the RNG of this and the other waves
         constexpr IssueSlot is = issue_slot<MT>(q, 0);
-        if constexpr (is.chunk >= 0) tiles(is.chunk, is.m0, is.m1, xs[is.chunk]);
+        if constexpr (is.chunk >= 0) {
+            tiles(is.chunk, is.m0, is.m1, xs[is.chunk]);
+            if constexpr (MT == kMfmaMTiles) {
+                // within the quad: one MFMA, then a share of the RNG's VALU instructions
+#pragma unroll
+                for (int i = 0; i < (is.m1 - is.m0) * kNT * 6; ++i) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x002, 7, 0);
+                }
+            }
+        }
+        // each quad is its own scheduling region (keeps the group solver's problems small)
+        if constexpr (MT == kMfmaMTiles) __builtin_amdgcn_sched_barrier(0);
     };
     for_each_index(quad, std::make_integer_sequence<int, kMfmaQuads>{});
     tiles(kMfmaChunks - 1, 0, MT, xs[kMfmaChunks - 1]);
