@@ -105,9 +105,13 @@ __global__ __launch_bounds__(kBlock) void pf_paths_kernel(const RefStepConsts<Re
     }
 }
 
-// Netting-set exposures at every date for one path per lane (dates uniform per wave).
+// Netting-set exposures at every date for one path per lane (one date per workgroup).
 // coef: [date][set][kpad] gamma (k = 1..kpad stored at index k-1), amax: [date][set] a,
-// kmax: [date][set] the last non-zero k (0 = no live swap).
+// kmax: [date][set] the last non-zero k (0 = no live swap). The date's coefficient rows are
+// staged in LDS, kValueSets sets at a time, and read there with workgroup-uniform
+// addresses (broadcasts).
+constexpr int kValueSets = 64;
+
 template <typename Real>
 __global__ __launch_bounds__(kBlock) void pf_value_kernel(const PfValueParams v,
                                                           const Real* __restrict__ curves,
@@ -117,15 +121,19 @@ __global__ __launch_bounds__(kBlock) void pf_value_kernel(const PfValueParams v,
                                                           const int* __restrict__ kmax,
                                                           Real* __restrict__ exposures)
 {
+    constexpr int KP = kMaxPeriods;
+    __shared__ Real sg[kValueSets * KP];
+    __shared__ Real sa[kValueSets];
+    __shared__ int sk[kValueSets];
+    const int e = blockIdx.y;
     const int64_t P = v.paths;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int e = blockIdx.y;
-    if (i >= P) return;
+    const bool live = i < P;
+    const int64_t ic = live ? i : 0;
     Real f[kTenors];
 #pragma unroll
-    for (int j = 0; j < kTenors; ++j) f[j] = curves[((int64_t)e * kTenors + j) * P + i];
-    const double disc = exp(-logb[(int64_t)e * P + i]);
-    constexpr int KP = kMaxPeriods;
+    for (int j = 0; j < kTenors; ++j) f[j] = curves[((int64_t)e * kTenors + j) * P + ic];
+    const double disc = exp(-logb[(int64_t)e * P + ic]);
     Real Pk[KP];
     Real I = (Real)0;
     const Real hw = (Real)(0.5 * v.dtau);
@@ -135,30 +143,41 @@ __global__ __launch_bounds__(kBlock) void pf_value_kernel(const PfValueParams v,
         else I += (Real)v.dtau * f[kTenors - 1];
         Pk[kk - 1] = exp(-I);
     }
-    kconst_ptr_pf<Real> cp = (kconst_ptr_pf<Real>)coef;
-    kconst_ptr_pf<Real> ap = (kconst_ptr_pf<Real>)aconst;
-    kconst_ptr_pf<int> kp = (kconst_ptr_pf<int>)kmax;
-    for (int gset = 0; gset < v.nsets; ++gset) {
-        const int64_t row = (int64_t)e * v.nsets + gset;
-        const int K = kp[row];
-        kconst_ptr_pf<Real> gam = cp + row * KP;
-        Real V = ap[row];
-        // chunks of 4 periods up to the set's last live period; the guard is uniform
-        // and the indices into Pk are constants, so Pk stays in VGPRs
-#pragma unroll
-        for (int k0 = 0; k0 < KP; k0 += 4) {
-            if (k0 < K) {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) V -= gam[k0 + u] * Pk[k0 + u];
-            }
+    for (int g0 = 0; g0 < v.nsets; g0 += kValueSets) {
+        const int ng = v.nsets - g0 < kValueSets ? v.nsets - g0 : kValueSets;
+        __syncthreads();
+        const int64_t base = (int64_t)e * v.nsets + g0;
+        for (int t = threadIdx.x; t < ng * KP; t += blockDim.x) sg[t] = coef[base * KP + t];
+        for (int t = threadIdx.x; t < ng; t += blockDim.x) {
+            sa[t] = aconst[base + t];
+            sk[t] = kmax[base + t];
         }
-        const Real ex = V > (Real)0 ? (Real)((double)V * disc) : (Real)0;
-        exposures[(row)*P + i] = ex;
+        __syncthreads();
+        for (int gg = 0; gg < ng; ++gg) {
+            const int K = sk[gg];
+            const Real* gam = sg + gg * KP;
+            Real V = sa[gg];
+            // chunks of 4 periods up to the set's last live period; the guard is uniform
+            // and the indices into Pk are constants, so Pk stays in VGPRs
+#pragma unroll
+            for (int k0 = 0; k0 < KP; k0 += 4) {
+                if (k0 < K) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) V -= gam[k0 + u] * Pk[k0 + u];
+                }
+            }
+            const Real ex = V > (Real)0 ? (Real)((double)V * disc) : (Real)0;
+            if (live) exposures[(base + gg) * P + i] = ex;
+        }
     }
 }
 
-// Per set g (blockIdx.y) and 256 paths (blockIdx.x): EE numerators at every date and the
-// per-path CVA sample's sum and square sum; fixed-order reductions.
+// Per set g (blockIdx.y) and kStatPaths paths (blockIdx.x; each thread takes kStatPerThread
+// of them, 256 apart, so one wave reduction serves 8 exposures per lane): EE numerators at
+// every date and the per-path CVA sample's sum and square sum; fixed-order reductions.
+constexpr int kStatPerThread = 8;
+constexpr int kStatPaths = kBlock * kStatPerThread;
+
 template <typename Real>
 __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
                                                           const Real* __restrict__ exposures,
@@ -168,18 +187,34 @@ __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
     __shared__ double red[kBlock / kWave][kMaxDates + 2];
     const int gset = blockIdx.y;
     const int64_t P = v.paths;
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const bool active = i < P;
+    const int64_t i0 = (int64_t)blockIdx.x * kStatPaths + threadIdx.x;
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
     const int nd = v.ndates;
-    double cva = 0.0;
+    double cva[kStatPerThread];
+#pragma unroll
+    for (int k = 0; k < kStatPerThread; ++k) cva[k] = 0.0;
     for (int e = 0; e < nd; ++e) {
-        const double x = active ? (double)exposures[((int64_t)e * v.nsets + gset) * P + i] : 0.0;
-        cva += w[(int64_t)gset * nd + e] * x;
-        const double s = wave_sum(x);
+        const Real* row = exposures + ((int64_t)e * v.nsets + gset) * P;
+        const double we = w[(int64_t)gset * nd + e];
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < kStatPerThread; ++k) {
+            const int64_t i = i0 + (int64_t)k * kBlock;
+            const double x = i < P ? (double)row[i] : 0.0;
+            s += x;
+            cva[k] += we * x;
+        }
+        s = wave_sum(s);
         if (lane == 0) red[wave][e] = s;
     }
-    const double c1 = wave_sum(cva), c2 = wave_sum(cva * cva);
+    double c1 = 0.0, c2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < kStatPerThread; ++k) {
+        c1 += cva[k];
+        c2 += cva[k] * cva[k];
+    }
+    c1 = wave_sum(c1);
+    c2 = wave_sum(c2);
     if (lane == 0) {
         red[wave][nd] = c1;
         red[wave][nd + 1] = c2;
@@ -187,10 +222,10 @@ __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
     __syncthreads();
     const int ncol = nd + 2;
     for (int c = threadIdx.x; c < ncol; c += kBlock) {
-        double s = 0.0;
+        double t = 0.0;
 #pragma unroll
-        for (int ww = 0; ww < kBlock / kWave; ++ww) s += red[ww][c];
-        partials[((int64_t)gset * gridDim.x + blockIdx.x) * ncol + c] = s;
+        for (int ww = 0; ww < kBlock / kWave; ++ww) t += red[ww][c];
+        partials[((int64_t)gset * gridDim.x + blockIdx.x) * ncol + c] = t;
     }
 }
 
@@ -269,7 +304,7 @@ template <typename Real>
 static hipError_t pf_stats_launch(const PfValueParams& v, const Real* ex, const double* w,
                                   double* partials, double* sums, hipStream_t st)
 {
-    const unsigned nb = (unsigned)((v.paths + kBlock - 1) / kBlock);
+    const unsigned nb = (unsigned)((v.paths + kStatPaths - 1) / kStatPaths);
     pf_stats_kernel<Real><<<dim3(nb, (unsigned)v.nsets), kBlock, 0, st>>>(v, ex, w, partials);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
