@@ -63,9 +63,12 @@ struct AdjGrid {
 // Sum of a lane value over the wave in FP64, fixed butterfly order.
 __device__ __forceinline__ double wave_sum_f(float v) { return wave_sum((double)v); }
 
-template <int kPass, bool kHostNormals>
+// kSrc: 0 regenerate the in-kernel normals, 1 host-supplied FP64 [path][step][3],
+// 2 the forward pass's stored FP32 normals [step][3][path].
+template <int kPass, int kSrc>
 __global__ __launch_bounds__(kBlock) void ref_adjoint_sweep_kernel(const AdjGrid g,
                                                                    const double* __restrict__ normals,
+                                                                   const float* __restrict__ zs,
                                                                    const float* __restrict__ gx,
                                                                    const float* __restrict__ gA,
                                                                    double* __restrict__ partials)
@@ -102,12 +105,20 @@ __global__ __launch_bounds__(kBlock) void ref_adjoint_sweep_kernel(const AdjGrid
         }
         for (int s = g.stride - 1; s >= 0; --s) {
             const int n = b * g.stride + s;  // global step index of this Euler step
-            float z0, z1, z2;
-            if constexpr (kHostNormals) {
+            float z0 = 0.f, z1 = 0.f, z2 = 0.f;
+            if constexpr (kSrc == 1) {
                 const double* q = normals + ((int64_t)(active ? i : 0) * g.nsteps + n) * 3;
                 z0 = (float)q[0];
                 z1 = (float)q[1];
                 z2 = (float)q[2];
+            } else if constexpr (kSrc == 2) {
+                const float* q = zs + (int64_t)n * 3 * P + (active ? i : 0);
+                if constexpr (kPass == 0) {
+                    z0 = q[0];
+                    z1 = q[P];
+                } else {
+                    z2 = q[2 * P];
+                }
             } else {
                 const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n);
                 z0 = z.z0;
@@ -194,8 +205,8 @@ hipError_t launch_ref_adjoint_seed(const ExposureParams& e, double dt, const dou
 }
 
 hipError_t launch_ref_adjoint_sweep(const RefGrid& rg, double c, double sqrt_dt, int pass,
-                                    const double* normals, const float* gx, const float* gA,
-                                    double* partials, hipStream_t st)
+                                    const double* normals, const float* zs, const float* gx,
+                                    const float* gA, double* partials, hipStream_t st)
 {
     AdjGrid g{};
     g.stride = rg.stride;
@@ -211,17 +222,19 @@ hipError_t launch_ref_adjoint_sweep(const RefGrid& rg, double c, double sqrt_dt,
     g.opc = (float)(1.0 + c);
     g.sqrt_dt = (float)sqrt_dt;
     const dim3 grid((unsigned)((rg.paths + kBlock - 1) / kBlock));
-    if (normals) {
-        if (pass == 0)
-            ref_adjoint_sweep_kernel<0, true><<<grid, kBlock, 0, st>>>(g, normals, gx, gA, partials);
-        else
-            ref_adjoint_sweep_kernel<1, true><<<grid, kBlock, 0, st>>>(g, normals, gx, gA, partials);
+#define HJM_SWEEP(P_, S_) \
+    ref_adjoint_sweep_kernel<P_, S_><<<grid, kBlock, 0, st>>>(g, normals, zs, gx, gA, partials)
+    const int src = normals ? 1 : (zs ? 2 : 0);
+    if (pass == 0) {
+        if (src == 1) HJM_SWEEP(0, 1);
+        else if (src == 2) HJM_SWEEP(0, 2);
+        else HJM_SWEEP(0, 0);
     } else {
-        if (pass == 0)
-            ref_adjoint_sweep_kernel<0, false><<<grid, kBlock, 0, st>>>(g, normals, gx, gA, partials);
-        else
-            ref_adjoint_sweep_kernel<1, false><<<grid, kBlock, 0, st>>>(g, normals, gx, gA, partials);
+        if (src == 1) HJM_SWEEP(1, 1);
+        else if (src == 2) HJM_SWEEP(1, 2);
+        else HJM_SWEEP(1, 0);
     }
+#undef HJM_SWEEP
     return hipGetLastError();
 }
 

@@ -198,11 +198,11 @@ int plan_init(hjm_plan p, const hjm_market* market, const hjm_swap* swap, const 
 }
 
 // Enqueue every kernel of one run on the plan's stream (asynchronous).
-int plan_launch(hjm_plan p, bool keep_exposures)
+int plan_launch(hjm_plan p, bool keep_exposures, float* zs = nullptr)
 {
     hipStream_t st = p->stream;
     HJM_HIP(hipEventRecord(p->ev[0], st), HJM_ERR_DEVICE);
-    hipError_t err = p->use_mfma ? launch_ref_paths_mfma(p->mg, p->d_wfrag, p->d_nx, p->d_ny, st)
+    hipError_t err = p->use_mfma ? launch_ref_paths_mfma(p->mg, p->d_wfrag, p->d_nx, p->d_ny, zs, st)
                      : (p->precision == HJM_FP64)
                          ? launch_ref_paths_f64(p->kd, p->grid, p->d_normals, p->d_nx, p->d_ny, st)
                          : launch_ref_paths_f32k(p->kf, p->grid, p->d_normals, p->d_nx, p->d_ny, st);
@@ -394,12 +394,13 @@ int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
     HJM_TRY(hjm_plan_create(&p, market, swap, cfg));
     struct Cleanup {
         hjm_plan p;
-        float *gx = nullptr, *gA = nullptr;
+        float *gx = nullptr, *gA = nullptr, *zs = nullptr;
         double *pa = nullptr, *pb = nullptr, *sa = nullptr, *sb = nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         ~Cleanup()
         {
             dev_free(gx);
+            dev_free(zs);
             dev_free(gA);
             dev_free(pa);
             dev_free(pb);
@@ -422,16 +423,20 @@ int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
     HJM_TRY(dev_alloc(&c.sb, 3 * kTenors));
     HJM_HIP(hipEventCreate(&c.e0), HJM_ERR_DEVICE);
     HJM_HIP(hipEventCreate(&c.e1), HJM_ERR_DEVICE);
+    // the matrix-core forward keeps its normals for the reverse sweeps (one FP32 per factor
+    // per evolved step and path); without the memory the sweeps regenerate them
+    if (p->use_mfma && dev_alloc(&c.zs, (size_t)p->grid.nsteps * 3 * (size_t)P) != HJM_OK)
+        c.zs = nullptr;
     hipStream_t st = p->stream;
-    HJM_TRY(plan_launch(p, false));
+    HJM_TRY(plan_launch(p, false, c.zs));
     HJM_HIP(hipEventRecord(c.e0, st), HJM_ERR_DEVICE);
     const double dt = p->grid.dt;
     const double cc = dt / swap->dtau;
     HJM_HIP(launch_ref_adjoint_seed(p->ep, dt, p->d_nx, p->d_ny, c.gx, c.gA, st), HJM_ERR_LAUNCH);
-    HJM_HIP(launch_ref_adjoint_sweep(p->grid, cc, sqrt(dt), 0, p->d_normals, c.gx, c.gA, c.pa, st),
+    HJM_HIP(launch_ref_adjoint_sweep(p->grid, cc, sqrt(dt), 0, p->d_normals, c.zs, c.gx, c.gA, c.pa, st),
             HJM_ERR_LAUNCH);
     HJM_HIP(launch_column_sum(c.pa, nb, 2 * kTenors, c.sa, st), HJM_ERR_LAUNCH);
-    HJM_HIP(launch_ref_adjoint_sweep(p->grid, cc, sqrt(dt), 1, p->d_normals, c.gx, c.gA, c.pb, st),
+    HJM_HIP(launch_ref_adjoint_sweep(p->grid, cc, sqrt(dt), 1, p->d_normals, c.zs, c.gx, c.gA, c.pb, st),
             HJM_ERR_LAUNCH);
     HJM_HIP(launch_column_sum(c.pb, nb, 3 * kTenors, c.sb, st), HJM_ERR_LAUNCH);
     HJM_HIP(hipEventRecord(c.e1, st), HJM_ERR_DEVICE);

@@ -263,7 +263,7 @@ int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfol
 
 // Enqueue the four passes on the plan's stream, with timing events between them unless
 // the sequence is being captured into a graph (a captured replay is timed as a whole).
-int pplan_enqueue(hjm_portfolio_plan p, bool events = true)
+int pplan_enqueue(hjm_portfolio_plan p, bool events = true, float* zs = nullptr)
 {
     hipStream_t st = p->stream;
     const bool f64 = p->precision == HJM_FP64;
@@ -271,7 +271,7 @@ int pplan_enqueue(hjm_portfolio_plan p, bool events = true)
     auto mark = [&](int i) { return events ? hipEventRecord(p->ev[i], st) : hipSuccess; };
     HJM_HIP(mark(0), HJM_ERR_DEVICE);
     hipError_t err =
-        p->use_mfma ? launch_pf_paths_mfma(p->mg, p->d_wfrag, (float*)p->d_curves, p->d_logb, st)
+        p->use_mfma ? launch_pf_paths_mfma(p->mg, p->d_wfrag, (float*)p->d_curves, p->d_logb, zs, st)
         : f64 ? launch_pf_paths_f64(p->kd, p->grid, p->d_normals, (double*)p->d_curves, p->d_logb,
                                     p->tuning.block_size, p->grid_blocks, st)
               : launch_pf_paths_f32(p->kf, p->grid, p->d_normals, (float*)p->d_curves, p->d_logb,
@@ -449,11 +449,12 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
     HJM_TRY(hjm_portfolio_plan_create(&p, market, portfolio, &cfg, tuning));
     struct Cleanup {
         hjm_portfolio_plan p;
-        float *gbar = nullptr, *beta = nullptr;
+        float *gbar = nullptr, *beta = nullptr, *zs = nullptr;
         double *pa = nullptr, *pb = nullptr, *sa = nullptr, *sb = nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         ~Cleanup()
         {
+            dev_free(zs);
             dev_free(gbar);
             dev_free(beta);
             dev_free(pa);
@@ -478,8 +479,12 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
     HJM_TRY(dev_alloc(&c.sb, 3 * kTenors));
     HJM_HIP(hipEventCreate(&c.e0), HJM_ERR_DEVICE);
     HJM_HIP(hipEventCreate(&c.e1), HJM_ERR_DEVICE);
+    // the matrix-core forward keeps its normals for the reverse sweeps; without the
+    // memory the sweeps regenerate them
+    if (p->use_mfma && dev_alloc(&c.zs, (size_t)p->grid.nsteps * 3 * (size_t)P) != HJM_OK)
+        c.zs = nullptr;
     hipStream_t st = p->stream;
-    HJM_TRY(pplan_enqueue(p));
+    HJM_TRY(pplan_enqueue(p, true, c.zs));
     HJM_HIP(hipEventRecord(c.e0, st), HJM_ERR_DEVICE);
     const bool f64 = p->precision == HJM_FP64;
     hipError_t err = f64 ? launch_pf_adjoint_seed_f64(p->vp, (const double*)p->d_curves, p->d_logb,
@@ -490,10 +495,10 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
                                                       p->d_kmax, p->d_w, c.gbar, c.beta, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
     const double cc = p->grid.dt / portfolio->dtau;
-    HJM_HIP(launch_pf_adjoint_sweep(p->grid, cc, 0, p->d_normals, c.gbar, c.beta, c.pa, st),
+    HJM_HIP(launch_pf_adjoint_sweep(p->grid, cc, 0, p->d_normals, c.zs, c.gbar, c.beta, c.pa, st),
             HJM_ERR_LAUNCH);
     HJM_HIP(launch_column_sum(c.pa, nb, 2 * kTenors, c.sa, st), HJM_ERR_LAUNCH);
-    HJM_HIP(launch_pf_adjoint_sweep(p->grid, cc, 1, p->d_normals, c.gbar, c.beta, c.pb, st),
+    HJM_HIP(launch_pf_adjoint_sweep(p->grid, cc, 1, p->d_normals, c.zs, c.gbar, c.beta, c.pb, st),
             HJM_ERR_LAUNCH);
     HJM_HIP(launch_column_sum(c.pb, nb, 3 * kTenors, c.sb, st), HJM_ERR_LAUNCH);
     HJM_HIP(hipEventRecord(c.e1, st), HJM_ERR_DEVICE);

@@ -30,17 +30,19 @@ hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nstep
 // Matrix-core FP32 path kernels (50 steps per date); prepare once per plan (sets the
 // kernels' dynamic LDS limits) before launching or capturing them.
 hipError_t mfma_kernels_prepare();
+// zs (optional): the forward's FP32 normals [step][factor][path] for a later reverse sweep
 hipError_t launch_ref_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, double* nx, double* ny,
-                                 hipStream_t st);
+                                 float* zs, hipStream_t st);
 hipError_t launch_pf_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, float* curves,
-                                double* logb, hipStream_t st);
+                                double* logb, float* zs, hipStream_t st);
 
 // Pathwise adjoint of the reference-semantics CVA.
 hipError_t launch_ref_adjoint_seed(const ExposureParams& e, double dt, const double* nx,
                                    const double* ny, float* gx, float* gA, hipStream_t st);
+// normals: host FP64 [path][step][3]; else zs: stored FP32 [step][3][path]; else regenerate
 hipError_t launch_ref_adjoint_sweep(const RefGrid& rg, double c, double sqrt_dt, int pass,
-                                    const double* normals, const float* gx, const float* gA,
-                                    double* partials, hipStream_t st);
+                                    const double* normals, const float* zs, const float* gx,
+                                    const float* gA, double* partials, hipStream_t st);
 
 // Portfolio semantics.
 hipError_t launch_pf_paths_f64(const RefStepConsts<double>& k, const PfGrid& g, const double* normals,
@@ -69,8 +71,8 @@ hipError_t launch_pf_adjoint_seed_f32(const PfValueParams& v, const float* curve
                                       const int* kmax, const double* w, float* gbar, float* beta,
                                       hipStream_t st);
 hipError_t launch_pf_adjoint_sweep(const PfGrid& g, double c, int pass, const double* normals,
-                                   const float* gbar, const float* beta, double* partials,
-                                   hipStream_t st);
+                                   const float* zs, const float* gbar, const float* beta,
+                                   double* partials, hipStream_t st);
 
 // The reference's own GPU design (baseline form).
 hipError_t launch_blockwise_paths(const RefStepConsts<double>& k, const RefGrid& g, int block_index,

@@ -108,11 +108,13 @@ __device__ __forceinline__ void for_each_index(F& f, std::integer_sequence<int, 
     (f(std::integral_constant<int, Q>{}), ...);
 }
 
-// One 50-step block: acc = W x [fin; shocks of steps b*50 .. b*50+49; 1].
-template <int MT>
+// One 50-step block: acc = W x [fin; shocks of steps b*50 .. b*50+49; 1]. kStore: also
+// write the block's normals to zs[step][factor][path] (FP32) for a later reverse sweep.
+template <int MT, bool kStore>
 __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const float (&fin)[kNT][16],
                                            f4_t (&acc)[MT][kNT], const MfmaGrid& g,
-                                           const int64_t (&path)[kNT], int b, int grp, int lane)
+                                           const int64_t (&path)[kNT], const bool (&live)[kNT],
+                                           int b, int grp, int lane, float* __restrict__ zs)
 {
 #pragma unroll
     for (int m = 0; m < MT; ++m)
@@ -163,6 +165,14 @@ __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const
         for (int n = 0; n < kNT; ++n) {
             const Normals3 v = step_normals(g.seed_lo, g.seed_hi, (uint32_t)(g.path0 + path[n]),
                                             (uint32_t)(b * kMfmaStride + tc));
+            if constexpr (kStore) {
+                if (in && live[n]) {
+                    float* q = zs + (int64_t)(b * kMfmaStride + t) * 3 * g.paths + path[n];
+                    q[0] = v.z0;
+                    q[g.paths] = v.z1;
+                    q[2 * g.paths] = v.z2;
+                }
+            }
             if constexpr (last) {
                 z[n][3 * q + 0] = in ? v.z0 : (t == kMfmaStride ? 1.f : 0.f);
                 z[n][3 * q + 1] = in ? v.z1 : 0.f;
@@ -199,10 +209,12 @@ __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const
     tiles(kMfmaChunks - 1, 0, MT, xs[kMfmaChunks - 1]);
 }
 
+template <bool kStore>
 __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGrid g,
                                                                 const u4_t* __restrict__ wfrag,
                                                                 double* __restrict__ num_x,
-                                                                double* __restrict__ num_y)
+                                                                double* __restrict__ num_y,
+                                                                float* __restrict__ zs)
 {
     extern __shared__ u4_t lds_w[];
     for (int t = threadIdx.x; t < kMfmaFragElems / 8; t += kMfmaBlock) lds_w[t] = wfrag[t];
@@ -255,7 +267,7 @@ __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGr
 
     for (int b = 1; b < g.ndates; ++b) {
         f4_t acc[kMfmaMTiles][kNT];
-        mfma_block<kMfmaMTiles>(lds_w, fin, acc, g, path, b, grp, lane);
+        mfma_block<kMfmaMTiles, kStore>(lds_w, fin, acc, g, path, live, b, grp, lane, zs);
         // integrals: Kahan-add the block's dA (tiles 3..6); curve rows become next input
 #pragma unroll
         for (int n = 0; n < kNT; ++n) {
@@ -302,10 +314,12 @@ __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGr
 // Portfolio semantics: dates e = 0 .. ndates-1 every 50 steps from t = 0. At each date the
 // curve goes to curves[e][tenor][path] and the log bank account dt sum_{n < 50 e} f_n(0)
 // to logb[e][path]; the block's row 51 (left-point sum of the short rate) extends it.
+template <bool kStore>
 __global__ __launch_bounds__(kMfmaBlock) void pf_paths_mfma_kernel(const MfmaGrid g,
                                                                const u4_t* __restrict__ wfrag,
                                                                float* __restrict__ curves,
-                                                               double* __restrict__ logb)
+                                                               double* __restrict__ logb,
+                                                               float* __restrict__ zs)
 {
     extern __shared__ u4_t lds_w[];
     for (int t = threadIdx.x; t < kMfmaPfFragElems / 8; t += kMfmaBlock) lds_w[t] = wfrag[t];
@@ -350,7 +364,7 @@ __global__ __launch_bounds__(kMfmaBlock) void pf_paths_mfma_kernel(const MfmaGri
         }
         if (e + 1 >= g.ndates) break;
         f4_t acc[kMfmaPfMTiles][kNT];
-        mfma_block<kMfmaPfMTiles>(lds_w, fin, acc, g, path, e, grp, lane);
+        mfma_block<kMfmaPfMTiles, kStore>(lds_w, fin, acc, g, path, live, e, grp, lane, zs);
 #pragma unroll
         for (int n = 0; n < kNT; ++n) {
 #pragma unroll
@@ -364,29 +378,38 @@ __global__ __launch_bounds__(kMfmaBlock) void pf_paths_mfma_kernel(const MfmaGri
 
 hipError_t mfma_kernels_prepare()
 {
-    hipError_t e = hipFuncSetAttribute((const void*)ref_paths_mfma_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)(2 * kMfmaFragElems));
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void*)pf_paths_mfma_kernel,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)(2 * kMfmaPfFragElems));
+    const void* fns[4] = {(const void*)ref_paths_mfma_kernel<false>, (const void*)ref_paths_mfma_kernel<true>,
+                          (const void*)pf_paths_mfma_kernel<false>, (const void*)pf_paths_mfma_kernel<true>};
+    const int lds[4] = {2 * kMfmaFragElems, 2 * kMfmaFragElems, 2 * kMfmaPfFragElems,
+                        2 * kMfmaPfFragElems};
+    for (int i = 0; i < 4; ++i) {
+        const hipError_t e = hipFuncSetAttribute(fns[i], hipFuncAttributeMaxDynamicSharedMemorySize, lds[i]);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_ref_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, double* nx, double* ny,
-                                 hipStream_t st)
+                                 float* zs, hipStream_t st)
 {
     const dim3 grid((unsigned)((g.paths + kMfmaPathsPerBlock - 1) / kMfmaPathsPerBlock));
-    ref_paths_mfma_kernel<<<grid, kMfmaBlock, 2 * kMfmaFragElems, st>>>(
-        g, reinterpret_cast<const u4_t*>(wfrag), nx, ny);
+    const u4_t* w = reinterpret_cast<const u4_t*>(wfrag);
+    if (zs)
+        ref_paths_mfma_kernel<true><<<grid, kMfmaBlock, 2 * kMfmaFragElems, st>>>(g, w, nx, ny, zs);
+    else
+        ref_paths_mfma_kernel<false><<<grid, kMfmaBlock, 2 * kMfmaFragElems, st>>>(g, w, nx, ny, zs);
     return hipGetLastError();
 }
 
 hipError_t launch_pf_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, float* curves,
-                                double* logb, hipStream_t st)
+                                double* logb, float* zs, hipStream_t st)
 {
     const dim3 grid((unsigned)((g.paths + kMfmaPathsPerBlock - 1) / kMfmaPathsPerBlock));
-    pf_paths_mfma_kernel<<<grid, kMfmaBlock, 2 * kMfmaPfFragElems, st>>>(
-        g, reinterpret_cast<const u4_t*>(wfrag), curves, logb);
+    const u4_t* w = reinterpret_cast<const u4_t*>(wfrag);
+    if (zs)
+        pf_paths_mfma_kernel<true><<<grid, kMfmaBlock, 2 * kMfmaPfFragElems, st>>>(g, w, curves, logb, zs);
+    else
+        pf_paths_mfma_kernel<false><<<grid, kMfmaBlock, 2 * kMfmaPfFragElems, st>>>(g, w, curves, logb, zs);
     return hipGetLastError();
 }
 

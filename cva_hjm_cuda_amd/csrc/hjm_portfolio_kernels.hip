@@ -430,9 +430,12 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_seed_kernel(const PfValuePa
 // Reverse sweep over the whole horizon (n = S-1 .. 0): grads with lambda(n+1) and the
 // shocks of step n, lambda <- T^T lambda + mu e_0 (mu = dt sum of beta of later dates),
 // + gbar_e at date steps. Pass 0: factors 0, 1; pass 1: factor 2, drift, curve.
-template <int kPass, bool kHostNormals>
+// kSrc: 0 regenerate the in-kernel normals, 1 host-supplied FP64 [path][step][3],
+// 2 the forward pass's stored FP32 normals [step][3][path].
+template <int kPass, int kSrc>
 __global__ __launch_bounds__(kBlock) void pf_adjoint_sweep_kernel(const PfGrid g, float c,
                                                                   const double* __restrict__ normals,
+                                                                  const float* __restrict__ zs,
                                                                   const float* __restrict__ gbar,
                                                                   const float* __restrict__ beta,
                                                                   double* __restrict__ partials)
@@ -462,12 +465,20 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_sweep_kernel(const PfGrid g
         mu += active ? dtf * beta[(int64_t)(e + 1) * P + i] : 0.f;
         for (int s = g.stride - 1; s >= 0; --s) {
             const int n = e * g.stride + s;
-            float z0, z1, z2;
-            if constexpr (kHostNormals) {
+            float z0 = 0.f, z1 = 0.f, z2 = 0.f;
+            if constexpr (kSrc == 1) {
                 const double* q = normals + ((int64_t)ii * g.nsteps + n) * 3;
                 z0 = (float)q[0];
                 z1 = (float)q[1];
                 z2 = (float)q[2];
+            } else if constexpr (kSrc == 2) {
+                const float* q = zs + (int64_t)n * 3 * P + ii;
+                if constexpr (kPass == 0) {
+                    z0 = q[0];
+                    z1 = q[P];
+                } else {
+                    z2 = q[2 * P];
+                }
             } else {
                 const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n);
                 z0 = z.z0;
@@ -568,22 +579,24 @@ hipError_t launch_pf_adjoint_seed_f32(const PfValueParams& v, const float* curve
 }
 
 hipError_t launch_pf_adjoint_sweep(const PfGrid& g, double c, int pass, const double* normals,
-                                   const float* gbar, const float* beta, double* partials,
-                                   hipStream_t st)
+                                   const float* zs, const float* gbar, const float* beta,
+                                   double* partials, hipStream_t st)
 {
     const dim3 grid((unsigned)((g.paths + kBlock - 1) / kBlock));
     const float cf = (float)c;
-    if (normals) {
-        if (pass == 0)
-            pf_adjoint_sweep_kernel<0, true><<<grid, kBlock, 0, st>>>(g, cf, normals, gbar, beta, partials);
-        else
-            pf_adjoint_sweep_kernel<1, true><<<grid, kBlock, 0, st>>>(g, cf, normals, gbar, beta, partials);
+#define HJM_SWEEP(P_, S_) \
+    pf_adjoint_sweep_kernel<P_, S_><<<grid, kBlock, 0, st>>>(g, cf, normals, zs, gbar, beta, partials)
+    const int src = normals ? 1 : (zs ? 2 : 0);
+    if (pass == 0) {
+        if (src == 1) HJM_SWEEP(0, 1);
+        else if (src == 2) HJM_SWEEP(0, 2);
+        else HJM_SWEEP(0, 0);
     } else {
-        if (pass == 0)
-            pf_adjoint_sweep_kernel<0, false><<<grid, kBlock, 0, st>>>(g, cf, normals, gbar, beta, partials);
-        else
-            pf_adjoint_sweep_kernel<1, false><<<grid, kBlock, 0, st>>>(g, cf, normals, gbar, beta, partials);
+        if (src == 1) HJM_SWEEP(1, 1);
+        else if (src == 2) HJM_SWEEP(1, 2);
+        else HJM_SWEEP(1, 0);
     }
+#undef HJM_SWEEP
     return hipGetLastError();
 }
 
