@@ -103,28 +103,11 @@ __global__ __launch_bounds__(kBlock) void ref_adjoint_sweep_kernel(const AdjGrid
                 mu[j] += sa;
             }
         }
-        for (int s = g.stride - 1; s >= 0; --s) {
-            const int n = b * g.stride + s;  // global step index of this Euler step
-            float z0 = 0.f, z1 = 0.f, z2 = 0.f;
-            if constexpr (kSrc == 1) {
-                const double* q = normals + ((int64_t)(active ? i : 0) * g.nsteps + n) * 3;
-                z0 = (float)q[0];
-                z1 = (float)q[1];
-                z2 = (float)q[2];
-            } else if constexpr (kSrc == 2) {
-                const float* q = zs + (int64_t)n * 3 * P + (active ? i : 0);
-                if constexpr (kPass == 0) {
-                    z0 = q[0];
-                    z1 = q[P];
-                } else {
-                    z2 = q[2 * P];
-                }
-            } else {
-                const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n);
-                z0 = z.z0;
-                z1 = z.z1;
-                z2 = z.z2;
-            }
+        // one reverse Euler step with the step's normals
+        auto step = [&](float z0, float z1, float z2) {
+            (void)z0;
+            (void)z1;
+            (void)z2;
             if constexpr (kPass == 0) {
                 const float s0 = g.sqrt_dt * z0, s1 = g.sqrt_dt * z1;
 #pragma unroll
@@ -161,6 +144,50 @@ __global__ __launch_bounds__(kBlock) void ref_adjoint_sweep_kernel(const AdjGrid
                     if (j0 - u >= 1) op_fma_inplace(lam[j0 - u], omc, t[u]);
             }
             op_fma_inplace(lam[0], omc, mu[0]);
+        };
+        if constexpr (kSrc == 2) {
+            // stored normals, read kPre steps ahead so the HBM latency hides behind the
+            // previous steps' arithmetic (the matrix-core forward runs 50 steps per date)
+            constexpr int kPre = 10;
+            float cur[kPre][2], nxt[kPre][2];
+            auto load = [&](int n_hi, float (&d)[kPre][2]) {
+#pragma unroll
+                for (int u = 0; u < kPre; ++u) {
+                    const float* q = zs + (int64_t)(n_hi - u) * 3 * P + (active ? i : 0);
+                    if constexpr (kPass == 0) {
+                        d[u][0] = q[0];
+                        d[u][1] = q[P];
+                    } else {
+                        d[u][0] = q[2 * P];
+                        d[u][1] = 0.f;
+                    }
+                }
+            };
+            load(b * g.stride + g.stride - 1, cur);
+            for (int s0 = g.stride - 1; s0 >= 0; s0 -= kPre) {
+                if (s0 - kPre >= 0) load(b * g.stride + s0 - kPre, nxt);
+#pragma unroll
+                for (int u = 0; u < kPre; ++u) {
+                    if constexpr (kPass == 0) step(cur[u][0], cur[u][1], 0.f);
+                    else step(0.f, 0.f, cur[u][0]);
+                }
+#pragma unroll
+                for (int u = 0; u < kPre; ++u) {
+                    cur[u][0] = nxt[u][0];
+                    cur[u][1] = nxt[u][1];
+                }
+            }
+        } else {
+            for (int s = g.stride - 1; s >= 0; --s) {
+                const int n = b * g.stride + s;  // global step index of this Euler step
+                if constexpr (kSrc == 1) {
+                    const double* q = normals + ((int64_t)(active ? i : 0) * g.nsteps + n) * 3;
+                    step((float)q[0], (float)q[1], (float)q[2]);
+                } else {
+                    const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n);
+                    step(z.z0, z.z1, z.z2);
+                }
+            }
         }
         // flush the block's FP32 partials into the wave's FP64 accumulators
 #pragma unroll
@@ -222,6 +249,7 @@ hipError_t launch_ref_adjoint_sweep(const RefGrid& rg, double c, double sqrt_dt,
     g.opc = (float)(1.0 + c);
     g.sqrt_dt = (float)sqrt_dt;
     const dim3 grid((unsigned)((rg.paths + kBlock - 1) / kBlock));
+    if (zs && rg.stride % 10 != 0) return hipErrorInvalidValue;  // stored-normal prefetch chunks
 #define HJM_SWEEP(P_, S_) \
     ref_adjoint_sweep_kernel<P_, S_><<<grid, kBlock, 0, st>>>(g, normals, zs, gx, gA, partials)
     const int src = normals ? 1 : (zs ? 2 : 0);

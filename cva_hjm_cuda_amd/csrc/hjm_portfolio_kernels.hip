@@ -463,28 +463,12 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_sweep_kernel(const PfGrid g
     float mu = 0.f;
     for (int e = E - 1; e >= 0; --e) {
         mu += active ? dtf * beta[(int64_t)(e + 1) * P + i] : 0.f;
-        for (int s = g.stride - 1; s >= 0; --s) {
-            const int n = e * g.stride + s;
-            float z0 = 0.f, z1 = 0.f, z2 = 0.f;
-            if constexpr (kSrc == 1) {
-                const double* q = normals + ((int64_t)ii * g.nsteps + n) * 3;
-                z0 = (float)q[0];
-                z1 = (float)q[1];
-                z2 = (float)q[2];
-            } else if constexpr (kSrc == 2) {
-                const float* q = zs + (int64_t)n * 3 * P + ii;
-                if constexpr (kPass == 0) {
-                    z0 = q[0];
-                    z1 = q[P];
-                } else {
-                    z2 = q[2 * P];
-                }
-            } else {
-                const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n);
-                z0 = z.z0;
-                z1 = z.z1;
-                z2 = z.z2;
-            }
+        // one reverse Euler step with the step's normals
+        auto step = [&](int n, float z0, float z1, float z2) {
+            (void)n;
+            (void)z0;
+            (void)z1;
+            (void)z2;
             if constexpr (kPass == 0) {
                 const float s0 = sqdt * z0, s1 = sqdt * z1;
 #pragma unroll
@@ -519,6 +503,51 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_sweep_kernel(const PfGrid g
                     if (j0 - u >= 1) op_fma_inplace(lam[j0 - u], omc, t[u]);
             }
             op_fma_inplace(lam[0], omc, mu);
+        };
+        if constexpr (kSrc == 2) {
+            // stored normals, read kPre steps ahead so the HBM latency hides behind the
+            // previous steps' arithmetic (the matrix-core forward runs 50 steps per date)
+            constexpr int kPre = 10;
+            float cur[kPre][2], nxt[kPre][2];
+            auto load = [&](int n_hi, float (&d)[kPre][2]) {
+#pragma unroll
+                for (int u = 0; u < kPre; ++u) {
+                    const float* q = zs + (int64_t)(n_hi - u) * 3 * P + ii;
+                    if constexpr (kPass == 0) {
+                        d[u][0] = q[0];
+                        d[u][1] = q[P];
+                    } else {
+                        d[u][0] = q[2 * P];
+                        d[u][1] = 0.f;
+                    }
+                }
+            };
+            load(e * g.stride + g.stride - 1, cur);
+            for (int s0 = g.stride - 1; s0 >= 0; s0 -= kPre) {
+                if (s0 - kPre >= 0) load(e * g.stride + s0 - kPre, nxt);
+#pragma unroll
+                for (int u = 0; u < kPre; ++u) {
+                    const int n = e * g.stride + s0 - u;
+                    if constexpr (kPass == 0) step(n, cur[u][0], cur[u][1], 0.f);
+                    else step(n, 0.f, 0.f, cur[u][0]);
+                }
+#pragma unroll
+                for (int u = 0; u < kPre; ++u) {
+                    cur[u][0] = nxt[u][0];
+                    cur[u][1] = nxt[u][1];
+                }
+            }
+        } else {
+            for (int s = g.stride - 1; s >= 0; --s) {
+                const int n = e * g.stride + s;
+                if constexpr (kSrc == 1) {
+                    const double* q = normals + ((int64_t)ii * g.nsteps + n) * 3;
+                    step(n, (float)q[0], (float)q[1], (float)q[2]);
+                } else {
+                    const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n);
+                    step(n, z.z0, z.z1, z.z2);
+                }
+            }
         }
         // date e: the valuation at e depends on f(n_e)
 #pragma unroll
@@ -584,6 +613,7 @@ hipError_t launch_pf_adjoint_sweep(const PfGrid& g, double c, int pass, const do
 {
     const dim3 grid((unsigned)((g.paths + kBlock - 1) / kBlock));
     const float cf = (float)c;
+    if (zs && g.stride % 10 != 0) return hipErrorInvalidValue;  // stored-normal prefetch chunks
 #define HJM_SWEEP(P_, S_) \
     pf_adjoint_sweep_kernel<P_, S_><<<grid, kBlock, 0, st>>>(g, cf, normals, zs, gbar, beta, partials)
     const int src = normals ? 1 : (zs ? 2 : 0);
