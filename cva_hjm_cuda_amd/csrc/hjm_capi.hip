@@ -15,6 +15,7 @@
 #include "hjm_kernels.h"
 #include "hjm_host.h"
 #include "hjm_mfma_plan.h"
+#include "hjm_select.h"
 
 using namespace hjm;
 using namespace hjm_host;
@@ -70,6 +71,7 @@ struct hjm_plan_s {
     double* d_partials = nullptr;
     double* d_sums = nullptr;
     double* d_pfe = nullptr;
+    char* d_selws = nullptr;  // bracket-selection workspace (PFE)
 };
 
 namespace {
@@ -88,6 +90,7 @@ void plan_release(hjm_plan p)
     dev_free(p->d_partials);
     dev_free(p->d_sums);
     dev_free(p->d_pfe);
+    dev_free(p->d_selws);
     for (auto& e : p->ev)
         if (e) (void)hipEventDestroy(e);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -164,6 +167,11 @@ int plan_init(hjm_plan p, const hjm_market* market, const hjm_swap* swap, const 
     HJM_TRY(dev_alloc(&p->d_partials, (size_t)p->nblocks * kStatSlots));
     HJM_TRY(dev_alloc(&p->d_sums, kStatSlots));
     HJM_TRY(dev_alloc(&p->d_pfe, kTenors));
+    if (p->do_pfe) {
+        const size_t wsb = bracket_workspace_bytes(
+            kTenors, bracket_plan(cfg->paths, p->pfe_rank, sizeof(double)), sizeof(double));
+        if (wsb) HJM_TRY(dev_alloc(&p->d_selws, wsb));
+    }
     if (p->use_mfma) {
         HJM_HIP(mfma_kernels_prepare(), HJM_ERR_DEVICE);
         std::vector<uint16_t> wf;
@@ -214,7 +222,8 @@ int plan_launch(hjm_plan p, bool keep_exposures, float* zs = nullptr)
     HJM_HIP(launch_column_sum(p->d_partials, p->nblocks, kStatSlots, p->d_sums, st), HJM_ERR_LAUNCH);
     HJM_HIP(hipEventRecord(p->ev[3], st), HJM_ERR_DEVICE);
     if (p->do_pfe)
-        HJM_HIP(launch_radix_select(p->d_ex, p->grid.ndates, p->paths, p->pfe_rank, p->d_pfe, st),
+        HJM_HIP(launch_radix_select(p->d_ex, p->grid.ndates, p->paths, p->pfe_rank, p->d_selws,
+                                    p->d_pfe, st),
                 HJM_ERR_LAUNCH);
     HJM_HIP(hipEventRecord(p->ev[4], st), HJM_ERR_DEVICE);
     return HJM_OK;
@@ -640,17 +649,21 @@ int hjm_select_kth(const double* values, int32_t rows, int64_t n, int64_t k, int
     DeviceGuard guard;
     HJM_TRY(guard.set(device));
     double *d = nullptr, *d_out = nullptr;
+    char* ws = nullptr;
+    const size_t wsb = bracket_workspace_bytes(rows, bracket_plan(n, k, sizeof(double)), sizeof(double));
     int rc = dev_alloc(&d, cells);
     if (rc == HJM_OK) rc = dev_alloc(&d_out, (size_t)rows);
+    if (rc == HJM_OK && wsb) rc = dev_alloc(&ws, wsb);
     if (rc == HJM_OK && hipMemcpy(d, values, cells * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
         rc = HJM_ERR_DEVICE;
-    if (rc == HJM_OK && launch_radix_select(d, rows, n, k, d_out, nullptr) != hipSuccess)
+    if (rc == HJM_OK && launch_radix_select(d, rows, n, k, ws, d_out, nullptr) != hipSuccess)
         rc = HJM_ERR_LAUNCH;
     if (rc == HJM_OK && hipDeviceSynchronize() != hipSuccess) rc = HJM_ERR_LAUNCH;
     if (rc == HJM_OK && hipMemcpy(out, d_out, rows * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
         rc = HJM_ERR_DEVICE;
     dev_free(d);
     dev_free(d_out);
+    dev_free(ws);
     return rc;
 }
 

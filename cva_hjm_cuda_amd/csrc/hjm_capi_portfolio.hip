@@ -9,6 +9,7 @@
 #include "hjm_host.h"
 #include "hjm_kernels.h"
 #include "hjm_mfma_plan.h"
+#include "hjm_select.h"
 
 using namespace hjm;
 using namespace hjm_host;
@@ -41,6 +42,7 @@ struct hjm_pplan_s {
     double* d_partials = nullptr;
     double* d_sums = nullptr;
     double* d_pfe = nullptr;
+    char* d_selws = nullptr;  // bracket-selection workspace (PFE)
     void* d_coef = nullptr;
     void* d_a = nullptr;
     int* d_kmax = nullptr;
@@ -64,6 +66,7 @@ void pplan_release(hjm_portfolio_plan p)
     dev_free(p->d_partials);
     dev_free(p->d_sums);
     dev_free(p->d_pfe);
+    dev_free(p->d_selws);
     dev_free(p->d_coef);
     dev_free(p->d_a);
     dev_free(p->d_kmax);
@@ -207,6 +210,11 @@ int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfol
     HJM_TRY(dev_alloc(&p->d_partials, (size_t)pf->nsets * nb * (nd + 2)));
     HJM_TRY(dev_alloc(&p->d_sums, (size_t)pf->nsets * (nd + 2)));
     HJM_TRY(dev_alloc(&p->d_pfe, (size_t)pf->nsets * nd));
+    if (p->do_pfe) {
+        const size_t wsb = bracket_workspace_bytes(pf->nsets * nd, bracket_plan(cfg->paths, p->pfe_rank, (int)rsz),
+                                                   (int)rsz);
+        if (wsb) HJM_TRY(dev_alloc(&p->d_selws, wsb));
+    }
     HJM_TRY(dev_alloc(&p->d_kmax, kmax.size()));
     HJM_TRY(dev_alloc(&p->d_w, p->w.size()));
     HJM_HIP(hipMemcpy(p->d_kmax, kmax.data(), kmax.size() * sizeof(int), hipMemcpyHostToDevice),
@@ -286,17 +294,35 @@ int pplan_enqueue(hjm_portfolio_plan p, bool events = true, float* zs = nullptr)
                                     (float*)p->d_ex, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
     HJM_HIP(mark(2), HJM_ERR_DEVICE);
+    // With a bracket-selection workspace the statistics pass also does the PFE's counting
+    // pass: the bracket sample runs before it and the selection's last launch after it.
+    void* selws = (p->do_pfe && p->d_selws &&
+                   pf_stats_stage(p->paths, p->pfe_rank, p->ndates, f64 ? 8 : 4) > 0)
+                      ? p->d_selws
+                      : nullptr;
+    if (selws) {
+        err = f64 ? launch_bracket_sample((const double*)p->d_ex, rows, p->paths, p->pfe_rank, selws, st)
+                  : launch_bracket_sample_f32((const float*)p->d_ex, rows, p->paths, p->pfe_rank, selws,
+                                              st);
+        HJM_HIP(err, HJM_ERR_LAUNCH);
+    }
     err = f64 ? launch_pf_stats_f64(p->vp, (const double*)p->d_ex, p->d_w, p->d_partials,
-                                    p->d_sums, st)
+                                    p->d_sums, selws, p->pfe_rank, st)
               : launch_pf_stats_f32(p->vp, (const float*)p->d_ex, p->d_w, p->d_partials,
-                                    p->d_sums, st);
+                                    p->d_sums, selws, p->pfe_rank, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
     HJM_HIP(mark(3), HJM_ERR_DEVICE);
     if (p->do_pfe) {
-        err = f64 ? launch_radix_select((const double*)p->d_ex, rows, p->paths, p->pfe_rank,
-                                        p->d_pfe, st)
-                  : launch_radix_select_f32((const float*)p->d_ex, rows, p->paths, p->pfe_rank,
-                                            p->d_pfe, st);
+        if (selws)
+            err = f64 ? launch_bracket_finish((const double*)p->d_ex, rows, p->paths, p->pfe_rank,
+                                              selws, p->d_pfe, st)
+                      : launch_bracket_finish_f32((const float*)p->d_ex, rows, p->paths, p->pfe_rank,
+                                                  selws, p->d_pfe, st);
+        else
+            err = f64 ? launch_radix_select((const double*)p->d_ex, rows, p->paths, p->pfe_rank,
+                                            p->d_selws, p->d_pfe, st)
+                      : launch_radix_select_f32((const float*)p->d_ex, rows, p->paths, p->pfe_rank,
+                                                p->d_selws, p->d_pfe, st);
         HJM_HIP(err, HJM_ERR_LAUNCH);
     }
     HJM_HIP(mark(4), HJM_ERR_DEVICE);

@@ -22,8 +22,9 @@ hipError_t launch_ref_exposure(const ExposureParams& e, const double* nx, const 
                                double* exposures, double* partials, hipStream_t st);
 hipError_t launch_column_sum(const double* partials, int64_t rows, int cols, double* out,
                              hipStream_t st);
-hipError_t launch_radix_select(const double* x, int rows, int64_t n, int64_t k, double* out,
-                               hipStream_t st);
+// k-th smallest of each row; ws: workspace of bracket_workspace_bytes (hjm_select.h) or null
+hipError_t launch_radix_select(const double* x, int rows, int64_t n, int64_t k, void* ws,
+                               double* out, hipStream_t st);
 hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nsteps, double* out,
                           hipStream_t st);
 
@@ -55,12 +56,27 @@ hipError_t launch_pf_value_f64(const PfValueParams& v, const double* curves, con
 hipError_t launch_pf_value_f32(const PfValueParams& v, const float* curves, const double* logb,
                                const float* coef, const float* a, const int* kmax, float* ex,
                                hipStream_t st);
+// selws (or null): bracket-selection workspace whose counting this pass does, for rank
+// pfe_rank of each (date, set) row, after launch_bracket_sample*; needs
+// pf_stats_stage(...) > 0 (LDS staging slice per wave, in entries)
+int pf_stats_stage(int64_t paths, int64_t pfe_rank, int ndates, int elem_bytes);
 hipError_t launch_pf_stats_f64(const PfValueParams& v, const double* ex, const double* w,
-                               double* partials, double* sums, hipStream_t st);
+                               double* partials, double* sums, void* selws, int64_t pfe_rank,
+                               hipStream_t st);
 hipError_t launch_pf_stats_f32(const PfValueParams& v, const float* ex, const double* w,
-                               double* partials, double* sums, hipStream_t st);
-hipError_t launch_radix_select_f32(const float* x, int rows, int64_t n, int64_t k, double* out,
-                                   hipStream_t st);
+                               double* partials, double* sums, void* selws, int64_t pfe_rank,
+                               hipStream_t st);
+hipError_t launch_radix_select_f32(const float* x, int rows, int64_t n, int64_t k, void* ws,
+                                   double* out, hipStream_t st);
+// the sampled-bracket selection split around a statistics pass that does its counting
+hipError_t launch_bracket_sample(const double* x, int rows, int64_t n, int64_t k, void* ws,
+                                 hipStream_t st);
+hipError_t launch_bracket_sample_f32(const float* x, int rows, int64_t n, int64_t k, void* ws,
+                                     hipStream_t st);
+hipError_t launch_bracket_finish(const double* x, int rows, int64_t n, int64_t k, void* ws,
+                                 double* out, hipStream_t st);
+hipError_t launch_bracket_finish_f32(const float* x, int rows, int64_t n, int64_t k, void* ws,
+                                     double* out, hipStream_t st);
 
 hipError_t launch_pf_adjoint_seed_f64(const PfValueParams& v, const double* curves,
                                       const double* logb, const double* coef, const double* a,

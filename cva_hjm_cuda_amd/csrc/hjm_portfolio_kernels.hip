@@ -175,37 +175,100 @@ __global__ __launch_bounds__(kBlock) void pf_value_kernel(const PfValueParams v,
 // Per set g (blockIdx.y) and kStatPaths paths (blockIdx.x; each thread takes kStatPerThread
 // of them, 256 apart, so one wave reduction serves 8 exposures per lane): EE numerators at
 // every date and the per-path CVA sample's sum and square sum; fixed-order reductions.
+// kCount: the same pass does the counting of the PFE's sampled-bracket selection for the
+// rows (date, set) it reads (hjm_select.h): counts below the bracket / zeros, and the
+// entries inside it. The counts are wave ballots. Each wave stages its inside entries in
+// its own slice of dynamic LDS (stage entries, in date order, at a wave-uniform offset);
+// after the last date one atomic per row claims the workgroup's candidate slots and the
+// waves copy their slices out. A slice that fills up flags the rows it could not hold
+// (kSelOverflow in the row's claim), whose selection then runs over the whole row.
 constexpr int kStatPerThread = 8;
 constexpr int kStatPaths = kBlock * kStatPerThread;
+constexpr uint32_t kSelOverflow = 0x80000000u;
 
-template <typename Real>
+template <typename Real, bool kCount>
 __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
                                                           const Real* __restrict__ exposures,
                                                           const double* __restrict__ w,
-                                                          double* __restrict__ partials)
+                                                          double* __restrict__ partials,
+                                                          SelRowState* __restrict__ sel,
+                                                          Real* __restrict__ cand, int64_t cap,
+                                                          int stage)
 {
-    __shared__ double red[kBlock / kWave][kMaxDates + 2];
+    typedef typename SelectBits<Real>::U U;
+    constexpr int kWaves = kBlock / kWave;
+    constexpr int kD = kCount ? kMaxDates : 1;
+    __shared__ double red[kWaves][kMaxDates + 2];
+    __shared__ uint32_t cnt[2][kWaves][kD];  // below-or-zero and inside counts per wave, date
+    __shared__ uint32_t beg[kWaves][kD];     // first staged entry of (wave, date)
+    __shared__ uint32_t base[kD];            // claimed candidate slot of the workgroup's rows
+    extern __shared__ __align__(16) unsigned char stage_raw[];
     const int gset = blockIdx.y;
     const int64_t P = v.paths;
     const int64_t i0 = (int64_t)blockIdx.x * kStatPaths + threadIdx.x;
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const uint64_t below_me = lane ? (~0ull >> (64 - lane)) : 0ull;
+    U* slice = reinterpret_cast<U*>(stage_raw) + (size_t)wave * stage;
+    uint32_t used = 0;  // wave-uniform
     const int nd = v.ndates;
     double cva[kStatPerThread];
 #pragma unroll
     for (int k = 0; k < kStatPerThread; ++k) cva[k] = 0.0;
     for (int e = 0; e < nd; ++e) {
-        const Real* row = exposures + ((int64_t)e * v.nsets + gset) * P;
+        const int64_t rid = (int64_t)e * v.nsets + gset;
+        const Real* row = exposures + rid * P;
         const double we = w[(int64_t)gset * nd + e];
         double s = 0.0;
+        Real xs[kStatPerThread];
 #pragma unroll
         for (int k = 0; k < kStatPerThread; ++k) {
             const int64_t i = i0 + (int64_t)k * kBlock;
-            const double x = i < P ? (double)row[i] : 0.0;
+            xs[k] = i < P ? row[i] : (Real)0;
+            const double x = (double)xs[k];
             s += x;
             cva[k] += we * x;
         }
         s = wave_sum(s);
         if (lane == 0) red[wave][e] = s;
+        if constexpr (kCount) {
+            const U lo = (U)sel[rid].lo, hi = (U)sel[rid].hi;
+            const U lo1 = lo ? lo : (U)1;
+            const bool any_in = hi >= lo1;
+            const U span = any_in ? (U)(hi - lo1) : (U)0;
+            const bool collect = any_in && lo != hi;
+            uint32_t nlo = 0, nin = 0;
+            uint64_t masks[kStatPerThread];
+#pragma unroll
+            for (int k = 0; k < kStatPerThread; ++k) {
+                const bool valid = i0 + (int64_t)k * kBlock < P;
+                const U b = __builtin_bit_cast(U, xs[k]);
+                nlo += (uint32_t)__popcll(__ballot(valid && b < lo1));
+                masks[k] = __ballot(valid && any_in && (U)(b - lo1) <= span);
+                nin += (uint32_t)__popcll(masks[k]);
+            }
+            uint32_t staged = 0;
+            if (collect && nin) {
+                if (used + nin <= (uint32_t)stage) {
+                    uint32_t off = used;
+#pragma unroll
+                    for (int k = 0; k < kStatPerThread; ++k) {
+                        const uint64_t mk = masks[k];
+                        if ((mk >> lane) & 1ull)
+                            slice[off + (uint32_t)__popcll(mk & below_me)] = __builtin_bit_cast(U, xs[k]);
+                        off += (uint32_t)__popcll(mk);
+                    }
+                    staged = nin;
+                } else if (lane == 0) {
+                    atomicOr(&sel[rid].claim, kSelOverflow);
+                }
+            }
+            if (lane == 0) {
+                cnt[0][wave][e] = nlo;
+                cnt[1][wave][e] = nin;
+                beg[wave][e] = staged ? used : ~0u;
+            }
+            used += staged;
+        }
     }
     double c1 = 0.0, c2 = 0.0;
 #pragma unroll
@@ -224,8 +287,34 @@ __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
     for (int c = threadIdx.x; c < ncol; c += kBlock) {
         double t = 0.0;
 #pragma unroll
-        for (int ww = 0; ww < kBlock / kWave; ++ww) t += red[ww][c];
+        for (int ww = 0; ww < kWaves; ++ww) t += red[ww][c];
         partials[((int64_t)gset * gridDim.x + blockIdx.x) * ncol + c] = t;
+    }
+    if constexpr (kCount) {
+        for (int e = threadIdx.x; e < nd; e += kBlock) {
+            const int64_t rid = (int64_t)e * v.nsets + gset;
+            uint32_t nlo = 0, nin = 0, nst = 0;
+#pragma unroll
+            for (int ww = 0; ww < kWaves; ++ww) {
+                nlo += cnt[0][ww][e];
+                nin += cnt[1][ww][e];
+                nst += beg[ww][e] != ~0u ? cnt[1][ww][e] : 0u;
+            }
+            if (nlo) atomicAdd(sel[rid].lo ? &sel[rid].below : &sel[rid].zin, nlo);
+            if (nin) atomicAdd(&sel[rid].nin, nin);
+            base[e] = nst ? atomicAdd(&sel[rid].claim, nst) : 0u;
+        }
+        __syncthreads();
+        for (int e = 0; e < nd; ++e) {
+            const uint32_t b0 = beg[wave][e];
+            if (b0 == ~0u) continue;
+            uint32_t off = base[e];
+            for (int ww = 0; ww < wave; ++ww) off += beg[ww][e] != ~0u ? cnt[1][ww][e] : 0u;
+            const uint32_t n = cnt[1][wave][e];
+            Real* out = cand + ((int64_t)e * v.nsets + gset) * cap;
+            for (uint32_t i = lane; i < n; i += kWave)
+                if ((int64_t)off + i < cap) out[off + i] = __builtin_bit_cast(Real, slice[b0 + i]);
+        }
     }
 }
 
@@ -300,12 +389,39 @@ hipError_t launch_pf_value_f32(const PfValueParams& v, const float* curves, cons
     return pf_value_launch<float>(v, curves, logb, coef, a, kmax, ex, st);
 }
 
+int pf_stats_stage(int64_t paths, int64_t pfe_rank, int ndates, int elem_bytes)
+{
+    // per-wave staging slice: twice the expected inside count of a wave's 512 paths over
+    // all dates, within 32 KB of LDS per workgroup; 0 = too wide to fuse
+    const BracketPlan bp = bracket_plan(paths, pfe_rank, elem_bytes);
+    if (!bp.use) return 0;
+    const double frac = (double)(bp.khi - bp.klo + 1) / bp.m;
+    const double want = 2.0 * frac * kWave * kStatPerThread * ndates + 256.0;
+    const int limit = 32 * 1024 / ((kBlock / kWave) * elem_bytes);
+    return want > limit ? 0 : ((int)want + 63) / 64 * 64;
+}
+
 template <typename Real>
 static hipError_t pf_stats_launch(const PfValueParams& v, const Real* ex, const double* w,
-                                  double* partials, double* sums, hipStream_t st)
+                                  double* partials, double* sums, void* selws, int64_t pfe_rank,
+                                  hipStream_t st)
 {
     const unsigned nb = (unsigned)((v.paths + kStatPaths - 1) / kStatPaths);
-    pf_stats_kernel<Real><<<dim3(nb, (unsigned)v.nsets), kBlock, 0, st>>>(v, ex, w, partials);
+    const dim3 grid(nb, (unsigned)v.nsets);
+    if (selws) {
+        const BracketPlan bp = bracket_plan(v.paths, pfe_rank, (int)sizeof(Real));
+        const int stage = pf_stats_stage(v.paths, pfe_rank, v.ndates, (int)sizeof(Real));
+        if (!bp.use || !stage) return hipErrorInvalidValue;
+        SelRowState* state;
+        Real* cand;
+        bracket_ws_split<Real>(selws, v.ndates * v.nsets, &state, &cand);
+        const size_t lds = (size_t)(kBlock / kWave) * stage * sizeof(Real);
+        pf_stats_kernel<Real, true><<<grid, kBlock, lds, st>>>(v, ex, w, partials, state, cand, bp.cap,
+                                                              stage);
+    } else {
+        pf_stats_kernel<Real, false><<<grid, kBlock, 0, st>>>(v, ex, w, partials, nullptr, nullptr, 0,
+                                                             0);
+    }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
     grouped_column_sum_kernel<<<dim3((unsigned)(v.ndates + 2), (unsigned)v.nsets), kBlock, 0, st>>>(
@@ -314,23 +430,17 @@ static hipError_t pf_stats_launch(const PfValueParams& v, const Real* ex, const 
 }
 
 hipError_t launch_pf_stats_f64(const PfValueParams& v, const double* ex, const double* w,
-                               double* partials, double* sums, hipStream_t st)
+                               double* partials, double* sums, void* selws, int64_t pfe_rank,
+                               hipStream_t st)
 {
-    return pf_stats_launch<double>(v, ex, w, partials, sums, st);
+    return pf_stats_launch<double>(v, ex, w, partials, sums, selws, pfe_rank, st);
 }
 
 hipError_t launch_pf_stats_f32(const PfValueParams& v, const float* ex, const double* w,
-                               double* partials, double* sums, hipStream_t st)
+                               double* partials, double* sums, void* selws, int64_t pfe_rank,
+                               hipStream_t st)
 {
-    return pf_stats_launch<float>(v, ex, w, partials, sums, st);
-}
-
-hipError_t launch_radix_select_f32(const float* x, int rows, int64_t n, int64_t k, double* out,
-                                   hipStream_t st)
-{
-    if (rows <= 0) return hipSuccess;
-    select_kth_kernel<float><<<rows, kSelBlock, 0, st>>>(x, n, k, out);
-    return hipGetLastError();
+    return pf_stats_launch<float>(v, ex, w, partials, sums, selws, pfe_rank, st);
 }
 
 }  // namespace hjm

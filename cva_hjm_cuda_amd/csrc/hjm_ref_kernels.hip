@@ -281,14 +281,6 @@ hipError_t launch_column_sum(const double* partials, int64_t rows, int cols, dou
     return hipGetLastError();
 }
 
-hipError_t launch_radix_select(const double* x, int rows, int64_t n, int64_t k, double* out,
-                               hipStream_t st)
-{
-    if (rows <= 0) return hipSuccess;
-    select_kth_kernel<double><<<rows, kSelBlock, 0, st>>>(x, n, k, out);
-    return hipGetLastError();
-}
-
 hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nsteps, double* out,
                           hipStream_t st)
 {

@@ -66,6 +66,28 @@ def test_portfolio_fp64_matches_oracle(small, oracle):
     assert np.all(r.exposures[:, :, 0] == r.exposures[0:1, :, 0])
 
 
+def test_portfolio_fp32_pfe_exact_on_long_rows():
+    """PFE over 131072 FP32 / 65536 FP64 paths (the sampled-bracket selection, counted in
+    the statistics pass): the exact order statistic of the plan's own exposures, for every
+    set and date."""
+    _flush("[gpu] portfolio FP32 PFE, 131072 paths x 30y, 16 swaps, 4 sets")
+    m = hjm.benchmark_market()
+    pf = hjm.synthetic_portfolio(16, 4, seed=3)
+    P = 131072
+    r = hjm.simulate_portfolio(m, pf, P, horizon=30.0, precision=hjm.FP32_KAHAN, seed=11,
+                               keep_paths=True)
+    k = math.ceil(0.975 * P)
+    want = np.sort(r.exposures.astype(np.float32), axis=0)[k - 1].astype(np.float64)
+    assert np.array_equal(r.pfe, want)
+    assert np.all(r.pfe[:, 1] > 0)
+    # FP64 rows (sample of 8192 paths) and another quantile
+    P = 65536
+    r = hjm.simulate_portfolio(m, pf, P, horizon=30.0, precision=hjm.FP64, seed=11,
+                               keep_paths=True, pfe_quantile=0.9)
+    k = math.ceil(0.9 * P)
+    assert np.array_equal(r.pfe, np.sort(r.exposures, axis=0)[k - 1])
+
+
 def test_portfolio_in_kernel_rng_equals_host_normals(small):
     m, pf, z, _ = small
     a = hjm.simulate_portfolio(m, pf, 128, horizon=30.0, precision=hjm.FP64, normals=z)

@@ -228,12 +228,22 @@ def test_select_kth_exact_on_hard_rows():
             np.where(rng.random(n) < 0.9, 0.0, rng.random(n)),
             np.full(n, 1234.5),                         # one value, > 4096 copies per bin
             np.round(rng.random(n) * 50.0) / 8.0,       # heavy ties
-            1.0 + rng.random(n) * 1e-12]                # one binade, narrow mantissas
+            1.0 + rng.random(n) * 1e-12,                # one binade, narrow mantissas
+            np.sort(rng.random(n)),                     # leading sample unrepresentative
+            np.sort(rng.random(n))[::-1].copy(),
+            # representative leading sample, then a bulk concentrated at the quantile
+            # (more candidates than the bracket's buffer holds)
+            np.concatenate([rng.random(8192), 0.975 + 1e-9 * rng.random(n - 8192)])]
     X = np.stack(rows)
     for k in (1, 2, 4097, math.ceil(0.975 * n), n - 1, n):
         got = hjm.select_kth(X, k)
         want = np.partition(X, k - 1, axis=1)[:, k - 1]
         assert np.array_equal(got, want), k
+    # even row length (aligned vector loads in the counting pass) and random ranks
+    Y = rng.lognormal(0.0, 1.0, (5, 1 << 17)) * (rng.random((5, 1 << 17)) < 0.6)
+    for k in rng.integers(1, 1 << 17, 6):
+        got = hjm.select_kth(Y, int(k))
+        assert np.array_equal(got, np.partition(Y, k - 1, axis=1)[:, k - 1]), k
     with pytest.raises(hjm.HJMError):
         hjm.select_kth(-X[:1, :10], 3)
     with pytest.raises(hjm.HJMError):
