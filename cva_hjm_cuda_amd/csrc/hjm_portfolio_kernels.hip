@@ -202,6 +202,8 @@ __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
     __shared__ uint32_t cnt[2][kWaves][kD];  // below-or-zero and inside counts per wave, date
     __shared__ uint32_t beg[kWaves][kD];     // first staged entry of (wave, date)
     __shared__ uint32_t base[kD];            // claimed candidate slot of the workgroup's rows
+    __shared__ U s_lo[kD], s_hi[kD];         // the rows' brackets
+    __shared__ double s_w[kMaxDates];        // the set's CVA weights
     extern __shared__ __align__(16) unsigned char stage_raw[];
     const int gset = blockIdx.y;
     const int64_t P = v.paths;
@@ -214,16 +216,44 @@ __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
     double cva[kStatPerThread];
 #pragma unroll
     for (int k = 0; k < kStatPerThread; ++k) cva[k] = 0.0;
+    // weights and brackets staged in LDS: a per-date global load in the loop would make the
+    // wait for it also wait for the next date's exposures already in flight
+    for (int e = threadIdx.x; e < nd; e += kBlock) {
+        s_w[e] = w[(int64_t)gset * nd + e];
+        if constexpr (kCount) {
+            s_lo[e] = (U)sel[(int64_t)e * v.nsets + gset].lo;
+            s_hi[e] = (U)sel[(int64_t)e * v.nsets + gset].hi;
+        }
+    }
+    __syncthreads();
+    // the next date's exposures are loaded while this date's are reduced and counted
+    const bool full = (int64_t)(blockIdx.x + 1) * kStatPaths <= P;  // no path out of range
+    Real xn[kStatPerThread];
+    auto load = [&](int e, Real* dst) {
+        const Real* row = exposures + ((int64_t)e * v.nsets + gset) * P;
+        if (full) {
+#pragma unroll
+            for (int k = 0; k < kStatPerThread; ++k)
+                dst[k] = __builtin_nontemporal_load(row + i0 + (int64_t)k * kBlock);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kStatPerThread; ++k) {
+                const int64_t i = i0 + (int64_t)k * kBlock;
+                dst[k] = i < P ? __builtin_nontemporal_load(row + i) : (Real)0;
+            }
+        }
+    };
+    load(0, xn);
     for (int e = 0; e < nd; ++e) {
         const int64_t rid = (int64_t)e * v.nsets + gset;
-        const Real* row = exposures + rid * P;
-        const double we = w[(int64_t)gset * nd + e];
+        const double we = s_w[e];
         double s = 0.0;
         Real xs[kStatPerThread];
 #pragma unroll
+        for (int k = 0; k < kStatPerThread; ++k) xs[k] = xn[k];
+        if (e + 1 < nd) load(e + 1, xn);
+#pragma unroll
         for (int k = 0; k < kStatPerThread; ++k) {
-            const int64_t i = i0 + (int64_t)k * kBlock;
-            xs[k] = i < P ? row[i] : (Real)0;
             const double x = (double)xs[k];
             s += x;
             cva[k] += we * x;
@@ -231,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
         s = wave_sum(s);
         if (lane == 0) red[wave][e] = s;
         if constexpr (kCount) {
-            const U lo = (U)sel[rid].lo, hi = (U)sel[rid].hi;
+            const U lo = s_lo[e], hi = s_hi[e];
             const U lo1 = lo ? lo : (U)1;
             const bool any_in = hi >= lo1;
             const U span = any_in ? (U)(hi - lo1) : (U)0;
@@ -300,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
                 nin += cnt[1][ww][e];
                 nst += beg[ww][e] != ~0u ? cnt[1][ww][e] : 0u;
             }
-            if (nlo) atomicAdd(sel[rid].lo ? &sel[rid].below : &sel[rid].zin, nlo);
+            if (nlo) atomicAdd(s_lo[e] ? &sel[rid].below : &sel[rid].zin, nlo);
             if (nin) atomicAdd(&sel[rid].nin, nin);
             base[e] = nst ? atomicAdd(&sel[rid].claim, nst) : 0u;
         }

@@ -67,13 +67,13 @@ def test_portfolio_fp64_matches_oracle(small, oracle):
 
 
 def test_portfolio_fp32_pfe_exact_on_long_rows():
-    """PFE over 131072 FP32 / 65536 FP64 paths (the sampled-bracket selection, counted in
+    """PFE over 132072 FP32 / 65536 FP64 paths (the sampled-bracket selection, counted in
     the statistics pass): the exact order statistic of the plan's own exposures, for every
     set and date."""
-    _flush("[gpu] portfolio FP32 PFE, 131072 paths x 30y, 16 swaps, 4 sets")
+    _flush("[gpu] portfolio FP32 PFE, 132072 paths x 30y, 16 swaps, 4 sets")
     m = hjm.benchmark_market()
     pf = hjm.synthetic_portfolio(16, 4, seed=3)
-    P = 131072
+    P = 132072  # the last statistics block is partial
     r = hjm.simulate_portfolio(m, pf, P, horizon=30.0, precision=hjm.FP32_KAHAN, seed=11,
                                keep_paths=True)
     k = math.ceil(0.975 * P)
