@@ -69,9 +69,10 @@ struct hjm_plan_s {
     double* d_ny = nullptr;
     double* d_ex = nullptr;
     double* d_partials = nullptr;
-    double* d_sums = nullptr;
+    double* d_sums = nullptr;  // kStatSlots sums, then the kTenors PFE values (d_pfe)
     double* d_pfe = nullptr;
     char* d_selws = nullptr;  // bracket-selection workspace (PFE)
+    double* h_res = nullptr;  // pinned host copy of d_sums (one asynchronous copy per run)
 };
 
 namespace {
@@ -89,8 +90,8 @@ void plan_release(hjm_plan p)
     dev_free(p->d_ex);
     dev_free(p->d_partials);
     dev_free(p->d_sums);
-    dev_free(p->d_pfe);
     dev_free(p->d_selws);
+    if (p->h_res) (void)hipHostFree(p->h_res);
     for (auto& e : p->ev)
         if (e) (void)hipEventDestroy(e);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -165,8 +166,11 @@ int plan_init(hjm_plan p, const hjm_market* market, const hjm_swap* swap, const 
     HJM_TRY(dev_alloc(&p->d_ny, slots));
     HJM_TRY(dev_alloc(&p->d_ex, slots));
     HJM_TRY(dev_alloc(&p->d_partials, (size_t)p->nblocks * kStatSlots));
-    HJM_TRY(dev_alloc(&p->d_sums, kStatSlots));
-    HJM_TRY(dev_alloc(&p->d_pfe, kTenors));
+    HJM_TRY(dev_alloc(&p->d_sums, kStatSlots + kTenors));
+    p->d_pfe = p->d_sums + kStatSlots;
+    HJM_HIP(hipHostMalloc((void**)&p->h_res, sizeof(double) * (kStatSlots + kTenors),
+                          hipHostMallocDefault),
+            HJM_ERR_ALLOC);
     if (p->do_pfe) {
         const size_t wsb = bracket_workspace_bytes(
             kTenors, bracket_plan(cfg->paths, p->pfe_rank, sizeof(double)), sizeof(double));
@@ -226,6 +230,9 @@ int plan_launch(hjm_plan p, bool keep_exposures, float* zs = nullptr)
                                     p->d_pfe, st),
                 HJM_ERR_LAUNCH);
     HJM_HIP(hipEventRecord(p->ev[4], st), HJM_ERR_DEVICE);
+    HJM_HIP(hipMemcpyAsync(p->h_res, p->d_sums, sizeof(double) * (kStatSlots + kTenors),
+                           hipMemcpyDeviceToHost, st),
+            HJM_ERR_DEVICE);
     return HJM_OK;
 }
 
@@ -252,16 +259,12 @@ void finish_stats(const double* sums, int64_t P, const double* w, double* ee, do
 int plan_collect(hjm_plan p, hjm_results* r)
 {
     HJM_HIP(hipStreamSynchronize(p->stream), HJM_ERR_LAUNCH);
-    double sums[kStatSlots];
-    HJM_HIP(hipMemcpy(sums, p->d_sums, sizeof(sums), hipMemcpyDeviceToHost), HJM_ERR_DEVICE);
-    if (r->sums) memcpy(r->sums, sums, sizeof(sums));
+    const double* sums = p->h_res;
+    if (r->sums) memcpy(r->sums, sums, sizeof(double) * kStatSlots);
     finish_stats(sums, p->paths, p->ep.w, r->expected_exposure, r->cva, r->cva_stderr);
     if (r->pfe) {
         for (int t = 0; t < kTenors; ++t) r->pfe[t] = 0.0;
-        if (p->do_pfe)
-            HJM_HIP(hipMemcpy(r->pfe, p->d_pfe, sizeof(double) * p->grid.ndates,
-                              hipMemcpyDeviceToHost),
-                    HJM_ERR_DEVICE);
+        if (p->do_pfe) memcpy(r->pfe, p->h_res + kStatSlots, sizeof(double) * p->grid.ndates);
     }
     const int64_t P = p->paths;
     auto transpose_back = [&](const double* dsrc, double* dst) -> int {
