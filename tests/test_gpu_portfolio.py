@@ -80,12 +80,23 @@ def test_portfolio_fp32_pfe_exact_on_long_rows():
     want = np.sort(r.exposures.astype(np.float32), axis=0)[k - 1].astype(np.float64)
     assert np.array_equal(r.pfe, want)
     assert np.all(r.pfe[:, 1] > 0)
-    # FP64 rows (sample of 8192 paths) and another quantile
+    # the median: a bracket too wide for the statistics pass's LDS staging, so the
+    # selection's own counting pass runs
+    r = hjm.simulate_portfolio(m, pf, P, horizon=30.0, precision=hjm.FP32_KAHAN, seed=11,
+                               keep_paths=True, pfe_quantile=0.5)
+    k = math.ceil(0.5 * P)
+    want = np.sort(r.exposures.astype(np.float32), axis=0)[k - 1].astype(np.float64)
+    assert np.array_equal(r.pfe, want)
+    # FP64 rows (sample of 8192 paths): 10y horizon (21 dates, counted in the statistics
+    # pass) and 30y at the 90% quantile (the selection's own counting pass)
     P = 65536
-    r = hjm.simulate_portfolio(m, pf, P, horizon=30.0, precision=hjm.FP64, seed=11,
-                               keep_paths=True, pfe_quantile=0.9)
-    k = math.ceil(0.9 * P)
-    assert np.array_equal(r.pfe, np.sort(r.exposures, axis=0)[k - 1])
+    pf10 = Portfolio(np.minimum(pf.periods, 20), pf.sign_notional, pf.fixed_rate, pf.set_of,
+                     pf.hazard)
+    for book, horizon, q in ((pf10, 10.0, 0.975), (pf, 30.0, 0.9)):
+        r = hjm.simulate_portfolio(m, book, P, horizon=horizon, precision=hjm.FP64, seed=11,
+                                   keep_paths=True, pfe_quantile=q)
+        k = math.ceil(q * P)
+        assert np.array_equal(r.pfe, np.sort(r.exposures, axis=0)[k - 1]), (horizon, q)
 
 
 def test_portfolio_in_kernel_rng_equals_host_normals(small):
@@ -113,11 +124,11 @@ def test_portfolio_matrix_kernel_matches_oracle(small):
 
 
 def test_graph_replays_and_tuned_grids_are_bit_identical():
-    _flush("[gpu] hipGraph replays vs eager, tuned grids")
+    _flush("[gpu] hipGraph replays vs eager, tuned grids (70000 paths: bracket PFE in the graph)")
     m = hjm.benchmark_market()
     pf = hjm.synthetic_portfolio(96, 12, seed=3)
-    eager = hjm.simulate_portfolio(m, pf, 20000, horizon=30.0, seed=9)
-    plan = hjm.PortfolioPlan(m, pf, 20000, horizon=30.0, seed=9,
+    eager = hjm.simulate_portfolio(m, pf, 70000, horizon=30.0, seed=9)
+    plan = hjm.PortfolioPlan(m, pf, 70000, horizon=30.0, seed=9,
                              tuning=hjm.Tuning(256, 0, use_graph=True))
     for rep in range(4):
         r = plan.execute()
@@ -127,14 +138,14 @@ def test_graph_replays_and_tuned_grids_are_bit_identical():
         hjm.simulate_reference(m, hjm.InterestRateSwap(1e6, 0.02, 5.0), 512, seed=rep)
     plan.close()
     # persistent grids of the VALU path kernel
-    eager_v = hjm.simulate_portfolio(m, pf, 20000, horizon=30.0, seed=9,
+    eager_v = hjm.simulate_portfolio(m, pf, 70000, horizon=30.0, seed=9,
                                      path_kernel=hjm.KERNEL_VECTOR)
     for t in (hjm.Tuning(64, 1), hjm.Tuning(128, 4), hjm.Tuning(256, 2, use_graph=True)):
-        r = hjm.simulate_portfolio(m, pf, 20000, horizon=30.0, seed=9, tuning=t,
+        r = hjm.simulate_portfolio(m, pf, 70000, horizon=30.0, seed=9, tuning=t,
                                    path_kernel=hjm.KERNEL_VECTOR)
         assert np.array_equal(r.expected_exposure, eager_v.expected_exposure), t
     assert np.max(np.abs(eager_v.cva / eager.cva - 1)) < 1e-4
-    tune, ms = hjm.autotune_portfolio(m, pf, 20000, horizon=30.0, seed=9)
+    tune, ms = hjm.autotune_portfolio(m, pf, 70000, horizon=30.0, seed=9)
     assert tune.block_size in (64, 128, 256) and ms > 0
 
 
