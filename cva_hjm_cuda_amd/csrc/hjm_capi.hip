@@ -165,6 +165,8 @@ int plan_init(hjm_plan p, const hjm_market* market, const hjm_swap* swap, const 
     HJM_TRY(dev_alloc(&p->d_nx, slots));
     HJM_TRY(dev_alloc(&p->d_ny, slots));
     HJM_TRY(dev_alloc(&p->d_ex, slots));
+    // exposure rows past the last date are never written by a run
+    HJM_HIP(hipMemset(p->d_ex, 0, slots * sizeof(double)), HJM_ERR_DEVICE);
     HJM_TRY(dev_alloc(&p->d_partials, (size_t)p->nblocks * kStatSlots));
     HJM_TRY(dev_alloc(&p->d_sums, kStatSlots + kTenors));
     p->d_pfe = p->d_sums + kStatSlots;

@@ -79,11 +79,33 @@ struct PfValueParams {
 };
 
 // ---- wave / block reductions (fixed order, deterministic) ------------------------
+// v + (v moved by the DPP control kCtrl), in the rows of kRowMask (other lanes add 0)
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ double dpp_add(double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, kCtrl, kRowMask, 0xf, false);
+    const uint32_t hi =
+        (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), kCtrl, kRowMask, 0xf, false);
+    return v + __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Sum over the 64 lanes, returned in every lane: quad butterflies, row shifts by 4 and 8
+// (lane 15 of each row of 16 then holds the row's sum), row broadcasts 15 and 31 (lane 63
+// holds the wave's sum), read back from lane 63. DPP moves between VALU operations: no
+// LDS traffic, unlike ds_bpermute shuffles.
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    return v;
+    v = dpp_add<0xb1, 0xf>(v);   // quad_perm [1, 0, 3, 2]
+    v = dpp_add<0x4e, 0xf>(v);   // quad_perm [2, 3, 0, 1]
+    v = dpp_add<0x114, 0xf>(v);  // row_shr:4
+    v = dpp_add<0x118, 0xf>(v);  // row_shr:8
+    v = dpp_add<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_add<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, kWave - 1);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), kWave - 1);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
 }  // namespace hjm

@@ -175,10 +175,12 @@ __global__ __launch_bounds__(kBlock) void ref_exposure_kernel(const ExposurePara
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
     const int64_t P = e.paths;
 
+    // dates t >= ndates hold no cash flow: E_t = 0 (their exposure rows are zeroed once
+    // when the plan is created)
     double V = 0.0, cva = 0.0;
-    for (int t = kTenors - 1; t >= 0; --t) {
+    for (int t = e.ndates - 1; t >= 0; --t) {
         double Et = 0.0;
-        if (active && t < e.ndates) {
+        if (active) {
             const int slot = t + 1;
             const double x = num_x[(int64_t)slot * P + i];
             const double y = num_y[(int64_t)slot * P + i];
@@ -192,6 +194,8 @@ __global__ __launch_bounds__(kBlock) void ref_exposure_kernel(const ExposurePara
         const double ws = wave_sum(Et);
         if (lane == 0) red[wave][t] = ws;
     }
+    if (lane == 0)
+        for (int t = e.ndates; t < kTenors; ++t) red[wave][t] = 0.0;
     const double c1 = wave_sum(cva), c2 = wave_sum(cva * cva);
     if (lane == 0) {
         red[wave][kTenors] = c1;
