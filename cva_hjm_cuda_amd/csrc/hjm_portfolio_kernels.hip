@@ -157,13 +157,19 @@ __global__ __launch_bounds__(kBlock) void pf_value_kernel(const PfValueParams v,
             const int K = sk[gg];
             const Real* gam = sg + gg * KP;
             Real V = sa[gg];
-            // chunks of 4 periods up to the set's last live period; the guard is uniform
-            // and the indices into Pk are constants, so Pk stays in VGPRs
+            // chunks of 16 periods up to the set's last live period: the guard is uniform,
+            // the indices into Pk are constants (Pk stays in VGPRs), and each chunk's LDS
+            // reads are issued together ahead of its FMAs (one wait per 16 FMAs)
 #pragma unroll
-            for (int k0 = 0; k0 < KP; k0 += 4) {
+            for (int k0 = 0; k0 < KP; k0 += 16) {
                 if (k0 < K) {
+                    Real gk[16];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) V -= gam[k0 + u] * Pk[k0 + u];
+                    for (int u = 0; u < 16; ++u)
+                        if (k0 + u < KP) gk[u] = gam[k0 + u];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u)
+                        if (k0 + u < KP) V -= gk[u] * Pk[k0 + u];
                 }
             }
             const Real ex = V > (Real)0 ? (Real)((double)V * disc) : (Real)0;
