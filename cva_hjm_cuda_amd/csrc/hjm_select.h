@@ -36,8 +36,11 @@ struct SelectBits<float> {
     static constexpr int kBits = 32;
 };
 
+// stage: a row of at most kSelCand entries goes to LDS first and every pass reads it
+// there (the bracket selection's candidate rows).
 template <typename T>
-__device__ void select_kth_row(const T* __restrict__ row, int64_t n, int64_t k, double* out)
+__device__ void select_kth_row(const T* __restrict__ row, int64_t n, int64_t k, double* out,
+                               bool stage = false)
 {
     typedef typename SelectBits<T>::U U;
     constexpr int NB = SelectBits<T>::kBits;
@@ -57,6 +60,12 @@ __device__ void select_kth_row(const T* __restrict__ row, int64_t n, int64_t k, 
     }
     bool collected = false;
     uint32_t ncand = 0;
+    if (stage && n <= kSelCand) {
+        // visible to every thread after the first pass's barrier
+        for (int64_t t = tid; t < n; t += kSelBlock) cand[t] = __builtin_bit_cast(U, row[t]);
+        collected = true;
+        ncand = (uint32_t)n;
+    }
     int known = 0;
     while (known < NB) {
         const int width = (NB - known) < 11 ? (NB - known) : 11;
@@ -465,7 +474,7 @@ __global__ __launch_bounds__(kSelBlock) void bracket_finish_kernel(const T* __re
         }
         if (nin <= cap && !(s.claim & 0x80000000u)) {
             // (the top bit of claim: a counting pass could not stage every candidate)
-            select_kth_row<T>(cand + (int64_t)row * cap, nin, rk, out + row);
+            select_kth_row<T>(cand + (int64_t)row * cap, nin, rk, out + row, true);
             return;
         }
     }
