@@ -277,7 +277,9 @@ int hjm_generate_normals(uint64_t seed, int64_t path_offset, int64_t paths, int3
 
 /* ---- the PFE selection on caller data (no counterpart in the reference): out[r] = the
  * k-th smallest (1-based) of row r of values [rows][n], all >= 0 (exact, by the GPU
- * radix selection the PFE uses). HJM_ERR_ARG for negative or non-finite values.       */
+ * selection the PFE uses: a bracket from the row's first entries, one counting pass and
+ * a radix selection among the entries inside it for rows of >= 32768 values, MSD radix
+ * selection of the whole row otherwise). HJM_ERR_ARG for negative or non-finite values. */
 int hjm_select_kth(const double* values, int32_t rows, int64_t n, int64_t k, int32_t device,
                    double* out);
 
