@@ -155,7 +155,7 @@ def _declare(lib):
     lib.hjm_select_kth.argtypes = [_dp, C.c_int32, C.c_int64, C.c_int64, C.c_int32, _dp]
     lib.hjm_calibrate_pca.argtypes = [C.POINTER(_Calibration), C.POINTER(_CalibResults)]
     lib.hjm_generate_normals.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_int32, C.c_int32,
-                                         _dp]
+                                         C.c_int32, _dp]
 
 
 def _ptr(a: Optional[np.ndarray]):
@@ -393,10 +393,12 @@ def select_kth(values, k: int, device: int = 0) -> np.ndarray:
 
 
 def generate_normals(seed: int, paths: int, nsteps: int, path_offset: int = 0,
-                     device: int = 0) -> np.ndarray:
-    """The in-kernel generator's normals, [paths][nsteps][3], computed on the GPU."""
+                     device: int = 0, stride: int = 50) -> np.ndarray:
+    """The in-kernel generator's normals, [paths][nsteps][3], computed on the GPU, for runs
+    whose exposure dates are `stride` steps apart (dtau / dt)."""
     out = np.zeros((paths, nsteps, 3))
-    _check(library().hjm_generate_normals(seed, path_offset, paths, nsteps, device, _ptr(out)),
+    _check(library().hjm_generate_normals(seed, path_offset, paths, nsteps, stride, device,
+                                          _ptr(out)),
            "hjm_generate_normals")
     return out
 

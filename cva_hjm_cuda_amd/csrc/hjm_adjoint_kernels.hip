@@ -184,7 +184,7 @@ __global__ __launch_bounds__(kBlock) void ref_adjoint_sweep_kernel(const AdjGrid
                     const double* q = normals + ((int64_t)(active ? i : 0) * g.nsteps + n) * 3;
                     step((float)q[0], (float)q[1], (float)q[2]);
                 } else {
-                    const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n);
+                    const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n, (uint32_t)g.stride);
                     step(z.z0, z.z1, z.z2);
                 }
             }

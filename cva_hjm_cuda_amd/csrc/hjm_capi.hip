@@ -559,7 +559,8 @@ int hjm_simulate_reference_blockwise(const hjm_market* market, const hjm_swap* s
                                hipMemcpyHostToDevice, b.st),
                 HJM_ERR_DEVICE);
     else
-        HJM_HIP(launch_normals(cfg->seed, cfg->path_offset, P, S, b.normals, b.st), HJM_ERR_LAUNCH);
+        HJM_HIP(launch_normals(cfg->seed, cfg->path_offset, P, S, stride, b.normals, b.st),
+                HJM_ERR_LAUNCH);
     HJM_HIP(hipEventRecord(b.ev[1], b.st), HJM_ERR_DEVICE);
     double* cur = b.s0;
     double* nxt = b.s1;
@@ -622,10 +623,11 @@ int hjm_block_map(const double* drifts, const double* vols, double dt, double dt
 }
 
 int hjm_generate_normals(uint64_t seed, int64_t path_offset, int64_t paths, int32_t nsteps,
-                         int32_t device, double* out)
+                         int32_t stride, int32_t device, double* out)
 {
     if (!out) return HJM_ERR_NULL;
-    if (paths <= 0 || nsteps <= 0 || path_offset < 0 || paths * (int64_t)nsteps > (1ll << 31))
+    if (paths <= 0 || nsteps <= 0 || stride <= 0 || path_offset < 0 ||
+        paths * (int64_t)nsteps > (1ll << 31))
         return HJM_ERR_ARG;
     DeviceGuard guard;
     HJM_TRY(guard.set(device));
@@ -633,7 +635,7 @@ int hjm_generate_normals(uint64_t seed, int64_t path_offset, int64_t paths, int3
     const size_t n = (size_t)paths * nsteps * 3;
     HJM_TRY(dev_alloc(&d, n));
     int rc = HJM_OK;
-    if (launch_normals(seed, path_offset, paths, nsteps, d, nullptr) != hipSuccess)
+    if (launch_normals(seed, path_offset, paths, nsteps, stride, d, nullptr) != hipSuccess)
         rc = HJM_ERR_LAUNCH;
     else if (hipDeviceSynchronize() != hipSuccess)
         rc = HJM_ERR_LAUNCH;

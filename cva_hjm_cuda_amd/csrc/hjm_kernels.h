@@ -25,8 +25,8 @@ hipError_t launch_column_sum(const double* partials, int64_t rows, int cols, dou
 // k-th smallest of each row; ws: workspace of bracket_workspace_bytes (hjm_select.h) or null
 hipError_t launch_radix_select(const double* x, int rows, int64_t n, int64_t k, void* ws,
                                double* out, hipStream_t st);
-hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nsteps, double* out,
-                          hipStream_t st);
+hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nsteps, int stride,
+                          double* out, hipStream_t st);
 
 // Matrix-core FP32 path kernels (50 steps per date); prepare once per plan (sets the
 // kernels' dynamic LDS limits) before launching or capturing them.

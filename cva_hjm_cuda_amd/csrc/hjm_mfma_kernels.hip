@@ -159,28 +159,35 @@ __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const
         // steps 4 q + grp < 50 for every lane group up to q = 11; only the last quad pads
         constexpr bool last = 4 * q + 3 >= kMfmaStride;
         const int t = 4 * q + grp;
-        const int tc = (!last || t < kMfmaStride) ? t : kMfmaStride - 1;
         const bool in = !last || t < kMfmaStride;
+        // the lane group's class stream: shocks 3 q .. 3 q + 2 of this quad come from calls
+        // (3 q) / 4 .. (3 q + 2) / 4; the calls this quad needs first are made here (quads
+        // 3, 7 and 11 make none: 10 calls per block)
+        constexpr int chi = (3 * q + 2) / 4;
+        constexpr int clo = q == 0 ? 0 : (3 * q - 1) / 4 + 1;
 #pragma unroll
         for (int n = 0; n < kNT; ++n) {
-            const Normals3 v = step_normals(g.seed_lo, g.seed_hi, (uint32_t)(g.path0 + path[n]),
-                                            (uint32_t)(b * kMfmaStride + tc));
-            if constexpr (kStore) {
-                if (in && live[n]) {
-                    float* q = zs + (int64_t)(b * kMfmaStride + t) * 3 * g.paths + path[n];
-                    q[0] = v.z0;
-                    q[g.paths] = v.z1;
-                    q[2 * g.paths] = v.z2;
-                }
+#pragma unroll
+            for (int c = clo; c <= chi; ++c) {
+                float v[4];
+                stream_call(g.seed_lo, g.seed_hi, (uint32_t)(g.path0 + path[n]), (uint32_t)b,
+                            (uint32_t)grp, (uint32_t)c, v);
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (4 * c + e < 3 * kMfmaQuads) z[n][4 * c + e] = v[e];
             }
             if constexpr (last) {
-                z[n][3 * q + 0] = in ? v.z0 : (t == kMfmaStride ? 1.f : 0.f);
-                z[n][3 * q + 1] = in ? v.z1 : 0.f;
-                z[n][3 * q + 2] = in ? v.z2 : 0.f;
-            } else {
-                z[n][3 * q + 0] = v.z0;
-                z[n][3 * q + 1] = v.z1;
-                z[n][3 * q + 2] = v.z2;
+                z[n][3 * q + 0] = in ? z[n][3 * q + 0] : (t == kMfmaStride ? 1.f : 0.f);
+                z[n][3 * q + 1] = in ? z[n][3 * q + 1] : 0.f;
+                z[n][3 * q + 2] = in ? z[n][3 * q + 2] : 0.f;
+            }
+            if constexpr (kStore) {
+                if (in && live[n]) {
+                    float* o = zs + (int64_t)(b * kMfmaStride + t) * 3 * g.paths + path[n];
+                    o[0] = z[n][3 * q + 0];
+                    o[g.paths] = z[n][3 * q + 1];
+                    o[2 * g.paths] = z[n][3 * q + 2];
+                }
             }
         }
 #pragma unroll

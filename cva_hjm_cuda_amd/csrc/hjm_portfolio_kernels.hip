@@ -65,7 +65,7 @@ __global__ __launch_bounds__(kBlock) void pf_paths_kernel(const RefStepConsts<Re
                     z1 = (Real)q[1];
                     z2 = (Real)q[2];
                 } else {
-                    const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n);
+                    const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n, (uint32_t)g.stride);
                     z0 = (Real)z.z0;
                     z1 = (Real)z.z1;
                     z2 = (Real)z.z2;
@@ -690,7 +690,7 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_sweep_kernel(const PfGrid g
                     const double* q = normals + ((int64_t)ii * g.nsteps + n) * 3;
                     step(n, (float)q[0], (float)q[1], (float)q[2]);
                 } else {
-                    const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n);
+                    const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n, (uint32_t)g.stride);
                     step(n, z.z0, z.z1, z.z2);
                 }
             }

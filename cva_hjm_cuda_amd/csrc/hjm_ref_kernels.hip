@@ -53,7 +53,7 @@ __device__ __forceinline__ kconst_ptr<RefStepConsts<Real>> step_consts_ptr()
 // Per tenor and step the update is five FMAs and one add, ordered so that the old
 // neighbour f_{j+1} is read first and the last FMA rewrites f_j in place: no copies.
 template <typename Real, bool kCompensated, bool kHostNormals>
-__global__ __launch_bounds__(kBlock) void ref_paths_kernel(const RefStepConsts<Real> k,
+__global__ __launch_bounds__(kBlock, 2) void ref_paths_kernel(const RefStepConsts<Real> k,
                                                            const RefGrid g,
                                                            const double* __restrict__ normals,
                                                            double* __restrict__ num_x,
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void ref_paths_kernel(const RefStepConsts<R
                 z1 = (Real)q[1];
                 z2 = (Real)q[2];
             } else {
-                const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n);
+                const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n, (uint32_t)g.stride);
                 z0 = (Real)z.z0;
                 z1 = (Real)z.z1;
                 z2 = (Real)z.z2;
@@ -230,19 +230,31 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(const double* __rest
     }
 }
 
-// The in-kernel generator's normals for [path0, path0 + paths) x [0, nsteps).
+// The in-kernel generator's normals for [path0, path0 + paths) x [0, nsteps), [path][step][3]:
+// one thread per (path, date block, class) walks its class stream.
 __global__ __launch_bounds__(kBlock) void normals_kernel(uint32_t seed_lo, uint32_t seed_hi,
                                                          int64_t path0, int64_t paths,
-                                                         int nsteps, double* __restrict__ out)
+                                                         int nsteps, int stride,
+                                                         double* __restrict__ out)
 {
+    const int64_t nblk = (nsteps + stride - 1) / stride;
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= paths * nsteps) return;
-    const int64_t p = t / nsteps;
-    const int n = (int)(t % nsteps);
-    const Normals3 z = step_normals(seed_lo, seed_hi, (uint32_t)(path0 + p), (uint32_t)n);
-    out[t * 3 + 0] = z.z0;
-    out[t * 3 + 1] = z.z1;
-    out[t * 3 + 2] = z.z2;
+    if (t >= paths * nblk * 4) return;
+    const int cls = (int)(t & 3);
+    const int64_t pb = t >> 2;
+    const int64_t p = pb / nblk;
+    const int b = (int)(pb % nblk);
+    StreamCache c;
+    c.reset();
+    for (int q = 0; 4 * q + cls < stride; ++q) {
+        const int n = b * stride + 4 * q + cls;
+        if (n >= nsteps) break;
+        const Normals3 z = c.step(seed_lo, seed_hi, (uint32_t)(path0 + p), (uint32_t)b, (uint32_t)cls, q);
+        double* o = out + (p * nsteps + n) * 3;
+        o[0] = z.z0;
+        o[1] = z.z1;
+        o[2] = z.z2;
+    }
 }
 
 // ---- launchers -------------------------------------------------------------------
@@ -285,13 +297,14 @@ hipError_t launch_column_sum(const double* partials, int64_t rows, int cols, dou
     return hipGetLastError();
 }
 
-hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nsteps, double* out,
-                          hipStream_t st)
+hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nsteps, int stride,
+                          double* out, hipStream_t st)
 {
-    const int64_t total = paths * nsteps;
+    if (stride <= 0) return hipErrorInvalidValue;
+    const int64_t total = paths * ((nsteps + stride - 1) / stride) * 4;
     const dim3 grid((unsigned)((total + kBlock - 1) / kBlock));
     normals_kernel<<<grid, kBlock, 0, st>>>((uint32_t)seed, (uint32_t)(seed >> 32), path0, paths,
-                                            nsteps, out);
+                                            nsteps, stride, out);
     return hipGetLastError();
 }
 

@@ -1,12 +1,19 @@
 // hjm_rng.h -- counter-based normal generator for the HJM path kernels.
 //
-// Step n of Monte Carlo path p draws its three factor shocks from one Philox4x32-10
-// block (Salmon et al., SC'11; constants as published with Random123) keyed by the
-// 64-bit seed, with counter (n, p, 0, 0). Box-Muller turns the four 32-bit outputs into
-// three N(0,1) floats: z0, z1 = r cos, r sin of (x0, x1); z2 = r' cos of (x2, x3). ln,
-// sin and cos are fixed FP32 polynomials evaluated with explicit FMAs (coefficients from
-// rng_poly_fit.py), so host and GPU produce the same bits; the square root is IEEE-rounded on
-// both.
+// The steps of a run are grouped in date blocks of `stride` steps (dtau / dt). Step n lies
+// in block b = n / stride at offset s = n % stride; it belongs to class g = s % 4 and is
+// the q-th (q = s / 4) step of that class in the block. The class's shocks form a stream:
+// shock j = 3 q + k is factor k of its q-th step. Call c of the stream (path p, block b,
+// class g) is one Philox4x32-10 block (Salmon et al., SC'11; constants as published with
+// Random123) keyed by the 64-bit seed, with counter (b, p, g, c); Box-Muller turns its
+// four 32-bit outputs into the stream's shocks 4c .. 4c + 3:
+//   (4c, 4c + 1) = r cos, r sin of (x0, x1);  (4c + 2, 4c + 3) = r cos, r sin of (x2, x3).
+// A 50-step block thus costs 38 Philox blocks and 76 Box-Muller pairs (0.76 and 1.52 per
+// step) against one block and 1.75 pairs per step when every step draws its own block,
+// and a matrix-core lane, which generates the steps of one class, makes every call once.
+// ln, sin and cos are fixed FP32 polynomials evaluated with explicit FMAs (coefficients
+// from rng_poly_fit.py), so host and GPU produce the same bits; the square root is
+// IEEE-rounded on both.
 //
 // Counter addressing makes any (path, step) directly reachable: paths shard across
 // devices without stream offsets, and the adjoint sweep regenerates shocks in reverse.
@@ -141,23 +148,95 @@ HJM_HD float rng_sqrt(float x)
 #endif
 }
 
-HJM_HD Normals3 step_normals(uint32_t seed_lo, uint32_t seed_hi, uint32_t path, uint32_t step)
+// (z0, z1) = r cos, r sin of the uniforms' pair (a, b): r = sqrt(-2 ln u(a)),
+// angle 2 pi (b >> 8) / 2^24, u(a) = ((a >> 8) + 1) 2^-24.
+HJM_HD void box_muller(uint32_t a, uint32_t b, float& z0, float& z1)
 {
 #pragma clang fp contract(off)
-    uint32_t x[4];
-    philox4x32_10(step, path, 0u, 0u, seed_lo, seed_hi, x);
-    const float u0 = (float)((x[0] >> 8) + 1u) * 0x1p-24f;
-    const float u2 = (float)((x[2] >> 8) + 1u) * 0x1p-24f;
-    const float r0 = rng_sqrt(-2.0f * rng_log(u0));
-    const float r2 = rng_sqrt(-2.0f * rng_log(u2));
-    float s0, c0;
-    rng_sincos_turn24(x[1] >> 8, s0, c0);
-    const float c2 = rng_cos_turn24(x[3] >> 8);
-    Normals3 z;
-    z.z0 = r0 * c0;
-    z.z1 = r0 * s0;
-    z.z2 = r2 * c2;
-    return z;
+    const float u = (float)((a >> 8) + 1u) * 0x1p-24f;
+    const float r = rng_sqrt(-2.0f * rng_log(u));
+    float s, c;
+    rng_sincos_turn24(b >> 8, s, c);
+    z0 = r * c;
+    z1 = r * s;
 }
+
+// Shocks 4 call .. 4 call + 3 of the stream (path, block, cls).
+HJM_HD void stream_call(uint32_t seed_lo, uint32_t seed_hi, uint32_t path, uint32_t block,
+                        uint32_t cls, uint32_t call, float (&z)[4])
+{
+    uint32_t x[4];
+    philox4x32_10(block, path, cls, call, seed_lo, seed_hi, x);
+    box_muller(x[0], x[1], z[0], z[1]);
+    box_muller(x[2], x[3], z[2], z[3]);
+}
+
+// The three factor shocks of step n, computed directly: one call when they lie in one
+// call's four shocks (q % 4 = 0 or 3), two otherwise. The one-path-per-lane kernels use
+// this: a cache of the four classes' last calls would hold 16 more registers per lane
+// across the step loop and halve their occupancy.
+HJM_HD Normals3 step_normals(uint32_t seed_lo, uint32_t seed_hi, uint32_t path, uint32_t n,
+                             uint32_t stride)
+{
+    const uint32_t b = n / stride, s = n % stride, g = s & 3u, j0 = 3u * (s >> 2);
+    const uint32_t c0 = j0 >> 2;
+    float a[4], c[4];
+    // one branch per position of the step's first shock in its call (uniform across a
+    // wave): each keeps only the Box-Muller outputs it returns
+    switch (j0 & 3u) {
+    case 0u:
+        stream_call(seed_lo, seed_hi, path, b, g, c0, a);
+        return Normals3{a[0], a[1], a[2]};
+    case 1u:
+        stream_call(seed_lo, seed_hi, path, b, g, c0, a);
+        return Normals3{a[1], a[2], a[3]};
+    case 2u:
+        stream_call(seed_lo, seed_hi, path, b, g, c0, a);
+        stream_call(seed_lo, seed_hi, path, b, g, c0 + 1u, c);
+        return Normals3{a[2], a[3], c[0]};
+    default:
+        stream_call(seed_lo, seed_hi, path, b, g, c0, a);
+        stream_call(seed_lo, seed_hi, path, b, g, c0 + 1u, c);
+        return Normals3{a[3], c[0], c[1]};
+    }
+}
+
+// Walks the steps of one class through a date block in either direction, holding the
+// stream's last call so that each call is made once (the walk visits q, q + 1, ... or
+// q, q - 1, ...; at most one call per step is new). Reset at every block.
+struct StreamCache {
+    float z[4];
+    int held;  // call index held, -1 none
+    HJM_HD void reset() { held = -1; }
+    HJM_HD float pick(const float (&v)[4], int e) const
+    {
+        return e == 0 ? v[0] : e == 1 ? v[1] : e == 2 ? v[2] : v[3];
+    }
+    // shocks of the q-th step of class cls in block b
+    HJM_HD Normals3 step(uint32_t seed_lo, uint32_t seed_hi, uint32_t path, uint32_t b,
+                         uint32_t cls, int q)
+    {
+        const int j0 = 3 * q, c0 = j0 >> 2, c1 = (j0 + 2) >> 2;
+        if (held != c0 && held != c1) {
+            stream_call(seed_lo, seed_hi, path, b, cls, (uint32_t)c0, z);
+            held = c0;
+        }
+        const int other = held == c0 ? c1 : c0;
+        float w[4] = {z[0], z[1], z[2], z[3]};
+        if (other != held) stream_call(seed_lo, seed_hi, path, b, cls, (uint32_t)other, w);
+        float r[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int j = j0 + k;
+            r[k] = (j >> 2) == held ? pick(z, j & 3) : pick(w, j & 3);
+        }
+        if (other != held) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) z[u] = w[u];
+            held = other;
+        }
+        return Normals3{r[0], r[1], r[2]};
+    }
+};
 
 }  // namespace hjm

@@ -270,10 +270,11 @@ typedef struct {
 int hjm_calibrate_pca(const hjm_calibration* calibration, hjm_calibration_results* results);
 
 /* ---- normals (for tests and host-supplied runs): the in-kernel generator's output
- * for paths [path_offset, path_offset + paths) and steps [0, nsteps), written as
- * [paths][nsteps][3] doubles. Runs on the GPU.                                       */
+ * for paths [path_offset, path_offset + paths) and steps [0, nsteps) of a run whose
+ * exposure dates are `stride` steps apart (dtau / dt; the generator's streams follow the
+ * date blocks), written as [paths][nsteps][3] doubles. Runs on the GPU.              */
 int hjm_generate_normals(uint64_t seed, int64_t path_offset, int64_t paths, int32_t nsteps,
-                         int32_t device, double* out);
+                         int32_t stride, int32_t device, double* out);
 
 /* ---- the PFE selection on caller data (no counterpart in the reference): out[r] = the
  * k-th smallest (1-based) of row r of values [rows][n], all >= 0 (exact, by the GPU

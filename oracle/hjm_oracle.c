@@ -116,31 +116,52 @@ void orc_sincos_turn24_f32(uint32_t k24, float* s, float* c)
     }
 }
 
-void orc_step_normals(uint64_t seed, uint32_t path, uint32_t step, float z[3])
+/* (z0, z1) = r cos, r sin: r = sqrt(-2 ln u), u = ((a >> 8) + 1) 2^-24, angle 2 pi (b >> 8) 2^-24. */
+void orc_box_muller(uint32_t a, uint32_t b, float* z0, float* z1)
 {
-    uint32_t ctr[4] = {step, path, 0u, 0u};
+    float u = (float)((a >> 8) + 1u) * 0x1p-24f;
+    float r = sqrtf(-2.0f * orc_log_f32(u));
+    float s, c;
+    orc_sincos_turn24_f32(b >> 8, &s, &c);
+    *z0 = r * c;
+    *z1 = r * s;
+}
+
+/* Shocks 4 call .. 4 call + 3 of the class stream (path, block, cls): Philox4x32-10 with
+ * counter (block, path, cls, call) and key = seed; Box-Muller on (x0, x1) and (x2, x3). */
+void orc_stream_call(uint64_t seed, uint32_t path, uint32_t block, uint32_t cls, uint32_t call,
+                     float z[4])
+{
+    uint32_t ctr[4] = {block, path, cls, call};
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     uint32_t x[4];
     orc_philox4x32_10(ctr, key, x);
-    float u0 = (float)((x[0] >> 8) + 1u) * 0x1p-24f;
-    float u2 = (float)((x[2] >> 8) + 1u) * 0x1p-24f;
-    float r0 = sqrtf(-2.0f * orc_log_f32(u0));
-    float r2 = sqrtf(-2.0f * orc_log_f32(u2));
-    float s0, c0, s2, c2;
-    orc_sincos_turn24_f32(x[1] >> 8, &s0, &c0);
-    orc_sincos_turn24_f32(x[3] >> 8, &s2, &c2);
-    z[0] = r0 * c0;
-    z[1] = r0 * s0;
-    z[2] = r2 * c2;
+    orc_box_muller(x[0], x[1], &z[0], &z[1]);
+    orc_box_muller(x[2], x[3], &z[2], &z[3]);
+}
+
+/* Step n of a run whose date blocks are `stride` steps long: block b = n / stride, offset
+ * s = n % stride, class g = s % 4, q = s / 4; its factor k is shock j = 3 q + k of the
+ * stream (path, b, g), i.e. element j % 4 of call j / 4. */
+void orc_step_normals(uint64_t seed, uint32_t path, uint32_t step, uint32_t stride, float z[3])
+{
+    uint32_t b = step / stride, s = step % stride, g = s % 4u, q = s / 4u;
+    for (int k = 0; k < 3; ++k) {
+        uint32_t j = 3u * q + (uint32_t)k;
+        float c[4];
+        orc_stream_call(seed, path, b, g, j / 4u, c);
+        z[k] = c[j % 4u];
+    }
 }
 
 /* Layout [path][step][3], as the reference stores its normals (simulation_cpu.h:205-207). */
-void orc_fill_normals(uint64_t seed, int64_t path0, int npaths, int nsteps, double* out)
+void orc_fill_normals(uint64_t seed, int64_t path0, int npaths, int nsteps, int stride,
+                      double* out)
 {
     for (int p = 0; p < npaths; ++p) {
         for (int n = 0; n < nsteps; ++n) {
             float z[3];
-            orc_step_normals(seed, (uint32_t)(path0 + p), (uint32_t)n, z);
+            orc_step_normals(seed, (uint32_t)(path0 + p), (uint32_t)n, (uint32_t)stride, z);
             double* o = out + ((int64_t)p * nsteps + n) * 3;
             o[0] = z[0];
             o[1] = z[1];

@@ -30,8 +30,12 @@ extern "C" {
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 float orc_log_f32(float u);
 void orc_sincos_turn24_f32(uint32_t k24, float* s, float* c);
-void orc_step_normals(uint64_t seed, uint32_t path, uint32_t step, float z[3]);
-void orc_fill_normals(uint64_t seed, int64_t path0, int npaths, int nsteps, double* out);
+void orc_box_muller(uint32_t a, uint32_t b, float* z0, float* z1);
+void orc_stream_call(uint64_t seed, uint32_t path, uint32_t block, uint32_t cls, uint32_t call,
+                     float z[4]);
+void orc_step_normals(uint64_t seed, uint32_t path, uint32_t step, uint32_t stride, float z[3]);
+void orc_fill_normals(uint64_t seed, int64_t path0, int npaths, int nsteps, int stride,
+                      double* out);
 
 /* ---- market data ---------------------------------------------------------------- */
 void orc_cumulative_simpson(const double* y, int n, double h, double* out);

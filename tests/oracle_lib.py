@@ -27,8 +27,12 @@ class Oracle:
         L.orc_log_f32.argtypes = [C.c_float]
         L.orc_log_f32.restype = C.c_float
         L.orc_sincos_turn24_f32.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
-        L.orc_step_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, _f32p]
-        L.orc_fill_normals.argtypes = [C.c_uint64, C.c_int64, C.c_int, C.c_int, _dp]
+        L.orc_box_muller.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_float),
+                                     C.POINTER(C.c_float)]
+        L.orc_stream_call.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.c_uint32, _f32p]
+        L.orc_step_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _f32p]
+        L.orc_fill_normals.argtypes = [C.c_uint64, C.c_int64, C.c_int, C.c_int, C.c_int, _dp]
         L.orc_cumulative_simpson.argtypes = [_dp, C.c_int, C.c_double, _dp]
         L.orc_hjm_drift.argtypes = [_dp, C.c_int, C.c_int, C.c_double, _dp]
         L.orc_nelson_siegel.argtypes = [C.c_double] * 4 + [C.c_int, C.c_double, _dp]
@@ -77,14 +81,25 @@ class Oracle:
         self.lib.orc_sincos_turn24_f32(int(k), C.byref(s), C.byref(c))
         return s.value, c.value
 
-    def step_normals(self, seed, path, step):
-        z = np.zeros(3, np.float32)
-        self.lib.orc_step_normals(seed, path, step, z)
+    def box_muller(self, a, b):
+        z0, z1 = C.c_float(), C.c_float()
+        self.lib.orc_box_muller(int(a), int(b), C.byref(z0), C.byref(z1))
+        return z0.value, z1.value
+
+    def stream_call(self, seed, path, block, cls, call):
+        z = np.zeros(4, np.float32)
+        self.lib.orc_stream_call(seed, path, block, cls, call, z)
         return z
 
-    def normals(self, seed, npaths, nsteps, path0=0):
+    def step_normals(self, seed, path, step, stride=50):
+        z = np.zeros(3, np.float32)
+        self.lib.orc_step_normals(seed, path, step, stride, z)
+        return z
+
+    def normals(self, seed, npaths, nsteps, path0=0, stride=50):
+        """[npaths][nsteps][3] shocks of a run with date blocks of `stride` steps."""
         out = np.zeros((npaths, nsteps, 3), np.float64)
-        self.lib.orc_fill_normals(seed, path0, npaths, nsteps, out)
+        self.lib.orc_fill_normals(seed, path0, npaths, nsteps, stride, out)
         return out
 
     # -- market ------------------------------------------------------------------

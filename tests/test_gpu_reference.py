@@ -50,10 +50,11 @@ def _rel(a, b):
 
 def test_gpu_normals_bit_identical_to_oracle(oracle):
     _flush("[gpu] normals")
-    for seed, p0 in ((42, 0), (7, 1000), (0x0123456789ABCDEF, 123456)):
-        g = hjm.generate_normals(seed, 48, 260, path_offset=p0)
-        o = oracle.normals(seed, 48, 260, path0=p0)
-        assert np.array_equal(g, o), seed
+    for seed, p0, stride in ((42, 0, 50), (7, 1000, 50), (0x0123456789ABCDEF, 123456, 50),
+                             (42, 17, 20), (9, 5, 7)):
+        g = hjm.generate_normals(seed, 48, 260, path_offset=p0, stride=stride)
+        o = oracle.normals(seed, 48, 260, path0=p0, stride=stride)
+        assert np.array_equal(g, o), (seed, stride)
 
 
 def test_gpu_normals_bit_identical_large_sample(oracle):

@@ -4,6 +4,7 @@
   open-source implementation (fixture tests/golden/philox_vectors.json).
 * The FP32 polynomial ln / sin / cos against float64 numpy, in float32 ulp.
 * The normals: moments and a Kolmogorov-Smirnov test on a large sample.
+* The class-stream layout, restated from the Philox and Box-Muller primitives.
 """
 import json
 import math
@@ -96,3 +97,29 @@ def test_normals_counter_addressing(oracle):
     assert np.array_equal(whole[7, 33], z.astype(np.float64))
     other = oracle.normals(seed=43, npaths=9, nsteps=60)
     assert not np.any(whole == other)
+
+
+def test_class_stream_layout(oracle):
+    """Step n of a run with date blocks of S steps: block b = n // S, offset s = n % S,
+    class g = s % 4, q = s // 4; factor k is shock j = 3 q + k of the stream (b, g), i.e.
+    element j % 4 of the Philox block with counter (b, path, g, j // 4), whose outputs go
+    through Box-Muller as the pairs (x0, x1) and (x2, x3). Restated here from the
+    primitives, for several strides (the product's kernels follow the same layout)."""
+    seed = (7 << 32) | 12345
+    key = (seed & 0xFFFFFFFF, seed >> 32)
+    for stride in (50, 20, 7, 4, 1):
+        for path in (0, 3, 99991):
+            for n in range(3 * stride + 5):
+                b, s = divmod(n, stride)
+                g, q = s % 4, s // 4
+                want = []
+                for k in range(3):
+                    j = 3 * q + k
+                    x = [int(v) for v in oracle.philox((b, path, g, j // 4), key)]
+                    z = oracle.box_muller(x[0], x[1]) + oracle.box_muller(x[2], x[3])
+                    want.append(z[j % 4])
+                got = oracle.step_normals(seed, path, n, stride)
+                assert np.array_equal(got, np.array(want, np.float32)), (stride, path, n)
+    # one call of a 50-step block feeds four shocks: 38 calls cover its 150 shocks
+    calls = {(g, (3 * (s // 4) + k) // 4) for s in range(50) for k in range(3) for g in [s % 4]}
+    assert len(calls) == 38
