@@ -76,6 +76,9 @@ __global__ __launch_bounds__(kBlock) void ref_adjoint_sweep_kernel(const AdjGrid
     // per wave FP64 accumulators: pass 0 -> g0, g1 (102); pass 1 -> g2, gm, dspot (153)
     constexpr int kAcc = 3 * kTenors;
     __shared__ double acc[kBlock / kWave][kAcc];
+    // each lane's column of regenerated shocks (16 steps at a time)
+    __shared__ float zbuf[kSrc == 0 ? kGroupShocks * kBlock : 1];
+    float* col = zbuf + (kSrc == 0 ? threadIdx.x : 0);
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
     for (int k = lane; k < kAcc; k += kWave) acc[wave][k] = 0.0;
     __syncthreads();
@@ -178,13 +181,22 @@ __global__ __launch_bounds__(kBlock) void ref_adjoint_sweep_kernel(const AdjGrid
                 }
             }
         } else {
+            const int whole = g.stride / kGroupSteps * kGroupSteps;
             for (int s = g.stride - 1; s >= 0; --s) {
                 const int n = b * g.stride + s;  // global step index of this Euler step
                 if constexpr (kSrc == 1) {
                     const double* q = normals + ((int64_t)(active ? i : 0) * g.nsteps + n) * 3;
                     step((float)q[0], (float)q[1], (float)q[2]);
                 } else {
-                    const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n, (uint32_t)g.stride);
+                    // groups of 16 steps staged in LDS, entered from their last step
+                    Normals3 z;
+                    if (s < whole) {
+                        if ((s & (kGroupSteps - 1)) == kGroupSteps - 1)
+                            fill_group(g.seed_lo, g.seed_hi, pid, (uint32_t)b, s / kGroupSteps, col, kBlock);
+                        z = group_step(col, kBlock, s);
+                    } else {
+                        z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n, (uint32_t)g.stride);
+                    }
                     step(z.z0, z.z1, z.z2);
                 }
             }

@@ -41,6 +41,12 @@ __global__ __launch_bounds__(kBlock) void pf_paths_kernel(const RefStepConsts<Re
     const int64_t P = g.paths;
     Real cv = k.c, mcv = -k.c, omc = k.omc, opc = k.opc;
     asm volatile("" : "+v"(cv), "+v"(mcv), "+v"(omc), "+v"(opc));
+    // each lane's column of staged shocks (16 steps at a time; dynamic LDS of
+    // kGroupShocks x blockDim floats when the normals are generated here)
+    extern __shared__ float pf_zbuf[];
+    float* col = pf_zbuf + threadIdx.x;
+    const int ld = (int)blockDim.x;
+    const int whole = g.stride / kGroupSteps * kGroupSteps;
     const int64_t ntiles = (P + blockDim.x - 1) / blockDim.x;
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int64_t i = tile * blockDim.x + threadIdx.x;
@@ -65,7 +71,14 @@ __global__ __launch_bounds__(kBlock) void pf_paths_kernel(const RefStepConsts<Re
                     z1 = (Real)q[1];
                     z2 = (Real)q[2];
                 } else {
-                    const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n, (uint32_t)g.stride);
+                    Normals3 z;
+                    if (s < whole) {
+                        if ((s & (kGroupSteps - 1)) == 0)
+                            fill_group(g.seed_lo, g.seed_hi, pid, (uint32_t)e, s / kGroupSteps, col, ld);
+                        z = group_step(col, ld, s);
+                    } else {
+                        z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n, (uint32_t)g.stride);
+                    }
                     z0 = (Real)z.z0;
                     z1 = (Real)z.z1;
                     z2 = (Real)z.z2;
@@ -385,7 +398,8 @@ static hipError_t pf_paths_launch(const RefStepConsts<Real>& k, const PfGrid& g,
     if (normals)
         pf_paths_kernel<Real, true><<<grid, block, 0, st>>>(k, g, normals, curves, logb);
     else
-        pf_paths_kernel<Real, false><<<grid, block, 0, st>>>(k, g, normals, curves, logb);
+        pf_paths_kernel<Real, false><<<grid, block, (size_t)kGroupShocks * block * sizeof(float), st>>>(
+            k, g, normals, curves, logb);
     return hipGetLastError();
 }
 
@@ -588,6 +602,9 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_sweep_kernel(const PfGrid g
 {
     constexpr int kAcc = 3 * kTenors;
     __shared__ double acc[kBlock / kWave][kAcc];
+    // each lane's column of regenerated shocks (16 steps at a time)
+    __shared__ float zbuf[kSrc == 0 ? kGroupShocks * kBlock : 1];
+    float* col = zbuf + (kSrc == 0 ? threadIdx.x : 0);
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
     for (int k = lane; k < kAcc; k += kWave) acc[wave][k] = 0.0;
     __syncthreads();
@@ -684,13 +701,22 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_sweep_kernel(const PfGrid g
                 }
             }
         } else {
+            const int whole = g.stride / kGroupSteps * kGroupSteps;
             for (int s = g.stride - 1; s >= 0; --s) {
                 const int n = e * g.stride + s;
                 if constexpr (kSrc == 1) {
                     const double* q = normals + ((int64_t)ii * g.nsteps + n) * 3;
                     step(n, (float)q[0], (float)q[1], (float)q[2]);
                 } else {
-                    const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n, (uint32_t)g.stride);
+                    // groups of 16 steps staged in LDS, entered from their last step
+                    Normals3 z;
+                    if (s < whole) {
+                        if ((s & (kGroupSteps - 1)) == kGroupSteps - 1)
+                            fill_group(g.seed_lo, g.seed_hi, pid, (uint32_t)e, s / kGroupSteps, col, kBlock);
+                        z = group_step(col, kBlock, s);
+                    } else {
+                        z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n, (uint32_t)g.stride);
+                    }
                     step(n, z.z0, z.z1, z.z2);
                 }
             }

@@ -80,6 +80,10 @@ __global__ __launch_bounds__(kBlock, 2) void ref_paths_kernel(const RefStepConst
     num_x[P + i] = (double)k.spot[1];
     num_y[P + i] = exp(-g.a0[1] * g.dt);
 
+    // each lane's column of staged shocks (16 steps at a time)
+    __shared__ float zbuf[kHostNormals ? 1 : kGroupShocks * kBlock];
+    float* col = zbuf + (kHostNormals ? 0 : threadIdx.x);
+    const int whole = g.stride / kGroupSteps * kGroupSteps;
     for (int b = 1; b < g.ndates; ++b) {
         for (int s = 0; s < g.stride; ++s) {
             const int n = b * g.stride + s;
@@ -90,7 +94,14 @@ __global__ __launch_bounds__(kBlock, 2) void ref_paths_kernel(const RefStepConst
                 z1 = (Real)q[1];
                 z2 = (Real)q[2];
             } else {
-                const Normals3 z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n, (uint32_t)g.stride);
+                Normals3 z;
+                if (s < whole) {
+                    if ((s & (kGroupSteps - 1)) == 0)
+                        fill_group(g.seed_lo, g.seed_hi, pid, (uint32_t)b, s / kGroupSteps, col, kBlock);
+                    z = group_step(col, kBlock, s);
+                } else {
+                    z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n, (uint32_t)g.stride);
+                }
                 z0 = (Real)z.z0;
                 z1 = (Real)z.z1;
                 z2 = (Real)z.z2;

@@ -172,9 +172,7 @@ HJM_HD void stream_call(uint32_t seed_lo, uint32_t seed_hi, uint32_t path, uint3
 }
 
 // The three factor shocks of step n, computed directly: one call when they lie in one
-// call's four shocks (q % 4 = 0 or 3), two otherwise. The one-path-per-lane kernels use
-// this: a cache of the four classes' last calls would hold 16 more registers per lane
-// across the step loop and halve their occupancy.
+// call's four shocks (q % 4 = 0 or 3), two otherwise.
 HJM_HD Normals3 step_normals(uint32_t seed_lo, uint32_t seed_hi, uint32_t path, uint32_t n,
                              uint32_t stride)
 {
@@ -199,6 +197,37 @@ HJM_HD Normals3 step_normals(uint32_t seed_lo, uint32_t seed_hi, uint32_t path, 
         stream_call(seed_lo, seed_hi, path, b, g, c0 + 1u, c);
         return Normals3{a[3], c[0], c[1]};
     }
+}
+
+// Steps [16 m, 16 m + 16) of a date block are quads 4 m .. 4 m + 3 of every class, i.e.
+// calls 3 m .. 3 m + 2 of the four class streams: 12 calls, 48 shocks, all used. Kernels
+// that walk a block step by step (one path per lane) stage a group's shocks in LDS, in a
+// column of their own (rows `ld` floats apart), and read each step's three back: 0.75
+// calls per step and nothing carried in registers across steps. Steps past the last
+// whole group of a block (2 of 50) use step_normals.
+constexpr int kGroupSteps = 16;
+constexpr int kGroupShocks = 48;
+
+__device__ __forceinline__ void fill_group(uint32_t seed_lo, uint32_t seed_hi, uint32_t path,
+                                           uint32_t block, int m, float* col, int ld)
+{
+    // one call at a time (the caller's loop holds the path state in registers)
+#pragma unroll 1
+    for (int g = 0; g < 4; ++g)
+#pragma unroll 1
+        for (int cc = 0; cc < 3; ++cc) {
+            float v[4];
+            stream_call(seed_lo, seed_hi, path, block, (uint32_t)g, (uint32_t)(3 * m + cc), v);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) col[(12 * g + 4 * cc + e) * ld] = v[e];
+        }
+}
+
+// shocks of step s of the staged group: class s % 4, quad (s / 4) % 4 of the group
+__device__ __forceinline__ Normals3 group_step(const float* col, int ld, int s)
+{
+    const int r = 12 * (s & 3) + 3 * ((s >> 2) & 3);
+    return Normals3{col[r * ld], col[(r + 1) * ld], col[(r + 2) * ld]};
 }
 
 // Walks the steps of one class through a date block in either direction, holding the
