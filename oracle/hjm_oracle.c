@@ -154,18 +154,31 @@ void orc_step_normals(uint64_t seed, uint32_t path, uint32_t step, uint32_t stri
     }
 }
 
-/* Layout [path][step][3], as the reference stores its normals (simulation_cpu.h:205-207). */
+/* Layout [path][step][3], as the reference stores its normals (simulation_cpu.h:205-207).
+ * Walks each stream (path, block, class) once: every call is made once and its four
+ * shocks are scattered to their steps (the same values as orc_step_normals). */
 void orc_fill_normals(uint64_t seed, int64_t path0, int npaths, int nsteps, int stride,
                       double* out)
 {
+    const int nblocks = (nsteps + stride - 1) / stride;
     for (int p = 0; p < npaths; ++p) {
-        for (int n = 0; n < nsteps; ++n) {
-            float z[3];
-            orc_step_normals(seed, (uint32_t)(path0 + p), (uint32_t)n, (uint32_t)stride, z);
-            double* o = out + ((int64_t)p * nsteps + n) * 3;
-            o[0] = z[0];
-            o[1] = z[1];
-            o[2] = z[2];
+        for (int b = 0; b < nblocks; ++b) {
+            for (int g = 0; g < 4 && g < stride; ++g) {
+                const int nq = (stride - g + 3) / 4;  /* steps of class g in a block */
+                const int ncall = (3 * nq + 3) / 4;
+                for (int c = 0; c < ncall; ++c) {
+                    float z[4];
+                    orc_stream_call(seed, (uint32_t)(path0 + p), (uint32_t)b, (uint32_t)g,
+                                    (uint32_t)c, z);
+                    for (int e = 0; e < 4; ++e) {
+                        const int j = 4 * c + e, q = j / 3, k = j % 3;
+                        if (q >= nq) break;
+                        const int n = b * stride + 4 * q + g;
+                        if (n >= nsteps) continue;
+                        out[((int64_t)p * nsteps + n) * 3 + k] = z[e];
+                    }
+                }
+            }
         }
     }
 }

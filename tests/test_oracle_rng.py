@@ -120,6 +120,11 @@ def test_class_stream_layout(oracle):
                     want.append(z[j % 4])
                 got = oracle.step_normals(seed, path, n, stride)
                 assert np.array_equal(got, np.array(want, np.float32)), (stride, path, n)
+    # the stream-walking fill (each call once, shocks scattered) equals the step form
+    for stride in (50, 20, 7, 4, 1):
+        z = oracle.normals(seed, 3, 2 * stride + 3, path0=11, stride=stride)
+        for n in range(z.shape[1]):
+            assert np.array_equal(z[1, n], oracle.step_normals(seed, 12, n, stride)), (stride, n)
     # one call of a 50-step block feeds four shocks: 38 calls cover its 150 shocks
     calls = {(g, (3 * (s // 4) + k) // 4) for s in range(50) for k in range(3) for g in [s % 4]}
     assert len(calls) == 38
