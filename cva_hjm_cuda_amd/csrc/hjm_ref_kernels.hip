@@ -6,6 +6,8 @@
 // loads, identical for every lane. Normals are generated in-kernel (counter Philox) or
 // read from the caller's [path][step][3] array. HBM is touched only at exposure dates:
 // one coalesced store per numeraire slot, slot-major [slot][path].
+#include <algorithm>
+
 #include "hjm_common.h"
 #include "hjm_rng.h"
 #include "hjm_select.h"
@@ -241,30 +243,43 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(const double* __rest
     }
 }
 
-// The in-kernel generator's normals for [path0, path0 + paths) x [0, nsteps), [path][step][3]:
-// one thread per (path, date block, class) walks its class stream.
+// The in-kernel generator's normals for [path0, path0 + paths) x [0, nsteps), [path][step][3].
+// A workgroup takes `ppw` (path, date block) pairs, four lanes per pair, one class stream
+// per lane (each call once), stages the pairs' shocks in LDS and writes every pair's
+// stride x 3 values as one contiguous run.
 __global__ __launch_bounds__(kBlock) void normals_kernel(uint32_t seed_lo, uint32_t seed_hi,
                                                          int64_t path0, int64_t paths,
-                                                         int nsteps, int stride,
+                                                         int nsteps, int stride, int ppw,
                                                          double* __restrict__ out)
 {
+    extern __shared__ float nbuf[];  // [ppw][stride * 3]
     const int64_t nblk = (nsteps + stride - 1) / stride;
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= paths * nblk * 4) return;
-    const int cls = (int)(t & 3);
-    const int64_t pb = t >> 2;
-    const int64_t p = pb / nblk;
-    const int b = (int)(pb % nblk);
-    StreamCache c;
-    c.reset();
-    for (int q = 0; 4 * q + cls < stride; ++q) {
-        const int n = b * stride + 4 * q + cls;
-        if (n >= nsteps) break;
-        const Normals3 z = c.step(seed_lo, seed_hi, (uint32_t)(path0 + p), (uint32_t)b, (uint32_t)cls, q);
-        double* o = out + (p * nsteps + n) * 3;
-        o[0] = z.z0;
-        o[1] = z.z1;
-        o[2] = z.z2;
+    const int64_t npairs = paths * nblk;
+    const int tid = threadIdx.x, lp = tid >> 2, cls = tid & 3;
+    const int64_t pair = (int64_t)blockIdx.x * ppw + lp;
+    const int row = 3 * stride;
+    if (lp < ppw && pair < npairs) {
+        const int64_t p = pair / nblk;
+        const int b = (int)(pair % nblk);
+        StreamCache c;
+        c.reset();
+        for (int q = 0; 4 * q + cls < stride; ++q) {
+            const Normals3 z = c.step(seed_lo, seed_hi, (uint32_t)(path0 + p), (uint32_t)b,
+                                      (uint32_t)cls, q);
+            float* o = nbuf + lp * row + 3 * (4 * q + cls);
+            o[0] = z.z0;
+            o[1] = z.z1;
+            o[2] = z.z2;
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < ppw * row; idx += blockDim.x) {
+        const int l = idx / row, r = idx % row;
+        const int64_t pr = (int64_t)blockIdx.x * ppw + l;
+        if (pr >= npairs) break;
+        const int64_t p = pr / nblk;
+        const int n = (int)(pr % nblk) * stride + r / 3;
+        if (n < nsteps) out[(p * nsteps + n) * 3 + r % 3] = (double)nbuf[idx];
     }
 }
 
@@ -312,10 +327,13 @@ hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nstep
                           double* out, hipStream_t st)
 {
     if (stride <= 0) return hipErrorInvalidValue;
-    const int64_t total = paths * ((nsteps + stride - 1) / stride) * 4;
-    const dim3 grid((unsigned)((total + kBlock - 1) / kBlock));
-    normals_kernel<<<grid, kBlock, 0, st>>>((uint32_t)seed, (uint32_t)(seed >> 32), path0, paths,
-                                            nsteps, stride, out);
+    // pairs per workgroup: up to 64 (256 lanes), within 64 KB of LDS
+    const int ppw = (int)std::min<int64_t>(kBlock / 4, std::max<int64_t>(1, 65536 / (12 * (int64_t)stride)));
+    if ((int64_t)ppw * 12 * stride > 65536) return hipErrorInvalidValue;
+    const int64_t npairs = paths * ((nsteps + stride - 1) / stride);
+    const dim3 grid((unsigned)((npairs + ppw - 1) / ppw));
+    normals_kernel<<<grid, 4 * ppw, (size_t)ppw * 12 * stride, st>>>(
+        (uint32_t)seed, (uint32_t)(seed >> 32), path0, paths, nsteps, stride, ppw, out);
     return hipGetLastError();
 }
 
