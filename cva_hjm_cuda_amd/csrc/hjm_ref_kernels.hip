@@ -244,42 +244,53 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(const double* __rest
 }
 
 // The in-kernel generator's normals for [path0, path0 + paths) x [0, nsteps), [path][step][3].
-// A workgroup takes `ppw` (path, date block) pairs, four lanes per pair, one class stream
-// per lane (each call once), stages the pairs' shocks in LDS and writes every pair's
-// stride x 3 values as one contiguous run.
+// A workgroup takes `ppw` consecutive (path, date block) pairs; each lane makes one call of
+// one class stream (work item: pair, class, call), scatters the call's four shocks to
+// their (step, factor) slots in LDS, and the workgroup then writes the pairs' values: one
+// contiguous run when the blocks tile the steps exactly, pair by pair otherwise.
 __global__ __launch_bounds__(kBlock) void normals_kernel(uint32_t seed_lo, uint32_t seed_hi,
                                                          int64_t path0, int64_t paths,
                                                          int nsteps, int stride, int ppw,
-                                                         double* __restrict__ out)
+                                                         int maxc, double* __restrict__ out)
 {
     extern __shared__ float nbuf[];  // [ppw][stride * 3]
-    const int64_t nblk = (nsteps + stride - 1) / stride;
+    const int nblk = (nsteps + stride - 1) / stride;
     const int64_t npairs = paths * nblk;
-    const int tid = threadIdx.x, lp = tid >> 2, cls = tid & 3;
-    const int64_t pair = (int64_t)blockIdx.x * ppw + lp;
     const int row = 3 * stride;
-    if (lp < ppw && pair < npairs) {
-        const int64_t p = pair / nblk;
-        const int b = (int)(pair % nblk);
-        StreamCache c;
-        c.reset();
-        for (int q = 0; 4 * q + cls < stride; ++q) {
-            const Normals3 z = c.step(seed_lo, seed_hi, (uint32_t)(path0 + p), (uint32_t)b,
-                                      (uint32_t)cls, q);
-            float* o = nbuf + lp * row + 3 * (4 * q + cls);
-            o[0] = z.z0;
-            o[1] = z.z1;
-            o[2] = z.z2;
+    const int64_t first = (int64_t)blockIdx.x * ppw;
+    const int64_t p0 = first / nblk;
+    const int b0 = (int)(first - p0 * nblk);
+    for (int w = threadIdx.x; w < ppw * 4 * maxc; w += blockDim.x) {
+        const int lp = w / (4 * maxc), g = (w / maxc) & 3, c = w - (w / maxc) * maxc;
+        const int nq = (stride - g + 3) / 4;  // steps of class g in a block
+        if (first + lp < npairs && g < stride && 4 * c < 3 * nq) {
+            const int bb = b0 + lp;
+            const int64_t p = p0 + bb / nblk;
+            const int b = bb % nblk;
+            float z[4];
+            stream_call(seed_lo, seed_hi, (uint32_t)(path0 + p), (uint32_t)b, (uint32_t)g,
+                        (uint32_t)c, z);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int j = 4 * c + e, q = j / 3;
+                if (q < nq) nbuf[lp * row + 3 * (4 * q + g) + (j - 3 * q)] = z[e];
+            }
         }
     }
     __syncthreads();
-    for (int idx = tid; idx < ppw * row; idx += blockDim.x) {
-        const int l = idx / row, r = idx % row;
-        const int64_t pr = (int64_t)blockIdx.x * ppw + l;
-        if (pr >= npairs) break;
-        const int64_t p = pr / nblk;
-        const int n = (int)(pr % nblk) * stride + r / 3;
-        if (n < nsteps) out[(p * nsteps + n) * 3 + r % 3] = (double)nbuf[idx];
+    const int npair = (int)(npairs - first < ppw ? npairs - first : ppw);
+    if (nblk * stride == nsteps) {
+        double* o = out + (p0 * nsteps + (int64_t)b0 * stride) * 3;
+        for (int r = threadIdx.x; r < npair * row; r += blockDim.x) o[r] = (double)nbuf[r];
+    } else {
+        for (int l = 0; l < npair; ++l) {
+            const int bb = b0 + l;
+            const int64_t p = p0 + bb / nblk;
+            const int b = bb % nblk;
+            const int len = 3 * min(stride, nsteps - b * stride);
+            double* o = out + (p * nsteps + (int64_t)b * stride) * 3;
+            for (int r = threadIdx.x; r < len; r += blockDim.x) o[r] = (double)nbuf[l * row + r];
+        }
     }
 }
 
@@ -327,13 +338,15 @@ hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nstep
                           double* out, hipStream_t st)
 {
     if (stride <= 0) return hipErrorInvalidValue;
-    // pairs per workgroup: up to 64 (256 lanes), within 64 KB of LDS
-    const int ppw = (int)std::min<int64_t>(kBlock / 4, std::max<int64_t>(1, 65536 / (12 * (int64_t)stride)));
-    if ((int64_t)ppw * 12 * stride > 65536) return hipErrorInvalidValue;
-    const int64_t npairs = paths * ((nsteps + stride - 1) / stride);
+    // work items: (pair, class, call); pairs per workgroup: as many as fit 256 lanes (one
+    // pair, looped over by the lanes, for strides above 340), within 64 KB of LDS
+    const int maxc = (3 * ((stride + 3) / 4) + 3) / 4;  // calls of the longest class
+    if (12 * (int64_t)stride > 65536) return hipErrorInvalidValue;
+    const int ppw = std::max(1, std::min(kBlock / (4 * maxc), 65536 / (12 * stride)));
+    const int64_t npairs = paths * (int64_t)((nsteps + stride - 1) / stride);
     const dim3 grid((unsigned)((npairs + ppw - 1) / ppw));
-    normals_kernel<<<grid, 4 * ppw, (size_t)ppw * 12 * stride, st>>>(
-        (uint32_t)seed, (uint32_t)(seed >> 32), path0, paths, nsteps, stride, ppw, out);
+    normals_kernel<<<grid, kBlock, (size_t)ppw * 12 * stride, st>>>(
+        (uint32_t)seed, (uint32_t)(seed >> 32), path0, paths, nsteps, stride, ppw, maxc, out);
     return hipGetLastError();
 }
 
