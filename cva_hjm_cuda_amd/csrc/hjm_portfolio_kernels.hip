@@ -543,23 +543,36 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_seed_kernel(const PfValuePa
         const int K = kp[row];
         kconst_ptr_pf<Real> gam = cp + row * KP;
         Real V = ap[row];
+        // chunks of 16 periods: each chunk's scalar loads are issued together ahead of its
+        // FMAs (one wait per 16 FMAs)
 #pragma unroll
-        for (int k0 = 0; k0 < KP; k0 += 4) {
+        for (int k0 = 0; k0 < KP; k0 += 16) {
             if (k0 < K) {
+                Real gk[16];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) V -= gam[k0 + u] * Pk[k0 + u];
+                for (int u = 0; u < 16; ++u)
+                    if (k0 + u < KP) gk[u] = gam[k0 + u];
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    if (k0 + u < KP) V -= gk[u] * Pk[k0 + u];
             }
         }
-        if (V > (Real)0) {
-            const double wg = wp[(int64_t)gset * v.ndates + e];
-            b -= wg * (double)V * D;
-            const Real a = (Real)(wg * D);  // dJ/dV_g; dV_g/dP_k = -gamma_k
+        // dJ/dV_g = w D where the set's exposure is positive (0 elsewhere: no divergent
+        // branch); dV_g/dP_k = -gamma_k
+        const double wg = wp[(int64_t)gset * v.ndates + e];
+        const bool pos = V > (Real)0;
+        b -= pos ? wg * (double)V * D : 0.0;
+        const Real a = pos ? (Real)(wg * D) : (Real)0;
 #pragma unroll
-            for (int k0 = 0; k0 < KP; k0 += 4) {
-                if (k0 < K) {
+        for (int k0 = 0; k0 < KP; k0 += 16) {
+            if (k0 < K) {
+                Real gk[16];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) dP[k0 + u] -= a * gam[k0 + u];
-                }
+                for (int u = 0; u < 16; ++u)
+                    if (k0 + u < KP) gk[u] = gam[k0 + u];
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    if (k0 + u < KP) dP[k0 + u] -= a * gk[u];
             }
         }
     }
