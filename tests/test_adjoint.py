@@ -143,8 +143,9 @@ def test_gpu_fp32_forward_greeks_close_to_fp64():
         assert np.max(np.abs(x - y)) <= 1e-3 * np.max(np.abs(y))
     ratio = b.timings_ms[2] / b.timings_ms[0]
     _flush("  CVA+Greeks / forward device time = %.2f" % ratio)
-    # the FP32 forward runs on the matrix cores, the reverse sweeps on the VALU
-    assert ratio <= 5.0
+    # the FP32 forward runs on the matrix cores, the reverse sweeps on the VALU; measured
+    # 3.5-3.7 on MI355X boxes (target 4), bounded with room for box-to-box spread
+    assert ratio <= 4.5
 
 
 @pytest.mark.gpu
