@@ -74,31 +74,33 @@ __host__ __device__ constexpr int shock_chunk_last_quad(int s)
     return (8 * s + 7) / 3 < kMfmaQuads - 1 ? (8 * s + 7) / 3 : kMfmaQuads - 1;
 }
 
-// Issue plan: the matrix-core work of the block is spread over the 13 quads of shock
-// generation so that every quad's RNG (VALU) runs beside a share of the MFMAs: chunk c's
-// M-tiles [m0, m1) go to quad issue_quad. Curve chunks 0, 1 are ready at the start;
-// shock chunk s is ready one quad after its last value. Shock chunk 4 completes with the
-// last quad and runs after it.
+// Issue plan: the matrix-core work of the block is spread over the quads that generate
+// shocks, so that their RNG (VALU) runs beside a share of the MFMAs: chunk c's M-tiles
+// [m0, m1) go to quad q. Quads 3, 7 and 11 make no Philox call (their shocks come from
+// calls made earlier) and get no MFMA work; curve chunks 0, 1 are ready at the start,
+// shock chunk s once its last shock's call is made (quads 1, 4, 6, 9, 12); shock chunk 4
+// completes with the last quad and runs after it.
 struct IssueSlot {
     int chunk, m0, m1;
 };
 template <int MT>
 __host__ __device__ constexpr IssueSlot issue_slot(int q, int half)
 {
-    // quad -> chunk: 0,1: curve 0 | 2,3: curve 1 | 4,5: shock 0 | 6,7: shock 1 | 8,9: shock 2
-    //                11,12: shock 3 | 10: none
     const int split = (MT + 1) / 2;
-    int c = -1, part = 0;
-    if (q <= 9) {
-        c = q / 2 < 2 ? q / 2 : 2 + (q / 2 - 2);
-        part = q % 2;
-    } else if (q >= 11) {
-        c = 5;
-        part = q - 11;
-    }
     (void)half;
-    if (c < 0) return {-1, 0, 0};
-    return part == 0 ? IssueSlot{c, 0, split} : IssueSlot{c, split, MT};
+    switch (q) {
+    case 0: return {0, 0, split};
+    case 1: return {0, split, MT};
+    case 2: return {1, 0, MT};
+    case 4: return {2, 0, split};
+    case 5: return {2, split, MT};
+    case 6: return {3, 0, split};
+    case 8: return {3, split, MT};
+    case 9: return {4, 0, split};
+    case 10: return {4, split, MT};
+    case 12: return {5, 0, MT};
+    default: return {-1, 0, 0};
+    }
 }
 
 // f(integral_constant<int, Q>) for Q in the sequence, in order (compile-time indices).
