@@ -230,42 +230,4 @@ __device__ __forceinline__ Normals3 group_step(const float* col, int ld, int s)
     return Normals3{col[r * ld], col[(r + 1) * ld], col[(r + 2) * ld]};
 }
 
-// Walks the steps of one class through a date block in either direction, holding the
-// stream's last call so that each call is made once (the walk visits q, q + 1, ... or
-// q, q - 1, ...; at most one call per step is new). Reset at every block.
-struct StreamCache {
-    float z[4];
-    int held;  // call index held, -1 none
-    HJM_HD void reset() { held = -1; }
-    HJM_HD float pick(const float (&v)[4], int e) const
-    {
-        return e == 0 ? v[0] : e == 1 ? v[1] : e == 2 ? v[2] : v[3];
-    }
-    // shocks of the q-th step of class cls in block b
-    HJM_HD Normals3 step(uint32_t seed_lo, uint32_t seed_hi, uint32_t path, uint32_t b,
-                         uint32_t cls, int q)
-    {
-        const int j0 = 3 * q, c0 = j0 >> 2, c1 = (j0 + 2) >> 2;
-        if (held != c0 && held != c1) {
-            stream_call(seed_lo, seed_hi, path, b, cls, (uint32_t)c0, z);
-            held = c0;
-        }
-        const int other = held == c0 ? c1 : c0;
-        float w[4] = {z[0], z[1], z[2], z[3]};
-        if (other != held) stream_call(seed_lo, seed_hi, path, b, cls, (uint32_t)other, w);
-        float r[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const int j = j0 + k;
-            r[k] = (j >> 2) == held ? pick(z, j & 3) : pick(w, j & 3);
-        }
-        if (other != held) {
-#pragma unroll
-            for (int u = 0; u < 4; ++u) z[u] = w[u];
-            held = other;
-        }
-        return Normals3{r[0], r[1], r[2]};
-    }
-};
-
 }  // namespace hjm
