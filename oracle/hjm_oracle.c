@@ -18,14 +18,17 @@
 /* ===================================================================================
  * Normal generator specification.
  *
- * Step n of Monte Carlo path p draws its three factor shocks from ONE Philox4x32-10
- * block (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
- * SC'11) with counter (n, p, 0, 0) and key (seed mod 2^32, seed >> 32):
+ * Class streams (Philox4x32-10: Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+ * easy as 1, 2, 3", SC'11). A run's steps form date blocks of S = dtau/dt steps. Step n
+ * of path p lies in block b = n / S at offset s = n % S; its class is g = s % 4 and it is
+ * the q = s / 4-th step of class g in the block. Its factor k is shock j = 3q + k of the
+ * stream (p, b, g). Call c of that stream is one Philox block with counter (b, p, g, c)
+ * and key (seed mod 2^32, seed >> 32), giving shocks 4c .. 4c + 3:
  *   x0..x3 = philox(ctr, key)
- *   u  = ((x0 >> 8) + 1) * 2^-24        in (0, 1]
- *   k  =  x1 >> 8                        angle 2*pi*k/2^24
- *   z0 = sqrt(-2 ln u) cos(2 pi k 2^-24), z1 = sqrt(-2 ln u) sin(2 pi k 2^-24)
- *   z2 = sqrt(-2 ln u') cos(2 pi k' 2^-24) with u', k' from x2, x3
+ *   (4c, 4c + 1)     = r cos, r sin of (x0, x1)
+ *   (4c + 2, 4c + 3) = r cos, r sin of (x2, x3)
+ * with, for a pair (a, b): u = ((a >> 8) + 1) * 2^-24 in (0, 1], r = sqrt(-2 ln u), angle
+ * 2 pi (b >> 8) 2^-24.
  * ln, sin and cos are the fixed FP32 polynomials below (coefficients fitted by
  * cva_hjm_cuda_amd/csrc/rng_poly_fit.py), evaluated with explicit fmaf so the GPU
  * reproduces them bit for bit. The reference draws cuRAND XORWOW normals instead
