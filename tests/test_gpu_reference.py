@@ -249,3 +249,27 @@ def test_select_kth_exact_on_hard_rows():
         hjm.select_kth(-X[:1, :10], 3)
     with pytest.raises(hjm.HJMError):
         hjm.select_kth(X[:1, :10], 11)
+
+
+def test_in_kernel_normals_at_another_stride(oracle):
+    """dt = 0.02, 25 steps per date: each block draws one whole 16-step group of class-stream
+    shocks through LDS and 9 steps directly. FP64 (one path per lane) and the FP32 VALU
+    kernel against the oracle on its own class-stream normals for that stride, and the
+    adjoint's regenerated shocks against the same normals supplied from the host."""
+    _flush("[gpu] in-kernel normals at stride 25 (dt = 0.02)")
+    m = hjm.benchmark_market()
+    swap = hjm.InterestRateSwap(1e6, 0.025, 5.0)
+    P, S = 512, 250
+    z = oracle.normals(3, P, S, stride=25)
+    nx, ny = oracle.reference_paths(m.spot_rates, m.drifts, m.volatilities, 5.0, 0.5, 0.02, z)
+    ee = oracle.expected_exposure(oracle.reference_exposure(nx, ny, m.accrual, 1e6, 0.025, 0.5))
+    pos = ee > 0
+    r = hjm.simulate_reference(m, swap, P, dt=0.02, seed=3, precision=hjm.FP64)
+    assert np.max(np.abs(r.expected_exposure[pos] / ee[pos] - 1)) < 1e-10
+    r32 = hjm.simulate_reference(m, swap, P, dt=0.02, seed=3, precision=hjm.FP32_KAHAN)
+    assert np.max(np.abs(r32.expected_exposure[pos] / ee[pos] - 1)) < 1e-4
+    a = hjm.cva_greeks(m, swap, P, dt=0.02, seed=3, precision=hjm.FP64)
+    b = hjm.cva_greeks(m, swap, P, dt=0.02, normals=z, precision=hjm.FP64)
+    assert a.cva == pytest.approx(b.cva, rel=1e-12)
+    for x, y in ((a.d_spot, b.d_spot), (a.d_vol, b.d_vol)):
+        assert np.max(np.abs(x - y)) <= 1e-10 * np.max(np.abs(y))
