@@ -244,53 +244,59 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(const double* __rest
 }
 
 // The in-kernel generator's normals for [path0, path0 + paths) x [0, nsteps), [path][step][3].
-// A workgroup takes `ppw` consecutive (path, date block) pairs; each lane makes one call of
-// one class stream (work item: pair, class, call), scatters the call's four shocks to
-// their (step, factor) slots in LDS, and the workgroup then writes the pairs' values: one
-// contiguous run when the blocks tile the steps exactly, pair by pair otherwise.
+// A workgroup takes np <= 64 paths of one date block (lane l < np of every wave is path l;
+// np = 64 unless the paths' LDS rows would exceed 64 KB):
+//   * the block's whole 16-step groups: 12 calls per path and group (calls 3 m .. 3 m + 2
+//     of the four classes), the wave-uniform work items (group, class, call) spread over
+//     the four waves;
+//   * the steps past the last whole group (2 of 50): one work item per step, computed
+//     directly;
+// each into the path's LDS row; then every path's stride x 3 values go out as one
+// contiguous run (coalesced across the lanes).
 __global__ __launch_bounds__(kBlock) void normals_kernel(uint32_t seed_lo, uint32_t seed_hi,
                                                          int64_t path0, int64_t paths,
-                                                         int nsteps, int stride, int ppw,
-                                                         int maxc, double* __restrict__ out)
+                                                         int nsteps, int stride, int np,
+                                                         double* __restrict__ out)
 {
-    extern __shared__ float nbuf[];  // [ppw][stride * 3]
+    extern __shared__ float nbuf[];  // [np][stride * 3]
     const int nblk = (nsteps + stride - 1) / stride;
-    const int64_t npairs = paths * nblk;
+    const int b = (int)(blockIdx.x % nblk);
+    const int64_t pbase = (int64_t)(blockIdx.x / nblk) * np;
+    const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave, nwv = blockDim.x / kWave;
     const int row = 3 * stride;
-    const int64_t first = (int64_t)blockIdx.x * ppw;
-    const int64_t p0 = first / nblk;
-    const int b0 = (int)(first - p0 * nblk);
-    for (int w = threadIdx.x; w < ppw * 4 * maxc; w += blockDim.x) {
-        const int lp = w / (4 * maxc), g = (w / maxc) & 3, c = w - (w / maxc) * maxc;
-        const int nq = (stride - g + 3) / 4;  // steps of class g in a block
-        if (first + lp < npairs && g < stride && 4 * c < 3 * nq) {
-            const int bb = b0 + lp;
-            const int64_t p = p0 + bb / nblk;
-            const int b = bb % nblk;
-            float z[4];
-            stream_call(seed_lo, seed_hi, (uint32_t)(path0 + p), (uint32_t)b, (uint32_t)g,
-                        (uint32_t)c, z);
+    const int64_t p = pbase + lane;
+    const uint32_t pid = (uint32_t)(path0 + (p < paths ? p : 0));
+    // lanes past np compute like the others but store nothing
+    float* mine = nbuf + (lane < np ? lane : 0) * row;
+    const bool owner = lane < np;
+    const int ngroups = stride / kGroupSteps, whole = ngroups * kGroupSteps;
+    // whole groups: item = (group m, class g, call cc), uniform across the wave
+    for (int it = wv; it < ngroups * 12; it += nwv) {
+        const int m = it / 12, g = (it / 3) % 4, cc = it % 3;
+        float z[4];
+        stream_call(seed_lo, seed_hi, pid, (uint32_t)b, (uint32_t)g, (uint32_t)(3 * m + cc), z);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int j = 4 * c + e, q = j / 3;
-                if (q < nq) nbuf[lp * row + 3 * (4 * q + g) + (j - 3 * q)] = z[e];
-            }
+        for (int e = 0; e < 4; ++e) {
+            const int jj = 4 * cc + e;  // shock of class g within the group
+            const int s = kGroupSteps * m + 4 * (jj / 3) + g;
+            if (owner) mine[3 * s + jj % 3] = z[e];
+        }
+    }
+    // the steps past the last whole group
+    for (int s = whole + wv; s < stride; s += nwv) {
+        const Normals3 z = step_normals(seed_lo, seed_hi, pid, (uint32_t)(b * stride + s), (uint32_t)stride);
+        if (owner) {
+            mine[3 * s + 0] = z.z0;
+            mine[3 * s + 1] = z.z1;
+            mine[3 * s + 2] = z.z2;
         }
     }
     __syncthreads();
-    const int npair = (int)(npairs - first < ppw ? npairs - first : ppw);
-    if (nblk * stride == nsteps) {
-        double* o = out + (p0 * nsteps + (int64_t)b0 * stride) * 3;
-        for (int r = threadIdx.x; r < npair * row; r += blockDim.x) o[r] = (double)nbuf[r];
-    } else {
-        for (int l = 0; l < npair; ++l) {
-            const int bb = b0 + l;
-            const int64_t p = p0 + bb / nblk;
-            const int b = bb % nblk;
-            const int len = 3 * min(stride, nsteps - b * stride);
-            double* o = out + (p * nsteps + (int64_t)b * stride) * 3;
-            for (int r = threadIdx.x; r < len; r += blockDim.x) o[r] = (double)nbuf[l * row + r];
-        }
+    const int len = 3 * min(stride, nsteps - b * stride);
+    for (int l = 0; l < np; ++l) {
+        if (pbase + l >= paths) break;
+        double* o = out + ((pbase + l) * nsteps + (int64_t)b * stride) * 3;
+        for (int r = threadIdx.x; r < len; r += blockDim.x) o[r] = (double)nbuf[l * row + r];
     }
 }
 
@@ -337,16 +343,14 @@ hipError_t launch_column_sum(const double* partials, int64_t rows, int cols, dou
 hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nsteps, int stride,
                           double* out, hipStream_t st)
 {
-    if (stride <= 0) return hipErrorInvalidValue;
-    // work items: (pair, class, call); pairs per workgroup: as many as fit 256 lanes (one
-    // pair, looped over by the lanes, for strides above 340), within 64 KB of LDS
-    const int maxc = (3 * ((stride + 3) / 4) + 3) / 4;  // calls of the longest class
-    if (12 * (int64_t)stride > 65536) return hipErrorInvalidValue;
-    const int ppw = std::max(1, std::min(kBlock / (4 * maxc), 65536 / (12 * stride)));
-    const int64_t npairs = paths * (int64_t)((nsteps + stride - 1) / stride);
-    const dim3 grid((unsigned)((npairs + ppw - 1) / ppw));
-    normals_kernel<<<grid, kBlock, (size_t)ppw * 12 * stride, st>>>(
-        (uint32_t)seed, (uint32_t)(seed >> 32), path0, paths, nsteps, stride, ppw, maxc, out);
+    // LDS: np paths x stride x 3 floats, within 64 KB
+    if (stride <= 0 || 12 * (int64_t)stride > 65536) return hipErrorInvalidValue;
+    const int np = (int)std::min<int64_t>(kWave, 65536 / (12 * (int64_t)stride));
+    const int64_t nblk = (nsteps + stride - 1) / stride;
+    const int64_t groups = (paths + np - 1) / np;
+    const dim3 grid((unsigned)(groups * nblk));
+    normals_kernel<<<grid, kBlock, (size_t)np * 12 * stride, st>>>(
+        (uint32_t)seed, (uint32_t)(seed >> 32), path0, paths, nsteps, stride, np, out);
     return hipGetLastError();
 }
 
