@@ -147,6 +147,7 @@ __global__ __launch_bounds__(kBlock) void pf_value_kernel(const PfValueParams v,
 #pragma unroll
     for (int j = 0; j < kTenors; ++j) f[j] = curves[((int64_t)e * kTenors + j) * P + ic];
     const double disc = exp(-logb[(int64_t)e * P + ic]);
+    const Real discr = (Real)disc;  // the discount factor in the exposure's precision
     Real Pk[KP];
     Real I = (Real)0;
     const Real hw = (Real)(0.5 * v.dtau);
@@ -185,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void pf_value_kernel(const PfValueParams v,
                         if (k0 + u < KP) V -= gk[u] * Pk[k0 + u];
                 }
             }
-            const Real ex = V > (Real)0 ? (Real)((double)V * disc) : (Real)0;
+            const Real ex = V > (Real)0 ? V * discr : (Real)0;
             if (live) exposures[(base + gg) * P + i] = ex;
         }
     }
