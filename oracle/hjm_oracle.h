@@ -1,7 +1,7 @@
 /*
  * hjm_oracle.h -- CPU oracle for the HJM path-generation + swap-exposure hot path.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker: it is loaded by tests/,
+ * TEST INFRASTRUCTURE ONLY. This library is the oracle: it is loaded by tests/,
  * by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, never by the product
  * library (cva_hjm_cuda_amd/), which must run on the GPU or fail.
  *
