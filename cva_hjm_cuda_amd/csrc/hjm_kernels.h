@@ -58,8 +58,10 @@ hipError_t launch_pf_value_f32(const PfValueParams& v, const float* curves, cons
                                hipStream_t st);
 // selws (or null): bracket-selection workspace whose counting this pass does, for rank
 // pfe_rank of each (date, set) row, after launch_bracket_sample*; needs
-// pf_stats_stage(...) > 0 (LDS staging slice per wave, in entries)
-int pf_stats_stage(int64_t paths, int64_t pfe_rank, int ndates, int elem_bytes);
+// pf_stats_stage(...) > 0 (LDS staging slice per wave, in entries; *flush_dates, if given,
+// receives the number of dates between the slice's flushes)
+int pf_stats_stage(int64_t paths, int64_t pfe_rank, int ndates, int elem_bytes,
+                   int* flush_dates = nullptr);
 hipError_t launch_pf_stats_f64(const PfValueParams& v, const double* ex, const double* w,
                                double* partials, double* sums, void* selws, int64_t pfe_rank,
                                hipStream_t st);

@@ -199,12 +199,12 @@ __global__ __launch_bounds__(kBlock) void pf_value_kernel(const PfValueParams v,
 // rows (date, set) it reads (hjm_select.h): counts below the bracket / zeros, and the
 // entries inside it. The counts are wave ballots. Each wave stages its inside entries in
 // its own slice of dynamic LDS (stage entries, in date order, at a wave-uniform offset);
-// after the last date one atomic per row claims the workgroup's candidate slots and the
-// waves copy their slices out. A slice that fills up flags the rows it could not hold
-// (kSelOverflow in the row's claim), whose selection then runs over the whole row.
+// every `flush` dates (and after the last) one atomic per row claims the workgroup's
+// candidate slots of those dates, the waves copy their slices out and start them again.
+// A slice that fills up flags the rows it could not hold (kSelOverflow in the row's
+// claim), whose selection then runs over the whole row.
 constexpr int kStatPerThread = 8;
 constexpr int kStatPaths = kBlock * kStatPerThread;
-constexpr uint32_t kSelOverflow = 0x80000000u;
 
 template <typename Real, bool kCount>
 __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
                                                           double* __restrict__ partials,
                                                           SelRowState* __restrict__ sel,
                                                           Real* __restrict__ cand, int64_t cap,
-                                                          int stage)
+                                                          int stage, int flush)
 {
     typedef typename SelectBits<Real>::U U;
     constexpr int kWaves = kBlock / kWave;
@@ -263,6 +263,36 @@ __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
             }
         }
     };
+    // claims the candidate slots of dates [e0, e1) and copies the waves' slices out (every
+    // wave calls it at the same dates)
+    auto flush_slices = [&](int e0, int e1) {
+        __syncthreads();
+        for (int e = e0 + threadIdx.x; e < e1; e += kBlock) {
+            const int64_t rid = (int64_t)e * v.nsets + gset;
+            uint32_t nlo = 0, nin = 0, nst = 0;
+#pragma unroll
+            for (int ww = 0; ww < kWaves; ++ww) {
+                nlo += cnt[0][ww][e];
+                nin += cnt[1][ww][e];
+                nst += beg[ww][e] != ~0u ? cnt[1][ww][e] : 0u;
+            }
+            if (nlo) atomicAdd(s_lo[e] ? &sel[rid].below : &sel[rid].zin, nlo);
+            if (nin) atomicAdd(&sel[rid].nin, nin);
+            base[e] = nst ? atomicAdd(&sel[rid].claim, nst) : 0u;
+        }
+        __syncthreads();
+        for (int e = e0; e < e1; ++e) {
+            const uint32_t b0 = beg[wave][e];
+            if (b0 == ~0u) continue;
+            uint32_t off = base[e];
+            for (int ww = 0; ww < wave; ++ww) off += beg[ww][e] != ~0u ? cnt[1][ww][e] : 0u;
+            const uint32_t n = cnt[1][wave][e];
+            Real* out = cand + ((int64_t)e * v.nsets + gset) * cap;
+            for (uint32_t i = lane; i < n; i += kWave)
+                if ((int64_t)off + i < cap) out[off + i] = __builtin_bit_cast(Real, slice[b0 + i]);
+        }
+    };
+    int e_flushed = 0;
     load(0, xn);
     for (int e = 0; e < nd; ++e) {
         const int64_t rid = (int64_t)e * v.nsets + gset;
@@ -318,6 +348,11 @@ __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
                 beg[wave][e] = staged ? used : ~0u;
             }
             used += staged;
+            if (e + 1 == nd || e + 1 - e_flushed == flush) {
+                flush_slices(e_flushed, e + 1);
+                e_flushed = e + 1;
+                used = 0;
+            }
         }
     }
     double c1 = 0.0, c2 = 0.0;
@@ -339,32 +374,6 @@ __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
 #pragma unroll
         for (int ww = 0; ww < kWaves; ++ww) t += red[ww][c];
         partials[((int64_t)gset * gridDim.x + blockIdx.x) * ncol + c] = t;
-    }
-    if constexpr (kCount) {
-        for (int e = threadIdx.x; e < nd; e += kBlock) {
-            const int64_t rid = (int64_t)e * v.nsets + gset;
-            uint32_t nlo = 0, nin = 0, nst = 0;
-#pragma unroll
-            for (int ww = 0; ww < kWaves; ++ww) {
-                nlo += cnt[0][ww][e];
-                nin += cnt[1][ww][e];
-                nst += beg[ww][e] != ~0u ? cnt[1][ww][e] : 0u;
-            }
-            if (nlo) atomicAdd(s_lo[e] ? &sel[rid].below : &sel[rid].zin, nlo);
-            if (nin) atomicAdd(&sel[rid].nin, nin);
-            base[e] = nst ? atomicAdd(&sel[rid].claim, nst) : 0u;
-        }
-        __syncthreads();
-        for (int e = 0; e < nd; ++e) {
-            const uint32_t b0 = beg[wave][e];
-            if (b0 == ~0u) continue;
-            uint32_t off = base[e];
-            for (int ww = 0; ww < wave; ++ww) off += beg[ww][e] != ~0u ? cnt[1][ww][e] : 0u;
-            const uint32_t n = cnt[1][wave][e];
-            Real* out = cand + ((int64_t)e * v.nsets + gset) * cap;
-            for (uint32_t i = lane; i < n; i += kWave)
-                if ((int64_t)off + i < cap) out[off + i] = __builtin_bit_cast(Real, slice[b0 + i]);
-        }
     }
 }
 
@@ -440,16 +449,23 @@ hipError_t launch_pf_value_f32(const PfValueParams& v, const float* curves, cons
     return pf_value_launch<float>(v, curves, logb, coef, a, kmax, ex, st);
 }
 
-int pf_stats_stage(int64_t paths, int64_t pfe_rank, int ndates, int elem_bytes)
+int pf_stats_stage(int64_t paths, int64_t pfe_rank, int ndates, int elem_bytes, int* flush_dates)
 {
     // per-wave staging slice: twice the expected inside count of a wave's 512 paths over
-    // all dates, within 32 KB of LDS per workgroup; 0 = too wide to fuse
+    // the dates between flushes, + 256, within 32 KB of LDS per workgroup; as many dates
+    // between flushes as fit; 0 = not even one date fits
     const BracketPlan bp = bracket_plan(paths, pfe_rank, elem_bytes);
-    if (!bp.use) return 0;
+    if (!bp.use || ndates < 1) return 0;
     const double frac = (double)(bp.khi - bp.klo + 1) / bp.m;
-    const double want = 2.0 * frac * kWave * kStatPerThread * ndates + 256.0;
+    const double per_date = 2.0 * frac * kWave * kStatPerThread;
     const int limit = 32 * 1024 / ((kBlock / kWave) * elem_bytes);
-    return want > limit ? 0 : ((int)want + 63) / 64 * 64;
+    int f = ndates;
+    if (per_date * f + 256.0 > limit) f = (int)((limit - 256.0) / per_date);
+    if (f < 1) return 0;
+    if (flush_dates) *flush_dates = f;
+    const int want = (int)(per_date * f + 256.0);
+    const int stage = (want + 63) / 64 * 64;
+    return stage < limit ? stage : limit;
 }
 
 template <typename Real>
@@ -461,17 +477,18 @@ static hipError_t pf_stats_launch(const PfValueParams& v, const Real* ex, const 
     const dim3 grid(nb, (unsigned)v.nsets);
     if (selws) {
         const BracketPlan bp = bracket_plan(v.paths, pfe_rank, (int)sizeof(Real));
-        const int stage = pf_stats_stage(v.paths, pfe_rank, v.ndates, (int)sizeof(Real));
+        int flush = 0;
+        const int stage = pf_stats_stage(v.paths, pfe_rank, v.ndates, (int)sizeof(Real), &flush);
         if (!bp.use || !stage) return hipErrorInvalidValue;
         SelRowState* state;
         Real* cand;
         bracket_ws_split<Real>(selws, v.ndates * v.nsets, &state, &cand);
         const size_t lds = (size_t)(kBlock / kWave) * stage * sizeof(Real);
         pf_stats_kernel<Real, true><<<grid, kBlock, lds, st>>>(v, ex, w, partials, state, cand, bp.cap,
-                                                              stage);
+                                                              stage, flush);
     } else {
         pf_stats_kernel<Real, false><<<grid, kBlock, 0, st>>>(v, ex, w, partials, nullptr, nullptr, 0,
-                                                             0);
+                                                             0, 0);
     }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;

@@ -363,9 +363,18 @@ __global__ __launch_bounds__(kSelBlock) void bracket_sample_kernel(const T* __re
 }
 
 // One pass over a chunk of a row: counts below lo / zeros, and stages the non-zero entries
-// inside [lo, hi] in LDS. Each wave owns a slice of the LDS buffer as large as its share of
-// the chunk, so the staging needs no atomics: the slot of a lane is the wave's running
-// count plus the lanes below it in the ballot.
+// inside [lo, hi] in LDS. Each wave reads its own kCountSlice entries of the chunk, all of
+// its loads issued before the first is counted, and stages its inside entries in its own
+// kCountStage-entry slice of LDS, so the staging needs no atomics: the slot of a lane is the
+// wave's running count plus the lanes below it in the ballot. The slice holds several times
+// the expected inside count of a wave's share at any quantile (the bracket covers at most
+// 12 sample standard deviations + 4 ranks, under 7 % of the sample); a wave whose inside
+// entries do not fit flags the row (kSelOverflow in its claim), whose selection then runs
+// over the whole row.
+constexpr uint32_t kSelOverflow = 0x80000000u;
+constexpr int kCountSlice = kCountChunk / (kCountBlock / kWave);  // entries read per wave
+constexpr int kCountStage = 512;                                  // entries staged per wave
+
 template <typename T>
 __global__ __launch_bounds__(kCountBlock) void bracket_count_kernel(const T* __restrict__ x, int64_t n,
                                                                     int chunks,
@@ -374,9 +383,8 @@ __global__ __launch_bounds__(kCountBlock) void bracket_count_kernel(const T* __r
 {
     typedef typename SelectBits<T>::U U;
     constexpr int kWaves = kCountBlock / kWave;
-    constexpr int kSlice = kCountChunk / kWaves;
-    __shared__ U buf[kCountChunk];
-    __shared__ uint32_t s_cnt[2][kWaves];
+    __shared__ U buf[kWaves * kCountStage];
+    __shared__ uint32_t s_cnt[3][kWaves];
     __shared__ uint32_t s_base;
     const int row = blockIdx.x / chunks, ch = blockIdx.x % chunks;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
@@ -389,33 +397,39 @@ __global__ __launch_bounds__(kCountBlock) void bracket_count_kernel(const T* __r
     const bool collect = any_in && lo != hi;
     const T* r = x + (int64_t)row * n + beg;
     const uint64_t below_me = lane ? (~0ull >> (64 - lane)) : 0ull;
-    U* slice = buf + wv * kSlice;
+    U* slice = buf + wv * kCountStage;
     uint32_t cnt_lo = 0;  // entries < lo1: below lo (lo > 0), or the zeros (lo = 0)
     uint32_t nin = 0;     // wave-uniform
+    bool ovf = false;     // wave-uniform: an inside entry was not staged
     auto visit = [&](U b, bool valid) {
         cnt_lo += (valid && b < lo1) ? 1u : 0u;
         const bool in = valid && any_in && (U)(b - lo1) <= span;
         const uint64_t mask = __ballot(in);
-        if (collect && in) slice[nin + (uint32_t)__popcll(mask & below_me)] = b;
-        nin += (uint32_t)__popcll(mask);
+        const uint32_t c = (uint32_t)__popcll(mask);
+        if (collect && c) {
+            if (!ovf && nin + c <= (uint32_t)kCountStage) {
+                if (in) slice[nin + (uint32_t)__popcll(mask & below_me)] = b;
+            } else {
+                ovf = true;
+            }
+        }
+        nin += c;
     };
     constexpr int V = 16 / (int)sizeof(T);
+    constexpr int NV = kCountSlice / (V * kWave);  // vectors per lane of a whole slice
     typedef U Vec __attribute__((ext_vector_type(V)));
-    // each wave visits its own kSlice entries of the chunk
-    const int wbeg = wv * kSlice;
-    const int wlen = len - wbeg < 0 ? 0 : (len - wbeg < kSlice ? len - wbeg : kSlice);
+    const int wbeg = wv * kCountSlice;
+    const int wlen = len - wbeg < 0 ? 0 : (len - wbeg < kCountSlice ? len - wbeg : kCountSlice);
     const U* wr = reinterpret_cast<const U*>(r) + wbeg;
-    if ((((uintptr_t)wr) & 15) == 0 && (wlen % V) == 0) {
+    if ((((uintptr_t)wr) & 15) == 0 && wlen == kCountSlice) {
         const Vec* rv = reinterpret_cast<const Vec*>(wr);
-        const int nv = wlen / V;
-        for (int i0 = 0; i0 < nv; i0 += kWave) {
-            const int i = i0 + lane;
-            const bool valid = i < nv;
-            Vec v = (Vec)(U)0;
-            if (valid) v = __builtin_nontemporal_load(rv + i);
+        Vec v[NV];
 #pragma unroll
-            for (int j = 0; j < V; ++j) visit(v[j], valid);
-        }
+        for (int j = 0; j < NV; ++j) v[j] = __builtin_nontemporal_load(rv + j * kWave + lane);
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+#pragma unroll
+            for (int e = 0; e < V; ++e) visit(v[j][e], true);
     } else {
         for (int i0 = 0; i0 < wlen; i0 += kWave) {
             const int i = i0 + lane;
@@ -427,8 +441,12 @@ __global__ __launch_bounds__(kCountBlock) void bracket_count_kernel(const T* __r
     if (lane == 0) {
         s_cnt[0][wv] = cnt_lo;
         s_cnt[1][wv] = nin;
+        s_cnt[2][wv] = ovf ? 1u : 0u;
     }
     __syncthreads();
+    bool any_ovf = false;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) any_ovf |= s_cnt[2][w] != 0u;
     if (tid == 0) {
         uint32_t b = 0, c = 0;
         for (int w = 0; w < kWaves; ++w) {
@@ -437,10 +455,11 @@ __global__ __launch_bounds__(kCountBlock) void bracket_count_kernel(const T* __r
         }
         if (b) atomicAdd(lo ? &state[row].below : &state[row].zin, b);
         if (c) atomicAdd(&state[row].nin, c);
-        s_base = (c && collect) ? atomicAdd(&state[row].claim, c) : 0u;
+        if (collect && any_ovf) atomicOr(&state[row].claim, kSelOverflow);
+        s_base = (c && collect && !any_ovf) ? atomicAdd(&state[row].claim, c) : 0u;
     }
     __syncthreads();
-    if (collect) {
+    if (collect && !any_ovf) {
         uint32_t off = s_base;
         for (int w = 0; w < wv; ++w) off += s_cnt[1][w];
         const uint32_t cnt = s_cnt[1][wv];
@@ -472,7 +491,7 @@ __global__ __launch_bounds__(kSelBlock) void bracket_finish_kernel(const T* __re
             if (threadIdx.x == 0) out[row] = (double)__builtin_bit_cast(T, (U)s.lo);
             return;
         }
-        if (nin <= cap && !(s.claim & 0x80000000u)) {
+        if (nin <= cap && !(s.claim & kSelOverflow)) {
             // (the top bit of claim: a counting pass could not stage every candidate)
             select_kth_row<T>(cand + (int64_t)row * cap, nin, rk, out + row, true);
             return;

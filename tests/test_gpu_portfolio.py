@@ -67,7 +67,7 @@ def test_portfolio_fp64_matches_oracle(small, oracle):
 
 
 def test_portfolio_fp32_pfe_exact_on_long_rows():
-    """PFE over 132072 FP32 / 65536 FP64 paths (the sampled-bracket selection, counted in
+    """PFE over 132072 FP32 / 66000 FP64 paths (the sampled-bracket selection, counted in
     the statistics pass): the exact order statistic of the plan's own exposures, for every
     set and date."""
     _flush("[gpu] portfolio FP32 PFE, 132072 paths x 30y, 16 swaps, 4 sets")
@@ -80,16 +80,17 @@ def test_portfolio_fp32_pfe_exact_on_long_rows():
     want = np.sort(r.exposures.astype(np.float32), axis=0)[k - 1].astype(np.float64)
     assert np.array_equal(r.pfe, want)
     assert np.all(r.pfe[:, 1] > 0)
-    # the median: a bracket too wide for the statistics pass's LDS staging, so the
-    # selection's own counting pass runs
+    # the median: a bracket too wide to stage all 61 dates in the statistics pass's LDS,
+    # so the staging slices are flushed every 36 dates
     r = hjm.simulate_portfolio(m, pf, P, horizon=30.0, precision=hjm.FP32_KAHAN, seed=11,
                                keep_paths=True, pfe_quantile=0.5)
     k = math.ceil(0.5 * P)
     want = np.sort(r.exposures.astype(np.float32), axis=0)[k - 1].astype(np.float64)
     assert np.array_equal(r.pfe, want)
-    # FP64 rows (sample of 8192 paths): 10y horizon (21 dates, counted in the statistics
-    # pass) and 30y at the 90% quantile (the selection's own counting pass)
-    P = 65536
+    # FP64 rows (sample of 8192 paths, a partial last statistics block), counted in the
+    # statistics pass: 10y horizon (21 dates, staged whole) and 30y at the 90% quantile
+    # (staging slices flushed every 17 dates)
+    P = 66000
     pf10 = Portfolio(np.minimum(pf.periods, 20), pf.sign_notional, pf.fixed_rate, pf.set_of,
                      pf.hazard)
     for book, horizon, q in ((pf10, 10.0, 0.975), (pf, 30.0, 0.9)):
