@@ -47,6 +47,9 @@ struct hjm_pplan_s {
     void* d_a = nullptr;
     int* d_kmax = nullptr;
     double* d_w = nullptr;
+    uint16_t* d_vfrag = nullptr;  // FP32: bf16 parts of -gamma (matrix-core valuation)
+    float* d_apad = nullptr;
+    int* d_tilek = nullptr;
 };
 
 namespace {
@@ -68,6 +71,9 @@ void pplan_release(hjm_portfolio_plan p)
     dev_free(p->d_pfe);
     dev_free(p->d_selws);
     dev_free(p->d_coef);
+    dev_free(p->d_vfrag);
+    dev_free(p->d_apad);
+    dev_free(p->d_tilek);
     dev_free(p->d_a);
     dev_free(p->d_kmax);
     dev_free(p->d_w);
@@ -242,6 +248,19 @@ int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfol
                 HJM_ERR_DEVICE);
         HJM_HIP(hipMemcpy(aa, af.data(), af.size() * sizeof(float), hipMemcpyHostToDevice),
                 HJM_ERR_DEVICE);
+        std::vector<uint16_t> vf;
+        std::vector<float> ap;
+        std::vector<int> tk;
+        build_value_fragments(coef, a, kmax, nd, pf->nsets, kMaxPeriods, vf, ap, tk);
+        HJM_TRY(dev_alloc(&p->d_vfrag, vf.size()));
+        HJM_TRY(dev_alloc(&p->d_apad, ap.size()));
+        HJM_TRY(dev_alloc(&p->d_tilek, tk.size()));
+        HJM_HIP(hipMemcpy(p->d_vfrag, vf.data(), vf.size() * sizeof(uint16_t), hipMemcpyHostToDevice),
+                HJM_ERR_DEVICE);
+        HJM_HIP(hipMemcpy(p->d_apad, ap.data(), ap.size() * sizeof(float), hipMemcpyHostToDevice),
+                HJM_ERR_DEVICE);
+        HJM_HIP(hipMemcpy(p->d_tilek, tk.data(), tk.size() * sizeof(int), hipMemcpyHostToDevice),
+                HJM_ERR_DEVICE);
     }
     if (cfg->rng == HJM_RNG_HOST) {
         const size_t n = P * (size_t)S * 3;
@@ -289,9 +308,8 @@ int pplan_enqueue(hjm_portfolio_plan p, bool events = true, float* zs = nullptr)
     err = f64 ? launch_pf_value_f64(p->vp, (const double*)p->d_curves, p->d_logb,
                                     (const double*)p->d_coef, (const double*)p->d_a, p->d_kmax,
                                     (double*)p->d_ex, st)
-              : launch_pf_value_f32(p->vp, (const float*)p->d_curves, p->d_logb,
-                                    (const float*)p->d_coef, (const float*)p->d_a, p->d_kmax,
-                                    (float*)p->d_ex, st);
+              : launch_pf_value_mfma(p->vp, (const float*)p->d_curves, p->d_logb, p->d_vfrag,
+                                     p->d_apad, p->d_tilek, (float*)p->d_ex, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
     HJM_HIP(mark(2), HJM_ERR_DEVICE);
     // With a bracket-selection workspace the statistics pass also does the PFE's counting

@@ -53,9 +53,10 @@ hipError_t launch_pf_paths_f32(const RefStepConsts<float>& k, const PfGrid& g, c
 hipError_t launch_pf_value_f64(const PfValueParams& v, const double* curves, const double* logb,
                                const double* coef, const double* a, const int* kmax, double* ex,
                                hipStream_t st);
-hipError_t launch_pf_value_f32(const PfValueParams& v, const float* curves, const double* logb,
-                               const float* coef, const float* a, const int* kmax, float* ex,
-                               hipStream_t st);
+// FP32 plans: the matrix-core valuation over build_value_fragments' tables
+hipError_t launch_pf_value_mfma(const PfValueParams& v, const float* curves, const double* logb,
+                                const uint16_t* gfrag, const float* apad, const int* tilek,
+                                float* ex, hipStream_t st);
 // selws (or null): bracket-selection workspace whose counting this pass does, for rank
 // pfe_rank of each (date, set) row, after launch_bracket_sample*; needs
 // pf_stats_stage(...) > 0 (LDS staging slice per wave, in entries; *flush_dates, if given,

@@ -15,6 +15,8 @@
 //     Kahan-added into (A, C); numeraire slot b+1 is recorded after block b.
 //   pf_paths_mfma_kernel: portfolio semantics (4 M-tiles): the curve is checkpointed at
 //     every date and the short rate's left-point sum (row 51) extends the log bank account.
+//   pf_value_mfma_kernel: the portfolio's netting-set values at one date, C = a - gamma P
+//     over 16-set x 16-path tiles (sets as M, paths as N, periods as K).
 #include "hjm_common.h"
 #include "hjm_kernels.h"
 #include "hjm_mfma_plan.h"
@@ -419,6 +421,157 @@ hipError_t launch_pf_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, float*
         pf_paths_mfma_kernel<true><<<grid, kMfmaBlock, 2 * kMfmaPfFragElems, st>>>(g, w, curves, logb, zs);
     else
         pf_paths_mfma_kernel<false><<<grid, kMfmaBlock, 2 * kMfmaPfFragElems, st>>>(g, w, curves, logb, zs);
+    return hipGetLastError();
+}
+
+// Netting-set exposures of one date on the matrix cores (FP32 plans):
+//   E[set][path] = max(a_set - sum_k gamma[set][k] P_k(path), 0) D(path)
+// as C = A B + a, A = -gamma (16 sets x 32 periods per fragment, its three bf16 parts
+// precomputed per date), B = the paths' bond prices P_k, split exactly in registers, six
+// split products per tile as in the path kernels. Lane l of a wave owns path 16 n + l % 16
+// of N-tile n and periods 8 (l / 16) .. + 7 of each of the two 32-period chunks: it builds
+// the trapezoid sums of its two 8-period segments from the curve checkpoint and takes the
+// sums of the segments before them from the other lane groups (shuffles), then P_k =
+// exp(-I_k). The accumulator layout (lane l: sets 4 (l / 16) + r of path l % 16) is stored
+// directly: per store, 16 consecutive paths of one set. A workgroup of 4 waves x 64 paths
+// stages the date's A fragments in LDS, 8 M-tiles (128 sets) at a time; an M-tile whose
+// sets have no live period past 32 skips the second chunk.
+constexpr int kValBlock = 256;
+constexpr int kValNT = 4;                                  // N-tiles (16 paths) per wave
+constexpr int kValPaths = kValBlock / kWave * kValNT * 16; // paths per workgroup
+constexpr int kValLdsTiles = 8;                            // M-tiles staged at a time
+
+__global__ __launch_bounds__(kValBlock) void pf_value_mfma_kernel(const PfValueParams v, int mtiles,
+                                                                const float* __restrict__ curves,
+                                                                const double* __restrict__ logb,
+                                                                const u4_t* __restrict__ gfrag,
+                                                                const float* __restrict__ apad,
+                                                                const int* __restrict__ tilek,
+                                                                float* __restrict__ exposures)
+{
+    __shared__ u4_t sfrag[kValLdsTiles * kValueFragsPerTile * kWave];
+    const int e = blockIdx.y;
+    const int64_t P = v.paths;
+    const int G = v.nsets;
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const int col = lane % 16, grp = lane / 16;
+    const int64_t pbase = (int64_t)blockIdx.x * kValPaths + wave * (kValNT * 16);
+    const float hw = (float)(0.5 * v.dtau), dtf = (float)v.dtau;
+    // discount factor of N-tile grp's path, shared with the other lane groups below
+    float disc[kValNT];
+    {
+        const int64_t p = pbase + 16 * grp + col;
+        const float d = (float)exp(-logb[(int64_t)e * P + (p < P ? p : 0)]);
+#pragma unroll
+        for (int n = 0; n < kValNT; ++n) disc[n] = __shfl(d, col + 16 * n, kWave);
+    }
+    Split8 bs[kValNT][2];
+#pragma unroll
+    for (int n = 0; n < kValNT; ++n) {
+        const int64_t p = pbase + 16 * n + col;
+        const float* fc = curves + (int64_t)e * kTenors * P + (p < P ? p : 0);
+        // segment 0: periods k = 8 grp + 1 .. 8 grp + 8 (tenors 8 grp .. 8 grp + 8 <= 32);
+        // segment 1: k = 33 + 8 grp .. 40 + 8 grp (tenors capped at the last, 50)
+        float fa[9], fb[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) fa[i] = fc[(int64_t)(8 * grp + i) * P];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const int j = 32 + 8 * grp + i;
+            fb[i] = fc[(int64_t)(j < kTenors - 1 ? j : kTenors - 1) * P];
+        }
+        float ca[8], cb[8];
+        float s = 0.f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            s += hw * (fa[i] + fa[i + 1]);
+            ca[i] = s;
+        }
+        s = 0.f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = 33 + 8 * grp + i;
+            s += k <= kTenors - 1 ? hw * (fb[i] + fb[i + 1]) : dtf * fb[i];
+            cb[i] = s;
+        }
+        float t0[4], t1[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            t0[g] = __shfl(ca[7], col + 16 * g, kWave);
+            t1[g] = __shfl(cb[7], col + 16 * g, kWave);
+        }
+        float offa = 0.f, offb = ((t0[0] + t0[1]) + t0[2]) + t0[3];
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+            offa += g < grp ? t0[g] : 0.f;
+            offb += g < grp ? t1[g] : 0.f;
+        }
+        float pa[8], pb[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            pa[i] = expf(-(offa + ca[i]));
+            pb[i] = 32 + 8 * grp + i < kMaxPeriods ? expf(-(offb + cb[i])) : 0.f;
+        }
+        bs[n][0] = split8(pa);
+        bs[n][1] = split8(pb);
+    }
+    for (int m0 = 0; m0 < mtiles; m0 += kValLdsTiles) {
+        const int mt = mtiles - m0 < kValLdsTiles ? mtiles - m0 : kValLdsTiles;
+        __syncthreads();
+        const u4_t* src = gfrag + ((int64_t)e * mtiles + m0) * kValueFragsPerTile * kWave;
+        for (int t = threadIdx.x; t < mt * kValueFragsPerTile * kWave; t += kValBlock) sfrag[t] = src[t];
+        __syncthreads();
+        for (int mm = 0; mm < mt; ++mm) {
+            const int m = m0 + mm;
+            const int tk = tilek[(int64_t)e * mtiles + m];
+            const f4_t a4 = *reinterpret_cast<const f4_t*>(apad + ((int64_t)e * mtiles + m) * 16 + 4 * grp);
+            f4_t acc[kValNT];
+#pragma unroll
+            for (int n = 0; n < kValNT; ++n) acc[n] = a4;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                if (tk > 32 * c) {
+                    const u4_t* fr = sfrag + (mm * kValueFragsPerTile + c * kMfmaParts) * kWave + lane;
+                    const u4_t ah = fr[0], am = fr[kWave], al = fr[2 * kWave];
+#pragma unroll
+                    for (int n = 0; n < kValNT; ++n) {
+                        f4_t x = acc[n];
+                        x = mfma_bf16(ah, bs[n][c].h, x);
+                        x = mfma_bf16(ah, bs[n][c].m, x);
+                        x = mfma_bf16(am, bs[n][c].h, x);
+                        x = mfma_bf16(ah, bs[n][c].l, x);
+                        x = mfma_bf16(al, bs[n][c].h, x);
+                        x = mfma_bf16(am, bs[n][c].m, x);
+                        acc[n] = x;
+                    }
+                }
+            }
+#pragma unroll
+            for (int n = 0; n < kValNT; ++n) {
+                const int64_t p = pbase + 16 * n + col;
+                if (p < P) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int g = 16 * m + 4 * grp + r;
+                        const float V = acc[n][r];
+                        if (g < G) exposures[((int64_t)e * G + g) * P + p] = V > 0.f ? V * disc[n] : 0.f;
+                    }
+                }
+            }
+        }
+    }
+}
+
+hipError_t launch_pf_value_mfma(const PfValueParams& v, const float* curves, const double* logb,
+                                const uint16_t* gfrag, const float* apad, const int* tilek,
+                                float* ex, hipStream_t st)
+{
+    if (v.paths <= 0 || v.nsets <= 0 || v.ndates <= 0) return hipErrorInvalidValue;
+    const int mtiles = value_mtiles(v.nsets);
+    const dim3 grid((unsigned)((v.paths + kValPaths - 1) / kValPaths), (unsigned)v.ndates);
+    pf_value_mfma_kernel<<<grid, kValBlock, 0, st>>>(v, mtiles, curves, logb,
+                                                     reinterpret_cast<const u4_t*>(gfrag), apad,
+                                                     tilek, ex);
     return hipGetLastError();
 }
 

@@ -29,6 +29,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "hjm_common.h"
@@ -199,6 +200,51 @@ inline void build_mfma_block_map(const double* drift, const double* vols, double
                     for (int p = 0; p < kMfmaParts; ++p)
                         frag[((((size_t)m * kMfmaChunks + c) * kMfmaParts + p) * kWave + l) * 8 + i] = parts[p];
                 }
+}
+
+// Matrix-core valuation of netting sets (pf_value_mfma_kernel): per date, M-tiles of 16
+// sets x 2 chunks of 32 periods x 3 bf16 parts of -gamma, in the A-operand order
+// [date][m][chunk][part][lane][i] = part of -gamma[date][16 m + lane % 16][32 chunk +
+// 8 (lane / 16) + i] (0 past the last set or period); apad[date][16 m + j] = a of set
+// 16 m + j (0 past the last set); tilek[date][m] = the largest kmax of the tile's sets.
+constexpr int kValueFragsPerTile = 2 * kMfmaParts;  // u4 fragments per lane and M-tile
+
+inline int value_mtiles(int nsets) { return (nsets + 15) / 16; }
+
+inline void build_value_fragments(const std::vector<double>& coef, const std::vector<double>& a,
+                                  const std::vector<int>& kmax, int ndates, int nsets, int kpad,
+                                  std::vector<uint16_t>& frag, std::vector<float>& apad,
+                                  std::vector<int>& tilek)
+{
+    const int mt = value_mtiles(nsets);
+    frag.assign((size_t)ndates * mt * kValueFragsPerTile * kWave * 8, 0);
+    apad.assign((size_t)ndates * mt * 16, 0.f);
+    tilek.assign((size_t)ndates * mt, 0);
+    for (int e = 0; e < ndates; ++e)
+        for (int m = 0; m < mt; ++m) {
+            for (int j = 0; j < 16; ++j) {
+                const int g = 16 * m + j;
+                if (g >= nsets) continue;
+                apad[((size_t)e * mt + m) * 16 + j] = (float)a[(size_t)e * nsets + g];
+                tilek[(size_t)e * mt + m] = std::max(tilek[(size_t)e * mt + m], kmax[(size_t)e * nsets + g]);
+            }
+            for (int c = 0; c < 2; ++c)
+                for (int l = 0; l < kWave; ++l)
+                    for (int i = 0; i < 8; ++i) {
+                        const int g = 16 * m + l % 16, kk = 32 * c + 8 * (l / 16) + i;
+                        const double w = (g < nsets && kk < kpad)
+                                             ? -coef[((size_t)e * nsets + g) * kpad + kk]
+                                             : 0.0;
+                        const uint16_t h = bf16_rne(w);
+                        const double r1 = w - bf16_value(h);
+                        const uint16_t mm = bf16_rne(r1);
+                        const uint16_t lo = bf16_rne(r1 - bf16_value(mm));
+                        const uint16_t parts[3] = {h, mm, lo};
+                        for (int q = 0; q < kMfmaParts; ++q)
+                            frag[(((((size_t)e * mt + m) * 2 + c) * kMfmaParts + q) * kWave + l) * 8 + i] =
+                                parts[q];
+                    }
+        }
 }
 
 }  // namespace hjm

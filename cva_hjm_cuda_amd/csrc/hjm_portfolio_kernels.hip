@@ -8,9 +8,11 @@
 // netted per set. Three passes:
 //   1. pf_paths_kernel: persistent, one path per lane, curve in VGPRs; at every date it
 //      checkpoints the curve and the log bank account (coalesced [date][tenor][path]).
-//   2. pf_value_kernel: per path and date, 60 bond prices and every netting set's value
-//      V_g = a_ge - sum_k gamma_gek P_k (host-aggregated coefficients, read through
-//      scalar loads), exposure max(V_g, 0) / B stored [date][set][path].
+//   2. pf_value_kernel (FP64; FP32 plans use pf_value_mfma_kernel, the same values with
+//      the set sums on the matrix cores, in hjm_mfma_kernels.hip): per path and date, 60
+//      bond prices and every netting set's value V_g = a_ge - sum_k gamma_gek P_k
+//      (host-aggregated coefficients staged in LDS), exposure max(V_g, 0) / B stored
+//      [date][set][path].
 //   3. pf_stats_kernel: per set, EE numerators and per-path CVA moments (FP64).
 #include "hjm_common.h"
 #include "hjm_kernels.h"
@@ -440,13 +442,6 @@ hipError_t launch_pf_value_f64(const PfValueParams& v, const double* curves, con
                                hipStream_t st)
 {
     return pf_value_launch<double>(v, curves, logb, coef, a, kmax, ex, st);
-}
-
-hipError_t launch_pf_value_f32(const PfValueParams& v, const float* curves, const double* logb,
-                               const float* coef, const float* a, const int* kmax, float* ex,
-                               hipStream_t st)
-{
-    return pf_value_launch<float>(v, curves, logb, coef, a, kmax, ex, st);
 }
 
 int pf_stats_stage(int64_t paths, int64_t pfe_rank, int ndates, int elem_bytes, int* flush_dates)
