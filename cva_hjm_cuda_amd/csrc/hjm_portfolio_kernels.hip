@@ -530,11 +530,12 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_seed_kernel(const PfValuePa
     const int64_t P = v.paths;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int e = blockIdx.y;
-    if (i >= P) return;
+    const bool live = i < P;  // every lane takes part in the workgroup's LDS staging
+    const int64_t ic = live ? i : 0;
     Real f[kTenors];
 #pragma unroll
-    for (int j = 0; j < kTenors; ++j) f[j] = curves[((int64_t)e * kTenors + j) * P + i];
-    const double D = exp(-logb[(int64_t)e * P + i]);
+    for (int j = 0; j < kTenors; ++j) f[j] = curves[((int64_t)e * kTenors + j) * P + ic];
+    const double D = exp(-logb[(int64_t)e * P + ic]);
     constexpr int KP = kMaxPeriods;
     Real Pk[KP], dP[KP];
     Real I = (Real)0;
@@ -546,18 +547,11 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_seed_kernel(const PfValuePa
         Pk[kk - 1] = exp(-I);
         dP[kk - 1] = (Real)0;
     }
-    kconst_ptr_pf<Real> cp = (kconst_ptr_pf<Real>)coef;
-    kconst_ptr_pf<Real> ap = (kconst_ptr_pf<Real>)aconst;
-    kconst_ptr_pf<int> kp = (kconst_ptr_pf<int>)kmax;
-    kconst_ptr_pf<double> wp = (kconst_ptr_pf<double>)w;
     double b = 0.0;
-    for (int gset = 0; gset < v.nsets; ++gset) {
-        const int64_t row = (int64_t)e * v.nsets + gset;
-        const int K = kp[row];
-        kconst_ptr_pf<Real> gam = cp + row * KP;
-        Real V = ap[row];
-        // chunks of 16 periods: each chunk's scalar loads are issued together ahead of its
-        // FMAs (one wait per 16 FMAs)
+    // one set: its value, then dP_k -= w D 1[V > 0] gamma_k
+    auto set_adjoint = [&](int K, auto gam, Real V, double wg) {
+        // chunks of 16 periods: each chunk's loads are issued together ahead of its FMAs
+        // (one wait per 16 FMAs)
 #pragma unroll
         for (int k0 = 0; k0 < KP; k0 += 16) {
             if (k0 < K) {
@@ -572,7 +566,6 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_seed_kernel(const PfValuePa
         }
         // dJ/dV_g = w D where the set's exposure is positive (0 elsewhere: no divergent
         // branch); dV_g/dP_k = -gamma_k
-        const double wg = wp[(int64_t)gset * v.ndates + e];
         const bool pos = V > (Real)0;
         b -= pos ? wg * (double)V * D : 0.0;
         const Real a = pos ? (Real)(wg * D) : (Real)0;
@@ -588,6 +581,38 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_seed_kernel(const PfValuePa
                     if (k0 + u < KP) dP[k0 + u] -= a * gk[u];
             }
         }
+    };
+    if constexpr (sizeof(Real) == 4) {
+        // FP32: the date's coefficient rows staged in LDS, kValueSets sets at a time, read
+        // with workgroup-uniform addresses (broadcasts); scalar loads of them from every
+        // wave cost a third more time
+        __shared__ Real sg[kValueSets * KP];
+        __shared__ Real sa[kValueSets];
+        __shared__ double sw[kValueSets];
+        __shared__ int sk[kValueSets];
+        for (int g0 = 0; g0 < v.nsets; g0 += kValueSets) {
+            const int ng = v.nsets - g0 < kValueSets ? v.nsets - g0 : kValueSets;
+            __syncthreads();
+            const int64_t base = (int64_t)e * v.nsets + g0;
+            for (int t = threadIdx.x; t < ng * KP; t += blockDim.x) sg[t] = coef[base * KP + t];
+            for (int t = threadIdx.x; t < ng; t += blockDim.x) {
+                sa[t] = aconst[base + t];
+                sk[t] = kmax[base + t];
+                sw[t] = w[(int64_t)(g0 + t) * v.ndates + e];
+            }
+            __syncthreads();
+            for (int gg = 0; gg < ng; ++gg) set_adjoint(sk[gg], sg + gg * KP, sa[gg], sw[gg]);
+        }
+    } else {
+        // FP64: scalar loads (LDS staging needs more registers than two waves per SIMD allow)
+        kconst_ptr_pf<Real> cp = (kconst_ptr_pf<Real>)coef;
+        kconst_ptr_pf<Real> ap = (kconst_ptr_pf<Real>)aconst;
+        kconst_ptr_pf<int> kp = (kconst_ptr_pf<int>)kmax;
+        kconst_ptr_pf<double> wp = (kconst_ptr_pf<double>)w;
+        for (int gset = 0; gset < v.nsets; ++gset) {
+            const int64_t row = (int64_t)e * v.nsets + gset;
+            set_adjoint(kp[row], cp + row * KP, ap[row], wp[(int64_t)gset * v.ndates + e]);
+        }
     }
     // suffix sums of iota_k = -P_k dP_k and the extrapolation moment
     Real S = (Real)0, ext = (Real)0;
@@ -601,6 +626,7 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_seed_kernel(const PfValuePa
         Sk[kk] = S;
         if (kk > kTenors - 1) ext += (Real)(kk - (kTenors - 1)) * iota;
     }
+    if (!live) return;
     float* gb = gbar + (int64_t)e * kTenors * P + i;
 #pragma unroll
     for (int l = 0; l < kTenors; ++l) {

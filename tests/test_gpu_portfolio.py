@@ -189,6 +189,6 @@ def test_full_config_greeks_cost_ratio():
            % (g.timings_ms[0], g.timings_ms[1], ratio))
     assert g.cva > 0 and np.all(np.isfinite(g.d_vol_total))
     # the FP32 forward (path kernel and valuation) runs on the matrix cores, the seed and
-    # the reverse sweeps on the VALU; measured 4.16 on an MI355X (forward 2.39 ms, adjoint
-    # 7.56 ms), bounded with room for box-to-box spread
+    # the reverse sweeps on the VALU; measured 3.69 on an MI355X (forward 2.38 ms, adjoint
+    # 6.41 ms), bounded with room for box-to-box spread
     assert ratio <= 4.5
