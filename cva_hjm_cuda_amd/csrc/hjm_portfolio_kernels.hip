@@ -209,13 +209,13 @@ constexpr int kStatPerThread = 8;
 constexpr int kStatPaths = kBlock * kStatPerThread;
 
 template <typename Real, bool kCount>
-__global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
-                                                          const Real* __restrict__ exposures,
-                                                          const double* __restrict__ w,
-                                                          double* __restrict__ partials,
-                                                          SelRowState* __restrict__ sel,
-                                                          Real* __restrict__ cand, int64_t cap,
-                                                          int stage, int flush)
+__device__ __forceinline__ void pf_stats_rows(const PfValueParams v,
+                                              const Real* __restrict__ exposures,
+                                              const double* __restrict__ w,
+                                              double* __restrict__ partials,
+                                              SelRowState* __restrict__ sel,
+                                              Real* __restrict__ cand, int64_t cap, int stage,
+                                              int flush)
 {
     typedef typename SelectBits<Real>::U U;
     constexpr int kWaves = kBlock / kWave;
@@ -379,6 +379,30 @@ __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
     }
 }
 
+template <typename Real, bool kCount>
+__global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
+                                                          const Real* __restrict__ exposures,
+                                                          const double* __restrict__ w,
+                                                          double* __restrict__ partials,
+                                                          SelRowState* __restrict__ sel,
+                                                          Real* __restrict__ cand, int64_t cap,
+                                                          int stage, int flush)
+{
+    pf_stats_rows<Real, kCount>(v, exposures, w, partials, sel, cand, cap, stage, flush);
+}
+
+// FP32 rows with the PFE counting: held to 80 VGPRs for 6 waves per SIMD (20 bytes per
+// lane spill), whose extra loads in flight outweigh the spills (7.87 -> 7.63 ms at 1048576
+// paths x 61 dates x 128 sets). The FP64 rows keep their 102 VGPRs (4 waves): at 80 they
+// would spill 80 bytes per lane.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void pf_stats_count_f32_kernel(
+    const PfValueParams v, const float* __restrict__ exposures, const double* __restrict__ w,
+    double* __restrict__ partials, SelRowState* __restrict__ sel, float* __restrict__ cand,
+    int64_t cap, int stage, int flush)
+{
+    pf_stats_rows<float, true>(v, exposures, w, partials, sel, cand, cap, stage, flush);
+}
+
 // out[g][c] = sum_b partials[g][b][c] in block order.
 __global__ __launch_bounds__(kBlock) void grouped_column_sum_kernel(const double* __restrict__ partials,
                                                                     int64_t rows, int cols,
@@ -479,8 +503,12 @@ static hipError_t pf_stats_launch(const PfValueParams& v, const Real* ex, const 
         Real* cand;
         bracket_ws_split<Real>(selws, v.ndates * v.nsets, &state, &cand);
         const size_t lds = (size_t)(kBlock / kWave) * stage * sizeof(Real);
-        pf_stats_kernel<Real, true><<<grid, kBlock, lds, st>>>(v, ex, w, partials, state, cand, bp.cap,
-                                                              stage, flush);
+        if constexpr (sizeof(Real) == sizeof(float))
+            pf_stats_count_f32_kernel<<<grid, kBlock, lds, st>>>(v, ex, w, partials, state, cand, bp.cap,
+                                                                stage, flush);
+        else
+            pf_stats_kernel<Real, true><<<grid, kBlock, lds, st>>>(v, ex, w, partials, state, cand,
+                                                                  bp.cap, stage, flush);
     } else {
         pf_stats_kernel<Real, false><<<grid, kBlock, 0, st>>>(v, ex, w, partials, nullptr, nullptr, 0,
                                                              0, 0);
