@@ -393,8 +393,8 @@ __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
 
 // FP32 rows with the PFE counting: held to 80 VGPRs for 6 waves per SIMD (20 bytes per
 // lane spill), whose extra loads in flight outweigh the spills (7.87 -> 7.63 ms at 1048576
-// paths x 61 dates x 128 sets). The FP64 rows keep their 102 VGPRs (4 waves): at 80 they
-// would spill 80 bytes per lane.
+// paths x 61 dates x 128 sets; at 72 VGPRs, 7 waves, the 60 bytes of spills cost 11.9 ms).
+// The FP64 rows keep their 102 VGPRs (4 waves): at 80 they would spill 80 bytes per lane.
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void pf_stats_count_f32_kernel(
     const PfValueParams v, const float* __restrict__ exposures, const double* __restrict__ w,
     double* __restrict__ partials, SelRowState* __restrict__ sel, float* __restrict__ cand,
