@@ -170,7 +170,43 @@ def measured_traffic(kernel):
     return rec.get("traffic_bytes") if rec.get("kernel") == kernel else None
 
 
-def cpu_baseline(market, target_s=10.0, nthreads=1):
+def host_cpus():
+    """The host's CPUs as this process sees them: model string, nproc, the CPUs in the
+    scheduler affinity mask, the cgroup CPU quota (cpu.max) and OMP_NUM_THREADS."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            quota = None if q == "max" else int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "cpus_in_affinity": len(os.sched_getaffinity(0)), "cgroup_cpu_quota": quota,
+            "omp_num_threads": int(omp) if omp and omp.isdigit() else None}
+
+
+def all_core_threads(info):
+    """Threads for the all-cores baseline: every CPU in the affinity mask, capped by the
+    job's CPU share where the host states one (cgroup quota, OMP_NUM_THREADS)."""
+    n = info["cpus_in_affinity"]
+    if info["cgroup_cpu_quota"]:
+        n = min(n, max(1, int(info["cgroup_cpu_quota"])))
+    if info["omp_num_threads"]:
+        n = min(n, info["omp_num_threads"])
+    return max(1, n)
+
+
+def cpu_baseline(market, target_s=10.0, nthreads=1, info=None):
     """The oracle's reference loop (simulation_cpu.h:413-436) in FP64 on `nthreads` threads
     (OpenMP over paths), timed around the path loop only: the first 1024 x nthreads paths
     of the workload, run repeatedly until ~target_s of timed work."""
@@ -187,11 +223,14 @@ def cpu_baseline(market, target_s=10.0, nthreads=1):
                                            nthreads=nthreads)
         ms += t
         reps += 1
-    return dict(value=reps * chunk * S / (ms * 1e-3), unit="path-steps/s", cores=nthreads,
-                kind="port",
-                sample="the first %d of the %d paths x %d steps, timed %d times, FP64 reference "
-                       "loop, %d thread(s), normals generated outside the timed region"
-                       % (chunk, PATHS, S, reps, nthreads))
+    out = dict(value=reps * chunk * S / (ms * 1e-3), unit="path-steps/s", cores=nthreads,
+               kind="port",
+               sample="the first %d of the %d paths x %d steps, timed %d times, FP64 reference "
+                      "loop, %d thread(s), normals generated outside the timed region"
+                      % (chunk, PATHS, S, reps, nthreads))
+    if info:
+        out.update(info)
+    return out
 
 
 def main():
@@ -305,19 +344,26 @@ def main():
                      cva_rel_err_vs_fp64=abs(r.cva / f64["cva"] - 1.0))
 
     big = f64["ee"] > 1e-3 * f64["ee"].max()
-    err = dict(
-        ee_max_rel=float(np.max(np.abs(f32["ee"][big] / f64["ee"][big] - 1))),
-        pfe_max_rel=float(np.max(np.abs(f32["pfe"][big] / f64["pfe"][big] - 1))),
-        cva_rel=abs(f32["cva"] / f64["cva"] - 1.0),
-        cva_se_rel=f64["se"] / f64["cva"])
+
+    def error_vs_fp64(r):
+        """EE / PFE97.5 (dates with EE above 1e-3 of its peak) and CVA of an FP32 run
+        against the FP64 run on the same in-kernel normals."""
+        return dict(
+            ee_max_rel=float(np.max(np.abs(r["ee"][big] / f64["ee"][big] - 1))),
+            pfe_max_rel=float(np.max(np.abs(r["pfe"][big] / f64["pfe"][big] - 1))),
+            cva_rel=abs(r["cva"] / f64["cva"] - 1.0),
+            cva_se_rel=f64["se"] / f64["cva"])
+
+    err = error_vs_fp64(f32)
 
     log("[bench] adjoint configuration: CVA + 204 Greeks vs forward")
     adj = adjoint_ratio(hjm, market)
     cpu = cpu_all = None
     if not args.no_cpu:
         log("[bench] CPU oracle, 1 thread and all cores")
-        cpu = cpu_baseline(market)
-        cpu_all = cpu_baseline(market, nthreads=min(16, os.cpu_count() or 1))
+        info = host_cpus()
+        cpu = cpu_baseline(market, info=info)
+        cpu_all = cpu_baseline(market, nthreads=all_core_threads(info), info=info)
     res = {
         "metric": "path-steps/s (51 tenors) FP32 vs FP64; CVA+204 Greeks time vs forward-only",
         "value": value,
@@ -347,6 +393,7 @@ def main():
         "fp32_" + other: {"kernel": KERNEL_NAMES[other], "kernel_ms": kalt,
                           "path_steps_per_s": PATHS * S * alt["kernel_ms"].shape[0] / alt["wall"],
                           "roofline_frac": flops / (kalt * 1e-3) / 1e12 / PEAK_TFLOPS["f32"],
+                          "fp32_vs_fp64_error": error_vs_fp64(alt),
                           "cva_rel_vs_headline": abs(alt["cva"] / f32["cva"] - 1.0)},
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,

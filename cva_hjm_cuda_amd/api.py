@@ -260,6 +260,18 @@ def _config(paths, dt, precision, rng, seed, normals, hazard, recovery, pfe_quan
                    int(device), int(path_kernel))
 
 
+def _host_normals(normals, paths: int, horizon: float, dt: float, what: str) -> np.ndarray:
+    """Caller-supplied normals as the C ABI reads them: contiguous float64
+    [paths][round(horizon / dt)][3]. Any other shape is rejected here, since the library
+    copies exactly paths x steps x 3 values from the pointer."""
+    z = np.ascontiguousarray(normals, np.float64)
+    want = (int(paths), int(round(horizon / dt)), 3)
+    if z.shape != want:
+        raise ValueError("%s: normals must have shape (paths, horizon/dt, 3) = %s, got %s"
+                         % (what, want, z.shape))
+    return z
+
+
 def simulate_reference(market: Market, swap: InterestRateSwap, paths: int, dt: float = 0.01,
                        precision: int = FP64, seed: int = 1234,
                        normals: Optional[np.ndarray] = None, hazard: float = 0.02,
@@ -275,11 +287,8 @@ def simulate_reference(market: Market, swap: InterestRateSwap, paths: int, dt: f
     lib = library()
     rng = RNG_PHILOX
     if normals is not None:
-        normals = np.ascontiguousarray(normals, np.float64)
+        normals = _host_normals(normals, paths, swap.expiry, dt, "simulate_reference")
         rng = RNG_HOST
-        steps = int(round(swap.expiry / dt))
-        if normals.shape != (paths, steps, 3):
-            raise ValueError("normals must have shape (paths, expiry/dt, 3)")
     mk = market._c()
     sw = _Swap(swap.notional, swap.K, swap.expiry, swap.dtau)
     cfg = _config(paths, dt, precision, rng, seed, normals, hazard, recovery, pfe_quantile,
@@ -316,7 +325,7 @@ def cva_greeks(market: Market, swap: InterestRateSwap, paths: int, dt: float = 0
     """CVA and its 51 + 153 (+ 51 drift) pathwise-adjoint sensitivities on the GPU."""
     rng = RNG_PHILOX
     if normals is not None:
-        normals = np.ascontiguousarray(normals, np.float64)
+        normals = _host_normals(normals, paths, swap.expiry, dt, "cva_greeks")
         rng = RNG_HOST
     mk = market._c()
     sw = _Swap(swap.notional, swap.K, swap.expiry, swap.dtau)
@@ -475,7 +484,7 @@ class PortfolioPlan:
         self._keep = (market, portfolio, normals)
         rng = RNG_PHILOX
         if normals is not None:
-            normals = np.ascontiguousarray(normals, np.float64)
+            normals = _host_normals(normals, paths, horizon, dt, "PortfolioPlan")
             self._keep = (market, portfolio, normals)
             rng = RNG_HOST
         mk = market._c()
@@ -546,7 +555,7 @@ def portfolio_cva_greeks(market: Market, portfolio, paths: int, horizon: float =
     """Total portfolio CVA and its 51 + 153 (+ 51 drift) pathwise-adjoint sensitivities."""
     rng = RNG_PHILOX
     if normals is not None:
-        normals = np.ascontiguousarray(normals, np.float64)
+        normals = _host_normals(normals, paths, horizon, dt, "portfolio_cva_greeks")
         rng = RNG_HOST
     mk = market._c()
     pfc = _pf_c(portfolio, horizon, recovery)

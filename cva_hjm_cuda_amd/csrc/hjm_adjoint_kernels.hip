@@ -256,9 +256,9 @@ hipError_t launch_ref_adjoint_sweep(const RefGrid& rg, double c, double sqrt_dt,
     g.path0 = rg.path0;
     g.seed_lo = rg.seed_lo;
     g.seed_hi = rg.seed_hi;
-    g.c = (float)c;
-    g.omc = (float)(1.0 - c);
-    g.opc = (float)(1.0 + c);
+    g.c = step_ratio_f32(c);  // the FP32 forward step's coefficients (exact pairs)
+    g.omc = 1.0f - g.c;
+    g.opc = 1.0f + g.c;
     g.sqrt_dt = (float)sqrt_dt;
     const dim3 grid((unsigned)((rg.paths + kBlock - 1) / kBlock));
     if (zs && rg.stride % 10 != 0) return hipErrorInvalidValue;  // stored-normal prefetch chunks

@@ -1,6 +1,7 @@
 // hjm_common.h -- shared definitions of the HJM exposure library (gfx950).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
 #include "hjm_cva.h"
@@ -12,6 +13,15 @@ constexpr int kFactors = HJM_FACTORS;  // 3 volatility factors
 constexpr int kWave = 64;              // CDNA wavefront
 constexpr int kBlock = 256;            // one path per lane, 4 waves per workgroup
 constexpr int kStatSlots = kTenors + 2;  // EE sums (51), per-path CVA sum and square sum
+
+// c = dt / dtau as the FP32 Euler step uses it: rounded to a multiple of 2^-23, so that
+// 1 - c and 1 + c are exact in FP32 and the coefficient pairs (1 - c, c) of the interior
+// update f' = (1 - c) f_j + c f_{j+1} + ... and (1 + c, -c) of the last tenor sum to 1
+// exactly. With c and 1 - c rounded independently (0.02 and 0.98 sum to 1 + 1.9e-8 in
+// FP32) every step scales a flat curve by 1 + 1.9e-8: a systematic +5.8e-7 on the
+// forwards after 950 steps and a 2.7e-5 error of the 10y CVA, where the rounded pair
+// leaves 6e-8 (CPU emulation of the FP32 step against the FP64 oracle, 2048 paths).
+inline float step_ratio_f32(double c) { return (float)(nearbyint(c * 8388608.0) / 8388608.0); }
 
 // Market constants of the reference-semantics path kernel, passed by value so that
 // they live in the kernarg segment and reach the ALUs through scalar loads: every lane

@@ -85,9 +85,15 @@ inline void fill_consts(RefStepConsts<Real>& k, const hjm_market* m, double dt, 
         k.drift[j] = (Real)m->drifts[j];
     }
     const double c = dt / dtau;
-    k.c = (Real)c;
-    k.omc = (Real)(1.0 - c);
-    k.opc = (Real)(1.0 + c);
+    if constexpr (sizeof(Real) == sizeof(float)) {
+        k.c = step_ratio_f32(c);
+        k.omc = 1.0f - k.c;   // exact
+        k.opc = 1.0f + k.c;   // exact
+    } else {
+        k.c = (Real)c;
+        k.omc = (Real)(1.0 - c);
+        k.opc = (Real)(1.0 + c);
+    }
     k.sqrt_dt = (Real)sqrt(dt);
     k.dtau = (Real)dtau;
 }

@@ -864,7 +864,7 @@ hipError_t launch_pf_adjoint_sweep(const PfGrid& g, double c, int pass, const do
                                    double* partials, hipStream_t st)
 {
     const dim3 grid((unsigned)((g.paths + kBlock - 1) / kBlock));
-    const float cf = (float)c;
+    const float cf = step_ratio_f32(c);  // 1 - cf, 1 + cf exact, as in the FP32 forward
     if (zs && g.stride % 10 != 0) return hipErrorInvalidValue;  // stored-normal prefetch chunks
 #define HJM_SWEEP(P_, S_) \
     pf_adjoint_sweep_kernel<P_, S_><<<grid, kBlock, 0, st>>>(g, cf, normals, zs, gbar, beta, partials)

@@ -11,9 +11,9 @@
 // A 50-step block thus costs 38 Philox blocks and 76 Box-Muller pairs (0.76 and 1.52 per
 // step) against one block and 1.75 pairs per step when every step draws its own block,
 // and a matrix-core lane, which generates the steps of one class, makes every call once.
-// ln, sin and cos are fixed FP32 polynomials evaluated with explicit FMAs (coefficients
-// from rng_poly_fit.py), so host and GPU produce the same bits; the square root is
-// IEEE-rounded on both.
+// The Philox words are exact (the CPU oracle reproduces them bit for bit); Box-Muller runs
+// on the hardware's log2 / sqrt / sin / cos instructions, within the bound stated at
+// box_muller of the exact transform of the same uniforms.
 //
 // Counter addressing makes any (path, step) directly reachable: paths shard across
 // devices without stream offsets, and the adjoint sweep regenerates shocks in reverse.
@@ -65,105 +65,30 @@ HJM_HD void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, ui
     out[3] = c3;
 }
 
-HJM_HD float bits_to_float(uint32_t b) { return __builtin_bit_cast(float, b); }
-HJM_HD uint32_t float_to_bits(float f) { return __builtin_bit_cast(uint32_t, f); }
-
-// ln(u), u in [2^-24, 1]: u = 2^e m with m in [sqrt(1/2), sqrt(2)], ln m = x + x^2 P(x).
-HJM_HD float rng_log(float u)
+// (z0, z1) = r cos, r sin of the Philox pair (a, b) on the hardware's transcendental
+// instructions (gfx950: v_log_f32 = log2, v_sqrt_f32, v_cos_f32 / v_sin_f32 of an angle in
+// turns -- exactly the form Box-Muller needs):
+//   u = fma(float(a), 2^-32, 2^-33) in (0, 1]    (float(a): a rounded to 24 bits)
+//   t = float(b) 2^-32 in [0, 1]                  (turns)
+//   r = sqrt(-2 ln 2 * log2(u)),  z0 = r cos(2 pi t),  z1 = r sin(2 pi t)
+// u and t are exact functions of the Philox outputs (the CPU oracle computes the same
+// floats); r, cos and sin carry the instructions' errors. Measured on an MI355X over every
+// distinct u and t: |r - r_exact| <= 1.4e-7 r, |cos - cos_exact|, |sin - sin_exact| <=
+// 1.3e-7, so |z - z_exact| <= 2.6e-7 |z| + 8.8e-7 (r <= sqrt(66 ln 2) = 6.77). Four
+// transcendental and seven plain VALU instructions per pair.
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float& z0, float& z1)
 {
-#pragma clang fp contract(off)
-    const uint32_t b = float_to_bits(u);
-    int e = (int)((b >> 23) & 0xffu) - 127;
-    uint32_t mb = (b & 0x007fffffu) | 0x3f800000u;
-    if (mb > 0x3fb504f3u) {
-        mb -= 0x00800000u;
-        e += 1;
-    }
-    const float x = bits_to_float(mb) - 1.0f;
-    float p = -0x1.383d96p-4f;
-    p = __builtin_fmaf(p, x, 0x1.084ep-3f);
-    p = __builtin_fmaf(p, x, -0x1.0f375cp-3f);
-    p = __builtin_fmaf(p, x, 0x1.22709cp-3f);
-    p = __builtin_fmaf(p, x, -0x1.54274ap-3f);
-    p = __builtin_fmaf(p, x, 0x1.99a402p-3f);
-    p = __builtin_fmaf(p, x, -0x1.000428p-2f);
-    p = __builtin_fmaf(p, x, 0x1.55555p-2f);
-    p = __builtin_fmaf(p, x, -0x1.fffff8p-2f);
-    const float x2 = x * x;
-    const float lnm = __builtin_fmaf(x2, p, x);
-    const float fe = (float)e;
-    return __builtin_fmaf(fe, 0x1.62e4p-1f, __builtin_fmaf(fe, 0x1.7f7d1ep-20f, lnm));
-}
-
-// sin and cos of 2 pi k / 2^24 for k in [0, 2^24), quadrant reduction in integers.
-HJM_HD void rng_sincos_turn24(uint32_t k24, float& s, float& c)
-{
-#pragma clang fp contract(off)
-    const uint32_t q = (k24 + (1u << 21)) >> 22;
-    const int32_t ri = (int32_t)k24 - (int32_t)(q << 22);
-    const float r = (float)ri * 0x1p-24f;
-    const float r2 = r * r;
-    float ps = 0x1.4bc89ap+5f;
-    ps = __builtin_fmaf(ps, r2, -0x1.32ca9ep+6f);
-    ps = __builtin_fmaf(ps, r2, 0x1.466bbap+6f);
-    ps = __builtin_fmaf(ps, r2, -0x1.4abbcep+5f);
-    ps = __builtin_fmaf(ps, r2, 0x1.921fb6p+2f);
-    const float sn = r * ps;
-    float pc = 0x1.d9d57cp+5f;
-    pc = __builtin_fmaf(pc, r2, -0x1.55c5ep+6f);
-    pc = __builtin_fmaf(pc, r2, 0x1.03c1dep+6f);
-    pc = __builtin_fmaf(pc, r2, -0x1.3bd3ccp+4f);
-    const float cs = __builtin_fmaf(r2, pc, 1.0f);
-    const uint32_t qq = q & 3u;
-    const float a = (qq & 1u) ? cs : sn;   // sin of the full angle, up to sign
-    const float bb = (qq & 1u) ? sn : cs;  // cos of the full angle, up to sign
-    s = (qq & 2u) ? -a : a;
-    c = ((qq + 1u) & 2u) ? -bb : bb;
-}
-
-HJM_HD float rng_cos_turn24(uint32_t k24)
-{
-    float s, c;
-    rng_sincos_turn24(k24, s, c);
-    return c;
-}
-
-// IEEE-rounded square root for x == 0 and x >= 2^-96 (the Box-Muller radii lie in
-// [0, 33.3]): on the GPU, v_sqrt_f32 (within 1 ulp) and a one-ulp correction from the two
-// neighbours' residuals, the hardware-rate form of the correctly rounded expansion
-// without its tiny-argument rescaling.
-HJM_HD float rng_sqrt(float x)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-#pragma clang fp contract(off)
-    const float s = __builtin_amdgcn_sqrtf(x);
-    const float dn = bits_to_float(float_to_bits(s) - 1u);
-    const float up = bits_to_float(float_to_bits(s) + 1u);
-    const float rdn = __builtin_fmaf(-dn, s, x);
-    const float rup = __builtin_fmaf(-up, s, x);
-    const float t = (rdn <= 0.0f) ? dn : s;
-    return (rup > 0.0f) ? up : t;
-#else
-    return __builtin_sqrtf(x);
-#endif
-}
-
-// (z0, z1) = r cos, r sin of the uniforms' pair (a, b): r = sqrt(-2 ln u(a)),
-// angle 2 pi (b >> 8) / 2^24, u(a) = ((a >> 8) + 1) 2^-24.
-HJM_HD void box_muller(uint32_t a, uint32_t b, float& z0, float& z1)
-{
-#pragma clang fp contract(off)
-    const float u = (float)((a >> 8) + 1u) * 0x1p-24f;
-    const float r = rng_sqrt(-2.0f * rng_log(u));
-    float s, c;
-    rng_sincos_turn24(b >> 8, s, c);
-    z0 = r * c;
-    z1 = r * s;
+    const float u = __builtin_fmaf((float)a, 0x1p-32f, 0x1p-33f);
+    const float r = __builtin_amdgcn_sqrtf(-0x1.62e430p+0f * __builtin_amdgcn_logf(u));
+    const float t = (float)b * 0x1p-32f;
+    z0 = r * __builtin_amdgcn_cosf(t);
+    z1 = r * __builtin_amdgcn_sinf(t);
 }
 
 // Shocks 4 call .. 4 call + 3 of the stream (path, block, cls).
-HJM_HD void stream_call(uint32_t seed_lo, uint32_t seed_hi, uint32_t path, uint32_t block,
-                        uint32_t cls, uint32_t call, float (&z)[4])
+__device__ __forceinline__ void stream_call(uint32_t seed_lo, uint32_t seed_hi, uint32_t path,
+                                            uint32_t block, uint32_t cls, uint32_t call,
+                                            float (&z)[4])
 {
     uint32_t x[4];
     philox4x32_10(block, path, cls, call, seed_lo, seed_hi, x);
@@ -173,8 +98,8 @@ HJM_HD void stream_call(uint32_t seed_lo, uint32_t seed_hi, uint32_t path, uint3
 
 // The three factor shocks of step n, computed directly: one call when they lie in one
 // call's four shocks (q % 4 = 0 or 3), two otherwise.
-HJM_HD Normals3 step_normals(uint32_t seed_lo, uint32_t seed_hi, uint32_t path, uint32_t n,
-                             uint32_t stride)
+__device__ __forceinline__ Normals3 step_normals(uint32_t seed_lo, uint32_t seed_hi,
+                                                 uint32_t path, uint32_t n, uint32_t stride)
 {
     const uint32_t b = n / stride, s = n % stride, g = s & 3u, j0 = 3u * (s >> 2);
     const uint32_t c0 = j0 >> 2;

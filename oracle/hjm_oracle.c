@@ -3,7 +3,7 @@
  *
  * Built with -O2 -ffp-contract=off so that every a*b+c below is a rounded multiply
  * followed by a rounded add, as the reference's sequential C++ computes it, and so
- * that the normal generator (which uses explicit fmaf) is bit-reproducible.
+ * that the generator's uniforms (which use an explicit fmaf) are bit-reproducible.
  */
 #include "hjm_oracle.h"
 
@@ -27,13 +27,15 @@
  *   x0..x3 = philox(ctr, key)
  *   (4c, 4c + 1)     = r cos, r sin of (x0, x1)
  *   (4c + 2, 4c + 3) = r cos, r sin of (x2, x3)
- * with, for a pair (a, b): u = ((a >> 8) + 1) * 2^-24 in (0, 1], r = sqrt(-2 ln u), angle
- * 2 pi (b >> 8) 2^-24.
- * ln, sin and cos are the fixed FP32 polynomials below (coefficients fitted by
- * cva_hjm_cuda_amd/csrc/rng_poly_fit.py), evaluated with explicit fmaf so the GPU
- * reproduces them bit for bit. The reference draws cuRAND XORWOW normals instead
- * (simulation_cpu.h:83-90, simulation_gpu.cu:1011-1013); that sequence is closed
- * library behaviour and is replaced, not reproduced.
+ * with, for a pair (a, b): u = fma(float(a), 2^-32, 2^-33) in (0, 1] (float(a) is a rounded
+ * to 24 bits, round to nearest even), t = float(b) 2^-32 in [0, 1] turns, r = sqrt(-2 ln u),
+ * shocks r cos(2 pi t) and r sin(2 pi t). u and t are exact FP32 values, reproduced here
+ * bit for bit; the transform itself is evaluated in FP64 and rounded once to FP32. The
+ * product runs the transform on the GPU's log2 / sqrt / sin / cos instructions, whose
+ * results differ from these by at most 2.6e-7 |z| + 8.8e-7 (the bound the GPU tests
+ * check). The reference draws cuRAND XORWOW normals instead (simulation_cpu.h:83-90,
+ * simulation_gpu.cu:1011-1013); that sequence is closed library behaviour and is replaced,
+ * not reproduced.
  * =================================================================================== */
 
 #define PHILOX_M0 0xD2511F53u
@@ -67,67 +69,16 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
     out[3] = c3;
 }
 
-static inline float f_from_bits(uint32_t b) { float f; memcpy(&f, &b, 4); return f; }
-static inline uint32_t bits_from_f(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
-
-/* ln(u) for u in [2^-24, 1]: u = 2^e m, m in [sqrt(1/2), sqrt(2)], ln m = x + x^2 P(x). */
-float orc_log_f32(float u)
-{
-    static const float P[9] = {
-        -0x1.fffff8p-2f, 0x1.55555p-2f, -0x1.000428p-2f, 0x1.99a402p-3f, -0x1.54274ap-3f,
-        0x1.22709cp-3f, -0x1.0f375cp-3f, 0x1.084ep-3f, -0x1.383d96p-4f};
-    uint32_t b = bits_from_f(u);
-    int e = (int)((b >> 23) & 0xffu) - 127;
-    uint32_t mb = (b & 0x007fffffu) | 0x3f800000u;
-    if (mb > 0x3fb504f3u) { /* m > sqrt(2): halve it */
-        mb -= 0x00800000u;
-        e += 1;
-    }
-    float x = f_from_bits(mb) - 1.0f;
-    float p = P[8];
-    for (int i = 7; i >= 0; --i) p = fmaf(p, x, P[i]);
-    float x2 = x * x;
-    float lnm = fmaf(x2, p, x);
-    const float LN2_HI = 0x1.62e4p-1f;     /* 0.693145751953125, 17 significant bits */
-    const float LN2_LO = 0x1.7f7d1ep-20f;  /* ln2 - LN2_HI, rounded to float */
-    float fe = (float)e;
-    return fmaf(fe, LN2_HI, fmaf(fe, LN2_LO, lnm));
-}
-
-/* sin and cos of 2*pi*k/2^24 for k in [0, 2^24): integer quadrant reduction. */
-void orc_sincos_turn24_f32(uint32_t k24, float* s, float* c)
-{
-    static const float S[5] = {0x1.921fb6p+2f, -0x1.4abbcep+5f, 0x1.466bbap+6f,
-                               -0x1.32ca9ep+6f, 0x1.4bc89ap+5f};
-    static const float C[4] = {-0x1.3bd3ccp+4f, 0x1.03c1dep+6f, -0x1.55c5ep+6f,
-                               0x1.d9d57cp+5f};
-    uint32_t q = (k24 + (1u << 21)) >> 22;              /* nearest quarter turn, 0..4 */
-    int32_t ri = (int32_t)k24 - (int32_t)(q << 22);      /* in [-2^21, 2^21) */
-    float r = (float)ri * 0x1p-24f;                      /* exact, |r| <= 1/8 */
-    float r2 = r * r;
-    float ps = S[4];
-    for (int i = 3; i >= 0; --i) ps = fmaf(ps, r2, S[i]);
-    float sn = r * ps;
-    float pc = C[3];
-    for (int i = 2; i >= 0; --i) pc = fmaf(pc, r2, C[i]);
-    float cs = fmaf(r2, pc, 1.0f);
-    switch (q & 3u) {
-    case 0: *s = sn; *c = cs; break;
-    case 1: *s = cs; *c = -sn; break;
-    case 2: *s = -sn; *c = -cs; break;
-    default: *s = -cs; *c = sn; break;
-    }
-}
-
-/* (z0, z1) = r cos, r sin: r = sqrt(-2 ln u), u = ((a >> 8) + 1) 2^-24, angle 2 pi (b >> 8) 2^-24. */
+/* (z0, z1) = r cos, r sin of the Philox pair (a, b): u = fma(float(a), 2^-32, 2^-33),
+ * t = float(b) 2^-32, r = sqrt(-2 ln u), angle 2 pi t; FP64 transform rounded to FP32. */
 void orc_box_muller(uint32_t a, uint32_t b, float* z0, float* z1)
 {
-    float u = (float)((a >> 8) + 1u) * 0x1p-24f;
-    float r = sqrtf(-2.0f * orc_log_f32(u));
-    float s, c;
-    orc_sincos_turn24_f32(b >> 8, &s, &c);
-    *z0 = r * c;
-    *z1 = r * s;
+    const float u = fmaf((float)a, 0x1p-32f, 0x1p-33f);
+    const float t = (float)b * 0x1p-32f;
+    const double r = sqrt(-2.0 * log((double)u));
+    const double ang = 6.283185307179586476925 * (double)t;
+    *z0 = (float)(r * cos(ang));
+    *z1 = (float)(r * sin(ang));
 }
 
 /* Shocks 4 call .. 4 call + 3 of the class stream (path, block, cls): Philox4x32-10 with

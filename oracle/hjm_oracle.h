@@ -26,10 +26,9 @@ extern "C" {
 #define ORC_TENORS 51
 #define ORC_FACTORS 3
 
-/* ---- normal generator (specification shared with the product) ---------------- */
+/* ---- normal generator (the product's Philox words and uniforms bit for bit; its
+ *      Box-Muller transform within the bound stated in hjm_oracle.c) ---------------- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
-float orc_log_f32(float u);
-void orc_sincos_turn24_f32(uint32_t k24, float* s, float* c);
 void orc_box_muller(uint32_t a, uint32_t b, float* z0, float* z1);
 void orc_stream_call(uint64_t seed, uint32_t path, uint32_t block, uint32_t cls, uint32_t call,
                      float z[4]);

@@ -24,9 +24,6 @@ class Oracle:
         self.lib = lib
         L = lib
         L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
-        L.orc_log_f32.argtypes = [C.c_float]
-        L.orc_log_f32.restype = C.c_float
-        L.orc_sincos_turn24_f32.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.orc_box_muller.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_float),
                                      C.POINTER(C.c_float)]
         L.orc_stream_call.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -72,14 +69,6 @@ class Oracle:
         out = np.zeros(4, np.uint32)
         self.lib.orc_philox4x32_10(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)
         return out
-
-    def log_f32(self, u):
-        return self.lib.orc_log_f32(float(u))
-
-    def sincos_turn24(self, k):
-        s, c = C.c_float(), C.c_float()
-        self.lib.orc_sincos_turn24_f32(int(k), C.byref(s), C.byref(c))
-        return s.value, c.value
 
     def box_muller(self, a, b):
         z0, z1 = C.c_float(), C.c_float()

@@ -135,3 +135,22 @@ def _has_device():
         return hjm.device_count() > 0
     except hjm.HJMError:
         return False
+
+
+@pytest.mark.parametrize("shape", [(16, 499, 3), (15, 500, 3), (16, 500, 2), (16 * 500 * 3,)])
+def test_host_normals_shape_rejected_before_the_library(shape):
+    """Every entry point that takes caller normals checks their shape in Python: the C
+    side copies exactly paths x steps x 3 values from the pointer."""
+    from cva_hjm_cuda_amd.portfolio import synthetic_portfolio
+    m = hjm.benchmark_market()
+    swap = hjm.InterestRateSwap(1e6, 0.025, 5.0)
+    z = np.zeros(shape)
+    with pytest.raises(ValueError):
+        hjm.simulate_reference(m, swap, 16, normals=z)
+    with pytest.raises(ValueError):
+        hjm.cva_greeks(m, swap, 16, normals=z)
+    pf = synthetic_portfolio(4, 2, seed=1)
+    with pytest.raises(ValueError):
+        hjm.PortfolioPlan(m, pf, 16, horizon=5.0, normals=z)
+    with pytest.raises(ValueError):
+        hjm.portfolio_cva_greeks(m, pf, 16, horizon=5.0, normals=z)

@@ -1,8 +1,8 @@
 """GPU parity of the portfolio semantics with the oracle, and its execution modes.
 
-FP64 on identical normals: per-path netting-set exposures, EE, CVA and SE within 1e-10
-relative of the oracle; PFE the exact order statistic; in-kernel Philox bit-identical to
-host normals; hipGraph replays bit-identical to eager runs (several replays with eager
+FP64 on identical normals (the generator's own, hjm_generate_normals): per-path
+netting-set exposures, EE, CVA and SE within 1e-10 relative of the oracle; PFE the exact
+order statistic; in-kernel Philox bit-identical to the same normals supplied from the host; hipGraph replays bit-identical to eager runs (several replays with eager
 work in between); tuned persistent grids bit-identical to the default grid; FP32 versus
 FP64 at the batched configuration's size; shard additivity.
 """
@@ -40,7 +40,7 @@ def small(oracle):
     _flush("[gpu] portfolio oracle run: 128 paths x 30y, 64 swaps, 8 sets")
     m = hjm.benchmark_market()
     pf = hjm.synthetic_portfolio(64, 8, seed=7)
-    z = oracle.normals(31, 128, 3000)
+    z = hjm.generate_normals(31, 128, 3000)
     ex = oracle.portfolio_exposure(m.spot_rates, m.drifts, m.volatilities, 30.0, 0.5, 0.01, z, pf)
     return m, pf, z, ex
 

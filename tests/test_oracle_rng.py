@@ -2,7 +2,8 @@
 
 * Philox4x32-10 against the Random123 known-answer vectors and against rocRAND's
   open-source implementation (fixture tests/golden/philox_vectors.json).
-* The FP32 polynomial ln / sin / cos against float64 numpy, in float32 ulp.
+* Box-Muller: the exact FP32 uniforms of a Philox pair, and the pair as the polar point
+  (sqrt(-2 ln u), 2 pi t) to FP32 rounding, including the ends of both ranges.
 * The normals: moments and a Kolmogorov-Smirnov test on a large sample.
 * The class-stream layout, restated from the Philox and Box-Muller primitives.
 """
@@ -38,40 +39,45 @@ def test_philox_matches_rocrand_vectors(oracle):
         assert list(int(x) for x in oracle.philox(v["ctr"], v["key"])) == v["out"]
 
 
-def _ulp(x):
-    return np.spacing(np.float32(abs(x))).astype(np.float64)
+def _uniforms(a, b):
+    """The generator's exact FP32 uniforms: u = fma(float(a), 2^-32, 2^-33), t = float(b)
+    2^-32 (float(x): round to nearest even; the fma's one rounding is that of an exact
+    float64 sum narrowed to float32)."""
+    fa = float(np.float32(np.uint32(a)))
+    u = float(np.float32(fa * 2.0**-32 + 2.0**-33))
+    t = float(np.float32(np.uint32(b))) * 2.0**-32
+    return u, t
 
 
-def test_log_f32_accuracy(oracle):
-    rng = np.random.default_rng(1)
-    ks = np.concatenate([np.arange(1, 4000), rng.integers(1, 2**24 + 1, 20000), [2**24]])
-    worst = 0.0
-    for k in ks:
-        u = np.float32(k * 2.0**-24)
-        got = oracle.log_f32(u)
-        ref = math.log(float(u))
-        if ref == 0.0:
-            assert got == 0.0
-            continue
-        worst = max(worst, abs(got - ref) / _ulp(ref))
-    assert worst <= 2.0, worst
-
-
-def test_sincos_turn24_accuracy(oracle):
-    rng = np.random.default_rng(2)
-    ks = np.concatenate([np.arange(0, 2**24, 4099), rng.integers(0, 2**24, 20000),
-                         [0, 2**22, 2**23, 3 * 2**22, 2**21, 2**24 - 1]])
-    worst = 0.0
-    for k in ks:
-        s, c = oracle.sincos_turn24(int(k))
-        a = 2 * math.pi * int(k) * 2.0**-24
-        for got, ref in ((s, math.sin(a)), (c, math.cos(a))):
-            # absolute error in units of the ulp of 1/8 near zero crossings
-            worst = max(worst, abs(got - ref) / max(_ulp(ref), 2.0**-27))
-    assert worst <= 3.0, worst
-    # exact quarter turns
-    assert oracle.sincos_turn24(0) == (0.0, 1.0)
-    assert oracle.sincos_turn24(2**22) == (1.0, -0.0) or oracle.sincos_turn24(2**22)[0] == 1.0
+def test_box_muller_polar_form(oracle):
+    """Each pair is the point of radius sqrt(-2 ln u) at angle 2 pi t, to FP32 rounding,
+    for random words and the ends of both ranges."""
+    rng = np.random.default_rng(3)
+    words = [int(x) for x in rng.integers(0, 2**32, 6000)]
+    edges = [0, 1, 127, 128, 129, 255, 256, 2**24 - 1, 2**24, 2**24 + 1, 2**31, 2**32 - 129,
+             2**32 - 128, 2**32 - 1]
+    pairs = list(zip(words[::2], words[1::2])) + [(a, b) for a in edges for b in edges]
+    worst_r = worst_a = 0.0
+    for a, b in pairs:
+        z0, z1 = oracle.box_muller(a, b)
+        u, t = _uniforms(a, b)
+        assert 0.0 < u <= 1.0 and 0.0 <= t <= 1.0
+        r = math.sqrt(-2.0 * math.log(u))
+        worst_r = max(worst_r, abs(math.hypot(z0, z1) - r) / max(r, 1e-30))
+        if r > 1e-3:
+            d = math.atan2(z1, z0) - 2.0 * math.pi * t
+            d = (d + math.pi) % (2.0 * math.pi) - math.pi
+            worst_a = max(worst_a, abs(d))
+    assert worst_r < 2.0**-23, worst_r      # each coordinate rounded once to FP32
+    assert worst_a < 2.0**-21, worst_a
+    # the ends of the radius: the smallest u is 2^-33 (r = sqrt(66 ln 2)); the largest
+    # words round to u = 1 and give the origin
+    z0, z1 = oracle.box_muller(0, 0)
+    assert z1 == 0.0 and z0 == np.float32(math.sqrt(66.0 * math.log(2.0)))
+    assert all(v == 0.0 for v in oracle.box_muller(2**32 - 1, 12345))
+    # quarter turns: t = 1/4 puts the whole radius on the sine
+    z0, z1 = oracle.box_muller(2**31, 2**30)
+    assert abs(z0) < 1e-7 * z1 and z1 == np.float32(math.sqrt(-2.0 * math.log(0.5 + 2.0**-33)))
 
 
 def test_normals_distribution(oracle):
