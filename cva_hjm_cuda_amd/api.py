@@ -30,7 +30,9 @@ RNG_HOST = 1
 STATUS = {
     0: "ok", 1: "null pointer", 2: "invalid argument", 3: "inconsistent time grid",
     4: "device error", 5: "allocation failed", 6: "launch failed", 7: "unsupported",
+    8: "size beyond the call's fixed limit",
 }
+NOALLOC_MAX_PATHS = 32768   # HJM_NOALLOC_MAX_PATHS
 
 
 class HJMError(RuntimeError):
@@ -132,6 +134,8 @@ def _declare(lib):
                                        C.POINTER(_Results)]
     lib.hjm_cva_greeks_reference.argtypes = [C.POINTER(_Market), C.POINTER(_Swap),
                                              C.POINTER(_Config), C.POINTER(_Greeks)]
+    lib.hjm_simulate_reference_noalloc.argtypes = [C.POINTER(_Market), C.POINTER(_Swap),
+                                                   C.POINTER(_Config), C.c_void_p, C.c_void_p]
     lib.hjm_portfolio_plan_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(_Market),
                                               C.POINTER(_Portfolio), C.POINTER(_Config),
                                               C.POINTER(_Tuning)]
@@ -304,6 +308,23 @@ def simulate_reference(market: Market, swap: InterestRateSwap, paths: int, dt: f
     fn = lib.hjm_simulate_reference_blockwise if blockwise else lib.hjm_simulate_reference
     _check(fn(C.byref(mk), C.byref(sw), C.byref(cfg), C.byref(res)), fn.__name__)
     return ReferenceResult(ee, pfe, float(cva[0]), float(se[0]), sums, tm, nx, ny, ex)
+
+
+def simulate_reference_noalloc(market: Market, swap: InterestRateSwap, paths: int,
+                               d_sums: int, stream: int = 0, dt: float = 0.01,
+                               precision: int = FP64, seed: int = 1234, hazard: float = 0.02,
+                               recovery: float = 0.4, device: int = 0, path_offset: int = 0,
+                               path_kernel: int = KERNEL_AUTO) -> None:
+    """Enqueue hjm_simulate_reference_noalloc on ``stream`` (a hipStream_t as an int, 0 =
+    the null stream): the 53 raw sums of the run go to the device address ``d_sums``; the
+    library allocates nothing and does not wait for the work."""
+    mk = market._c()
+    sw = _Swap(swap.notional, swap.K, swap.expiry, swap.dtau)
+    cfg = _config(paths, dt, precision, RNG_PHILOX, seed, None, hazard, recovery, 0.0, device,
+                  path_offset, path_kernel)
+    _check(library().hjm_simulate_reference_noalloc(C.byref(mk), C.byref(sw), C.byref(cfg),
+                                                    C.c_void_p(d_sums), C.c_void_p(stream)),
+           "hjm_simulate_reference_noalloc")
 
 
 @dataclass

@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "hjm_common.h"
@@ -99,7 +100,10 @@ void plan_release(hjm_plan p)
     delete p;
 }
 
-int plan_init(hjm_plan p, const hjm_market* market, const hjm_swap* swap, const hjm_config* cfg)
+// Argument checks of a reference-semantics run (plans and the no-allocation entry point)
+// and its time grid.
+int check_ref_config(const hjm_market* market, const hjm_swap* swap, const hjm_config* cfg,
+                     int* stride, int* S, int* D)
 {
     HJM_TRY(check_market(market));
     if (!swap || !cfg) return HJM_ERR_NULL;
@@ -113,9 +117,47 @@ int plan_init(hjm_plan p, const hjm_market* market, const hjm_swap* swap, const 
         return HJM_ERR_ARG;
     if (cfg->pfe_quantile > 1.0) return HJM_ERR_ARG;
     if (cfg->path_kernel < HJM_KERNEL_AUTO || cfg->path_kernel > HJM_KERNEL_MATRIX) return HJM_ERR_ARG;
-    int stride, S, D;
-    HJM_TRY(ref_grid(swap->expiry, swap->dtau, cfg->dt, &stride, &S, &D));
+    HJM_TRY(ref_grid(swap->expiry, swap->dtau, cfg->dt, stride, S, D));
     if ((int64_t)cfg->paths * kTenors > (1ll << 40)) return HJM_ERR_ARG;
+    return HJM_OK;
+}
+
+// Kernel parameters of a checked run: step constants, grid, exposure/CVA parameters.
+void fill_ref_params(const hjm_market* market, const hjm_swap* swap, const hjm_config* cfg,
+                     int stride, int S, int D, RefStepConsts<double>& kd, RefStepConsts<float>& kf,
+                     RefGrid& g, ExposureParams& e)
+{
+    fill_consts(kd, market, cfg->dt, swap->dtau);
+    fill_consts(kf, market, cfg->dt, swap->dtau);
+    g.stride = stride;
+    g.nsteps = S;
+    g.ndates = D;
+    g.paths = cfg->paths;
+    g.path0 = cfg->path_offset;
+    g.seed_lo = (uint32_t)cfg->seed;
+    g.seed_hi = (uint32_t)(cfg->seed >> 32);
+    g.dt = cfg->dt;
+    for (int j = 0; j < kTenors; ++j) {
+        // block 0 keeps the initial curve for `stride` steps (simulation_cpu.h:212-213, 235)
+        double a = 0.0;
+        const double f0 = (cfg->precision == HJM_FP64) ? market->spot_rates[j]
+                                                       : (double)(float)market->spot_rates[j];
+        for (int s = 0; s < stride; ++s) a += f0;
+        g.a0[j] = a;
+    }
+    e.paths = cfg->paths;
+    e.ndates = D;
+    e.notional = swap->notional;
+    e.K = swap->fixed_rate;
+    e.m = 1.0 / swap->dtau;
+    for (int j = 0; j < kTenors; ++j) e.accrual[j] = market->accrual[j];
+    cva_weights(kTenors, swap->dtau, cfg->hazard_rate, cfg->recovery, e.w);
+}
+
+int plan_init(hjm_plan p, const hjm_market* market, const hjm_swap* swap, const hjm_config* cfg)
+{
+    int stride, S, D;
+    HJM_TRY(check_ref_config(market, swap, cfg, &stride, &S, &D));
     const bool mfma_ok = cfg->precision == HJM_FP32_KAHAN && cfg->rng == HJM_RNG_PHILOX &&
                          stride == kMfmaStride;
     if (cfg->path_kernel == HJM_KERNEL_MATRIX && !mfma_ok) return HJM_ERR_UNSUPPORTED;
@@ -128,34 +170,8 @@ int plan_init(hjm_plan p, const hjm_market* market, const hjm_swap* swap, const 
     p->nblocks = (cfg->paths + kBlock - 1) / kBlock;
     p->do_pfe = cfg->pfe_quantile > 0.0;
     p->pfe_rank = quantile_rank(cfg->paths, cfg->pfe_quantile);
-
-    fill_consts(p->kd, market, cfg->dt, swap->dtau);
-    fill_consts(p->kf, market, cfg->dt, swap->dtau);
-    RefGrid& g = p->grid;
-    g.stride = stride;
-    g.nsteps = S;
-    g.ndates = D;
-    g.paths = cfg->paths;
-    g.path0 = cfg->path_offset;
-    g.seed_lo = (uint32_t)cfg->seed;
-    g.seed_hi = (uint32_t)(cfg->seed >> 32);
-    g.dt = cfg->dt;
-    for (int j = 0; j < kTenors; ++j) {
-        // block 0 keeps the initial curve for `stride` steps (simulation_cpu.h:212-213, 235)
-        double a = 0.0;
-        const double f0 = (p->precision == HJM_FP64) ? market->spot_rates[j]
-                                                     : (double)(float)market->spot_rates[j];
-        for (int s = 0; s < stride; ++s) a += f0;
-        g.a0[j] = a;
-    }
-    ExposureParams& e = p->ep;
-    e.paths = cfg->paths;
-    e.ndates = D;
-    e.notional = swap->notional;
-    e.K = swap->fixed_rate;
-    e.m = 1.0 / swap->dtau;
-    for (int j = 0; j < kTenors; ++j) e.accrual[j] = market->accrual[j];
-    cva_weights(kTenors, swap->dtau, cfg->hazard_rate, cfg->recovery, e.w);
+    fill_ref_params(market, swap, cfg, stride, S, D, p->kd, p->kf, p->grid, p->ep);
+    const RefGrid& g = p->grid;
 
     DeviceGuard guard;
     HJM_TRY(guard.set(cfg->device));
@@ -309,6 +325,8 @@ const char* hjm_status_string(int status)
     case HJM_ERR_ALLOC: return "device allocation failed";
     case HJM_ERR_LAUNCH: return "kernel launch or execution failed";
     case HJM_ERR_UNSUPPORTED: return "unsupported option combination";
+    case HJM_ERR_LIMIT: return "size beyond a fixed limit of the call (the no-allocation "
+                               "entry point runs at most HJM_NOALLOC_MAX_PATHS paths)";
     default: return "unknown status";
     }
 }
@@ -766,3 +784,74 @@ int hjm_calculate_exposure_multi_gpu_f32(float* expected_exposure, float notiona
 }
 
 }  // extern "C"
+
+// ---- no-allocation variant of hjm_simulate_reference ---------------------------------
+// Every intermediate buffer of the run (numeraire pairs num_x, num_y [51][paths] and the
+// workgroup partials [blocks][53], all FP64) lives in this static device array, sized for
+// HJM_NOALLOC_MAX_PATHS paths: 2 * 51 * 32768 + 128 * 53 doubles = 26.8 MB, held on every
+// device the library runs on until the process ends. Nothing in it must start at zero: the
+// path kernel writes every numeraire slot the exposure kernel reads (1 .. ndates), and the
+// exposure kernel writes every partial row the column sums read.
+namespace {
+
+constexpr int64_t kNoallocBlocks = (HJM_NOALLOC_MAX_PATHS + kBlock - 1) / kBlock;
+constexpr size_t kScratchDoubles =
+    2 * (size_t)kTenors * HJM_NOALLOC_MAX_PATHS + (size_t)kNoallocBlocks * kStatSlots;
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// The array's address on each device, looked up once per device with that device current
+// (every device holds its own copy at its own address). On the host `scratch` names a
+// stand-in, never device memory: kernels and copies take these addresses only.
+constexpr int kMaxDevices = 64;
+std::once_flag g_scratch_once[kMaxDevices];
+double* g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+
+int device_scratch(int device, double** out)
+{
+    if (device < 0 || device >= kMaxDevices) return HJM_ERR_ARG;
+    std::call_once(g_scratch_once[device], [device] {
+        void* ptr = nullptr;
+        g_scratch_err[device] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        g_scratch_ptr[device] = static_cast<double*>(ptr);
+    });
+    if (g_scratch_err[device] != hipSuccess || !g_scratch_ptr[device]) return HJM_ERR_DEVICE;
+    *out = g_scratch_ptr[device];
+    return HJM_OK;
+}
+
+}  // namespace
+
+extern "C" int hjm_simulate_reference_noalloc(const hjm_market* market, const hjm_swap* swap,
+                                              const hjm_config* cfg, double* d_sums,
+                                              void* stream)
+{
+    int stride, S, D;
+    HJM_TRY(check_ref_config(market, swap, cfg, &stride, &S, &D));
+    if (!d_sums) return HJM_ERR_NULL;
+    if (cfg->paths > HJM_NOALLOC_MAX_PATHS) return HJM_ERR_LIMIT;
+    if (cfg->rng != HJM_RNG_PHILOX || cfg->path_kernel == HJM_KERNEL_MATRIX)
+        return HJM_ERR_UNSUPPORTED;
+    RefStepConsts<double> kd{};
+    RefStepConsts<float> kf{};
+    RefGrid g{};
+    ExposureParams e{};
+    fill_ref_params(market, swap, cfg, stride, S, D, kd, kf, g, e);
+
+    DeviceGuard guard;
+    HJM_TRY(guard.set(cfg->device));
+    double* base = nullptr;
+    HJM_TRY(device_scratch(cfg->device, &base));
+    const int64_t P = cfg->paths, nblocks = (P + kBlock - 1) / kBlock;
+    double* nx = base;
+    double* ny = base + (size_t)kTenors * P;
+    double* partials = base + 2 * (size_t)kTenors * P;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    HJM_HIP(cfg->precision == HJM_FP64 ? launch_ref_paths_f64(kd, g, nullptr, nx, ny, st)
+                                       : launch_ref_paths_f32k(kf, g, nullptr, nx, ny, st),
+            HJM_ERR_LAUNCH);
+    HJM_HIP(launch_ref_exposure(e, nx, ny, nullptr, partials, st), HJM_ERR_LAUNCH);
+    HJM_HIP(launch_column_sum(partials, nblocks, kStatSlots, d_sums, st), HJM_ERR_LAUNCH);
+    return HJM_OK;
+}

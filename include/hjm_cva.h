@@ -35,7 +35,9 @@ enum hjm_status_code {
     HJM_ERR_DEVICE = 4,      /* no HIP device, or a HIP runtime error */
     HJM_ERR_ALLOC = 5,       /* device memory allocation failed */
     HJM_ERR_LAUNCH = 6,      /* kernel launch or execution failed */
-    HJM_ERR_UNSUPPORTED = 7  /* option combination not implemented */
+    HJM_ERR_UNSUPPORTED = 7, /* option combination not implemented */
+    HJM_ERR_LIMIT = 8        /* size beyond a fixed limit of the call
+                                (hjm_simulate_reference_noalloc) */
 };
 
 enum hjm_precision {
@@ -125,6 +127,25 @@ int hjm_plan_execute(hjm_plan plan, hjm_results* results);
 void hjm_plan_destroy(hjm_plan plan);
 int hjm_simulate_reference(const hjm_market* market, const hjm_swap* swap,
                            const hjm_config* config, hjm_results* results);
+
+/* No counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * work of hjm_simulate_reference with the one-path-per-lane path kernel (FP64, or FP32 +
+ * compensated) and in-kernel Philox, enqueued on the caller's stream without any
+ * allocation or synchronisation. Every intermediate buffer lives in the library's static
+ * device scratch array (26.8 MB per device, held until the process ends).
+ *   d_sums  device pointer to 53 doubles: the raw sums of hjm_results.sums (EE numerators
+ *           sum_p E_pj for j = 0..50, sum_p CVA_p, sum_p CVA_p^2), written by the enqueued
+ *           work; EE = sums / paths, and shards with disjoint path_offset add up.
+ *   stream  hipStream_t on config->device (NULL: the null stream).
+ * Limits: config->paths <= HJM_NOALLOC_MAX_PATHS per call (HJM_ERR_LIMIT beyond: run
+ * larger jobs as several calls with path_offset and add the sums); rng HJM_RNG_PHILOX
+ * and path_kernel AUTO or VECTOR (HJM_ERR_UNSUPPORTED otherwise); no PFE (pfe_quantile is
+ * ignored). All calls on one device share the scratch array: calls that overlap in time
+ * on one device (different streams or threads) race; serialise them on one stream. */
+#define HJM_NOALLOC_MAX_PATHS 32768
+int hjm_simulate_reference_noalloc(const hjm_market* market, const hjm_swap* swap,
+                                   const hjm_config* config, double* d_sums, void* stream);
 
 /* CVA sensitivities by the pathwise adjoint (no counterpart in the reference; its
  * roadmap names them, doc/TODO:6,14). Forward pass as hjm_simulate_reference (use

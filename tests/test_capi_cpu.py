@@ -35,7 +35,7 @@ def test_library_exports_all_declared_symbols():
 
 def test_status_strings():
     lib = hjm.library()
-    for code in range(8):
+    for code in range(9):
         assert lib.hjm_status_string(code)
     assert b"grid" in lib.hjm_status_string(3)
 
