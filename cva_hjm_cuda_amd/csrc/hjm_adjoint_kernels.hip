@@ -5,14 +5,22 @@
 // steps. The step Jacobian T = I + (dt/dtau) Dfwd does not depend on the state, so
 // the sweep needs no forward states between exposure dates: the forward kernel's
 // numeraire pairs at the dates (one per 50 steps) are the only checkpoint, and the
-// factor shocks are regenerated in reverse from the counter-based generator.
+// factor shocks are read back from the forward pass or regenerated in reverse from the
+// counter-based generator.
 //
 // Backward recursion (n = N..1, injections at the exposure dates n_i = (i-1) stride):
 //   mu(n) = mu(n+1) + gA_i e_i,   lambda(n) = T^T lambda(n+1) + mu(n) + gx_i e_i
 //   dJ/dsigma_kj += lambda_j(n) sqrt(dt) phi_k(n),  dJ/dm_j += dt lambda_j(n)
-// The sweep runs twice per path (pass 0: factors 0, 1; pass 1: factor 2, drift and the
-// curve), which keeps the per-lane state at 204 FP32 registers. Per-lane FP32 partials
-// cover one 50-step block, then are reduced across the wave and accumulated in FP64.
+// Two lanes share a path: the lower half-wave holds tenors 0..25, the upper half tenors
+// 25..50 (its tenor 25 a copy received from the lower lane every step: lambda_26 reads
+// lambda_25, and no lower tenor reads an upper one), so lambda, mu and the three loading
+// gradients take 130 registers per lane and the recurrence runs once. The drift gradient
+// needs no per-step work: within a date block mu is constant, so
+//   sum over the block's steps of lambda(n) = A lambda_e + B mu,
+//   A = sum_{k<S} (T^T)^k,  B = sum_{r<S-1} (S-1-r) (T^T)^r,
+// with lambda_e the adjoint at the block's end; the kernel sums lambda_e over blocks and
+// paths and the host applies A to it and B to the mu injections (their sums over paths,
+// weighted by the number of blocks each is live in).
 #include "hjm_common.h"
 #include "hjm_kernels.h"
 #include "hjm_rng.h"
@@ -53,185 +61,254 @@ __global__ __launch_bounds__(kBlock) void ref_adjoint_seed_kernel(const Exposure
     }
 }
 
+// out[r] = sum of row r of an FP32 [rows][n] array, in FP64 and a fixed order (one
+// workgroup per row).
+__global__ __launch_bounds__(kBlock) void row_sum_f32_kernel(const float* __restrict__ x,
+                                                             int64_t n, double* __restrict__ out)
+{
+    __shared__ double red[kBlock / kWave];
+    const float* row = x + (int64_t)blockIdx.x * n;
+    double s = 0.0;
+    for (int64_t k = threadIdx.x; k < n; k += kBlock) s += (double)row[k];
+    s = wave_sum(s);
+    if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / kWave; ++w) t += red[w];
+        out[blockIdx.x] = t;
+    }
+}
+
 struct AdjGrid {
-    int stride, nsteps, ndates, pass;
+    int stride, nsteps, ndates, pad;
     int64_t paths, path0;
     uint32_t seed_lo, seed_hi;
     float c, omc, opc, sqrt_dt;
 };
 
-// Sum of a lane value over the wave in FP64, fixed butterfly order.
-__device__ __forceinline__ double wave_sum_f(float v) { return wave_sum((double)v); }
+constexpr int kHalfSlots = 26;               // tenor slots per lane (half a curve + 1)
+constexpr int kAdjPaths = kBlock / 2;        // paths per workgroup: two lanes per path
+constexpr int kAdjKinds = 5;                 // g0, g1, g2, sum of lambda_e, d spot
+
+// FP32 sum over each half-wave (quad butterflies, row shifts by 4 and 8, row broadcast 15
+// into rows 1 and 3): lane 31 receives the sum of lanes 0..31, lane 63 that of 32..63.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ float dpp_add_f(float v)
+{
+    const int m = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), kCtrl, kRowMask, 0xf, false);
+    return v + __builtin_bit_cast(float, m);
+}
+__device__ __forceinline__ float half_wave_sum(float v)
+{
+    v = dpp_add_f<0xb1, 0xf>(v);   // quad_perm [1, 0, 3, 2]
+    v = dpp_add_f<0x4e, 0xf>(v);   // quad_perm [2, 3, 0, 1]
+    v = dpp_add_f<0x114, 0xf>(v);  // row_shr:4
+    v = dpp_add_f<0x118, 0xf>(v);  // row_shr:8
+    v = dpp_add_f<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    return v;
+}
+
+// The shocks of the steps of a 16-step group, shared by a path's two lanes: the lower lane
+// makes the calls of classes 0 and 1, the upper lane those of classes 2 and 3, each into
+// the path's LDS column (the layout of fill_group).
+__device__ __forceinline__ void fill_group_half(uint32_t seed_lo, uint32_t seed_hi, uint32_t path,
+                                                uint32_t block, int m, int half, float* col, int ld)
+{
+#pragma unroll 1
+    for (int g = 2 * half; g < 2 * half + 2; ++g)
+#pragma unroll 1
+        for (int cc = 0; cc < 3; ++cc) {
+            float v[4];
+            stream_call(seed_lo, seed_hi, path, block, (uint32_t)g, (uint32_t)(3 * m + cc), v);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) col[(12 * g + 4 * cc + e) * ld] = v[e];
+        }
+}
 
 // kSrc: 0 regenerate the in-kernel normals, 1 host-supplied FP64 [path][step][3],
 // 2 the forward pass's stored FP32 normals [step][3][path].
-template <int kPass, int kSrc>
-__global__ __launch_bounds__(kBlock) void ref_adjoint_sweep_kernel(const AdjGrid g,
-                                                                   const double* __restrict__ normals,
-                                                                   const float* __restrict__ zs,
-                                                                   const float* __restrict__ gx,
-                                                                   const float* __restrict__ gA,
-                                                                   double* __restrict__ partials)
+// partials[workgroup][5 * 51]: loading gradients g0, g1, g2 (tenor-major within each),
+// sum over blocks of lambda_e, and d J / d spot, each summed over the workgroup's paths.
+template <int kSrc>
+__global__ __launch_bounds__(kBlock) void ref_adjoint_kernel(const AdjGrid g,
+                                                             const double* __restrict__ normals,
+                                                             const float* __restrict__ zs,
+                                                             const float* __restrict__ gx,
+                                                             const float* __restrict__ gA,
+                                                             double* __restrict__ partials)
 {
-    // per wave FP64 accumulators: pass 0 -> g0, g1 (102); pass 1 -> g2, gm, dspot (153)
-    constexpr int kAcc = 3 * kTenors;
-    __shared__ double acc[kBlock / kWave][kAcc];
-    // each lane's column of regenerated shocks (16 steps at a time)
-    __shared__ float zbuf[kSrc == 0 ? kGroupShocks * kBlock : 1];
-    float* col = zbuf + (kSrc == 0 ? threadIdx.x : 0);
+    __shared__ double acc[kBlock / kWave][2][kAdjKinds][kHalfSlots];
+    __shared__ float lacc[kHalfSlots][kBlock];  // per lane: sum over blocks of lambda_e
+    __shared__ float zbuf[kSrc == 0 ? kGroupShocks * kAdjPaths : 1];
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-    for (int k = lane; k < kAcc; k += kWave) acc[wave][k] = 0.0;
+    const int half = lane >> 5;                      // 0: tenors 0..25, 1: tenors 25..50
+    const int pslot = wave * 32 + (lane & 31);       // the path's index in the workgroup
+    for (int k = threadIdx.x; k < (kBlock / kWave) * 2 * kAdjKinds * kHalfSlots; k += kBlock)
+        (&acc[0][0][0][0])[k] = 0.0;
+#pragma unroll
+    for (int i = 0; i < kHalfSlots; ++i) lacc[i][threadIdx.x] = 0.f;
     __syncthreads();
 
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * kAdjPaths + pslot;
     const bool active = i < g.paths;
-    const int64_t P = g.paths;
-    const uint32_t pid = (uint32_t)(g.path0 + (active ? i : 0));
-    float cv = g.c, mcv = -g.c, omc = g.omc, opc = g.opc;
-    asm volatile("" : "+v"(cv), "+v"(mcv), "+v"(omc), "+v"(opc));
+    const int64_t P = g.paths, ic = active ? i : 0;
+    const uint32_t pid = (uint32_t)(g.path0 + ic);
+    float* col = zbuf + (kSrc == 0 ? pslot : 0);
+    const bool upper = half == 1;
+    // the coefficients of slots 25 and 24: tenor 50 (1 + c, and -c on tenor 49's row) in
+    // the upper half, an interior tenor (1 - c, 0) in the lower half
+    float cv = g.c, omc = g.omc, a25 = upper ? g.opc : g.omc, b24 = upper ? -g.c : 0.f;
+    asm volatile("" : "+v"(cv), "+v"(omc), "+v"(a25), "+v"(b24));
+    const int tbase = 25 * half;   // tenor of slot 0
 
-    float lam[kTenors], mu[kTenors], ga[kTenors], gb[kTenors];
+    float lam[kHalfSlots], mu[kHalfSlots], g0[kHalfSlots], g1[kHalfSlots], g2[kHalfSlots];
 #pragma unroll
-    for (int j = 0; j < kTenors; ++j) lam[j] = mu[j] = ga[j] = gb[j] = 0.f;
+    for (int k = 0; k < kHalfSlots; ++k) lam[k] = mu[k] = g0[k] = g1[k] = g2[k] = 0.f;
 
+    // the half-wave sums of a lane family go to this wave's FP64 accumulators (one lane per
+    // half adds, always the same lane: a fixed order)
+    auto flush = [&](float (&v)[kHalfSlots], int kind) {
+#pragma unroll
+        for (int k = 0; k < kHalfSlots; ++k) v[k] = half_wave_sum(v[k]);
+        if ((lane & 31) == 31) {
+#pragma unroll
+            for (int k = 0; k < kHalfSlots; ++k) atomicAdd(&acc[wave][half][kind][k], (double)v[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kHalfSlots; ++k) v[k] = 0.f;
+    };
+
+    // one reverse Euler step with the step's normals
+    auto step = [&](float z0, float z1, float z2) {
+        const float s0 = g.sqrt_dt * z0, s1 = g.sqrt_dt * z1, s2 = g.sqrt_dt * z2;
+#pragma unroll
+        for (int k = 0; k < kHalfSlots; ++k) {
+            op_fmac_vv(g0[k], lam[k], s0);
+            op_fmac_vv(g1[k], lam[k], s1);
+            op_fmac_vv(g2[k], lam[k], s2);
+        }
+        // the upper lanes take tenor 25 from the lower lanes before it changes
+        const uint32_t l25 = __builtin_bit_cast(uint32_t, lam[kHalfSlots - 1]);
+        const float recv = __builtin_bit_cast(float, __builtin_amdgcn_permlane32_swap(l25, l25, false, false)[0]);
+        lam[0] = upper ? recv : lam[0];
+        // lambda <- T^T lambda + mu, descending so lambda_{k-1} is still old:
+        //   slot 25: a25 l25 + c l24 + mu25;  slot 24: (1-c) l24 + c l23 + b24 l25 + mu24
+        //   1 <= k <= 23: (1-c) l_k + c l_{k-1} + mu_k;  slot 0: (1-c) l0 + mu0
+        float t25, t24;
+        op_fma_vvv(t25, cv, lam[24], mu[25]);
+        op_fma_vvv(t24, cv, lam[23], mu[24]);
+        op_fma_vvv(t24, b24, lam[25], t24);
+        op_fma_inplace(lam[25], a25, t25);
+        op_fma_inplace(lam[24], omc, t24);
+        constexpr int G = 4;
+#pragma unroll
+        for (int k0 = 23; k0 >= 1; k0 -= G) {
+            float t[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                if (k0 - u >= 1) op_fma_vvv(t[u], cv, lam[k0 - u - 1], mu[k0 - u]);
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                if (k0 - u >= 1) op_fma_inplace(lam[k0 - u], omc, t[u]);
+        }
+        op_fma_inplace(lam[0], omc, mu[0]);
+    };
+
+    const int S = g.stride;
+    constexpr int kPre = 5;   // stored normals: steps read ahead (the stride is a multiple)
+    float zq[kPre][3];
+    auto loadz = [&](int n, float (&d)[3]) {
+        const float* q = zs + (int64_t)n * 3 * P + ic;
+        d[0] = q[0];
+        d[1] = q[P];
+        d[2] = q[2 * P];
+    };
+    if constexpr (kSrc == 2) {
+#pragma unroll
+        for (int u = 0; u < kPre; ++u) loadz((g.ndates - 1) * S + S - 1 - u, zq[u]);
+    }
+    const int whole = S / kGroupSteps * kGroupSteps;
     for (int b = g.ndates - 1; b >= 1; --b) {
-        // exposure date n = b * stride records slot b + 1
-        const int slot = b + 1;
-        const float sx = active ? gx[(int64_t)slot * P + i] : 0.f;
-        const float sa = active ? gA[(int64_t)slot * P + i] : 0.f;
+        // exposure date n = b * stride records slot b + 1 (tenor b + 1 >= 2)
+        const int slot = b + 1 - tbase;
+        const float sx = active ? gx[(int64_t)(b + 1) * P + ic] : 0.f;
+        const float sa = active ? gA[(int64_t)(b + 1) * P + ic] : 0.f;
 #pragma unroll
-        for (int j = 1; j < kTenors; ++j) {
-            if (j == slot) {
-                lam[j] += sx + sa;
-                mu[j] += sa;
+        for (int k = 0; k < kHalfSlots; ++k) {
+            if (k == slot && !(upper && k == 0)) {
+                lam[k] += sx + sa;
+                mu[k] += sa;
             }
         }
-        // one reverse Euler step with the step's normals
-        auto step = [&](float z0, float z1, float z2) {
-            (void)z0;
-            (void)z1;
-            (void)z2;
-            if constexpr (kPass == 0) {
-                const float s0 = g.sqrt_dt * z0, s1 = g.sqrt_dt * z1;
 #pragma unroll
-                for (int j = 0; j < kTenors; ++j) {
-                    op_fmac_vv(ga[j], lam[j], s0);
-                    op_fmac_vv(gb[j], lam[j], s1);
-                }
-            } else {
-                const float s2 = g.sqrt_dt * z2;
-#pragma unroll
-                for (int j = 0; j < kTenors; ++j) {
-                    op_fmac_vv(ga[j], lam[j], s2);
-                    op_add_inplace(gb[j], lam[j]);
-                }
-            }
-            // lambda <- T^T lambda + mu, descending so lambda_{j-1} is still old:
-            //   j=50: (1+c) l50 + c l49 + mu50;  j=49: (1-c) l49 + c l48 - c l50 + mu49
-            //   1<=j<=48: (1-c) l_j + c l_{j-1} + mu_j;  j=0: (1-c) l0 + mu0
-            float t50, t49;
-            op_fma_vvv(t50, cv, lam[49], mu[50]);
-            op_fma_vvv(t49, cv, lam[48], mu[49]);
-            op_fma_vvv(t49, mcv, lam[50], t49);
-            op_fma_inplace(lam[50], opc, t50);
-            op_fma_inplace(lam[49], omc, t49);
-            constexpr int G = 4;
-#pragma unroll
-            for (int j0 = 48; j0 >= 1; j0 -= G) {
-                float t[G];
-#pragma unroll
-                for (int u = 0; u < G; ++u)
-                    if (j0 - u >= 1) op_fma_vvv(t[u], cv, lam[j0 - u - 1], mu[j0 - u]);
-#pragma unroll
-                for (int u = 0; u < G; ++u)
-                    if (j0 - u >= 1) op_fma_inplace(lam[j0 - u], omc, t[u]);
-            }
-            op_fma_inplace(lam[0], omc, mu[0]);
-        };
+        for (int k = 0; k < kHalfSlots; ++k) lacc[k][threadIdx.x] += lam[k];
         if constexpr (kSrc == 2) {
-            // stored normals, read kPre steps ahead so the HBM latency hides behind the
-            // previous steps' arithmetic (the matrix-core forward runs 50 steps per date)
-            constexpr int kPre = 10;
-            float cur[kPre][2], nxt[kPre][2];
-            auto load = [&](int n_hi, float (&d)[kPre][2]) {
+            for (int s0 = S - 1; s0 >= 0; s0 -= kPre) {
 #pragma unroll
                 for (int u = 0; u < kPre; ++u) {
-                    const float* q = zs + (int64_t)(n_hi - u) * 3 * P + (active ? i : 0);
-                    if constexpr (kPass == 0) {
-                        d[u][0] = q[0];
-                        d[u][1] = q[P];
-                    } else {
-                        d[u][0] = q[2 * P];
-                        d[u][1] = 0.f;
-                    }
-                }
-            };
-            load(b * g.stride + g.stride - 1, cur);
-            for (int s0 = g.stride - 1; s0 >= 0; s0 -= kPre) {
-                if (s0 - kPre >= 0) load(b * g.stride + s0 - kPre, nxt);
-#pragma unroll
-                for (int u = 0; u < kPre; ++u) {
-                    if constexpr (kPass == 0) step(cur[u][0], cur[u][1], 0.f);
-                    else step(0.f, 0.f, cur[u][0]);
-                }
-#pragma unroll
-                for (int u = 0; u < kPre; ++u) {
-                    cur[u][0] = nxt[u][0];
-                    cur[u][1] = nxt[u][1];
+                    const float a0 = zq[u][0], a1 = zq[u][1], a2 = zq[u][2];
+                    const int nn = b * S + s0 - kPre - u;   // same slot of the next chunk
+                    if (nn >= S) loadz(nn, zq[u]);
+                    step(a0, a1, a2);
                 }
             }
         } else {
-            const int whole = g.stride / kGroupSteps * kGroupSteps;
-            for (int s = g.stride - 1; s >= 0; --s) {
-                const int n = b * g.stride + s;  // global step index of this Euler step
+            for (int s = S - 1; s >= 0; --s) {
+                const int n = b * S + s;  // global step index of this Euler step
                 if constexpr (kSrc == 1) {
-                    const double* q = normals + ((int64_t)(active ? i : 0) * g.nsteps + n) * 3;
+                    const double* q = normals + (ic * g.nsteps + n) * 3;
                     step((float)q[0], (float)q[1], (float)q[2]);
                 } else {
                     // groups of 16 steps staged in LDS, entered from their last step
                     Normals3 z;
                     if (s < whole) {
-                        if ((s & (kGroupSteps - 1)) == kGroupSteps - 1)
-                            fill_group(g.seed_lo, g.seed_hi, pid, (uint32_t)b, s / kGroupSteps, col, kBlock);
-                        z = group_step(col, kBlock, s);
+                        if ((s & (kGroupSteps - 1)) == kGroupSteps - 1) {
+                            __syncthreads();   // the previous group's reads are done
+                            fill_group_half(g.seed_lo, g.seed_hi, pid, (uint32_t)b, s / kGroupSteps,
+                                            half, col, kAdjPaths);
+                            __syncthreads();   // both lanes' calls are in the column
+                        }
+                        z = group_step(col, kAdjPaths, s);
                     } else {
-                        z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n, (uint32_t)g.stride);
+                        z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n, (uint32_t)S);
                     }
                     step(z.z0, z.z1, z.z2);
                 }
             }
         }
-        // flush the block's FP32 partials into the wave's FP64 accumulators
-#pragma unroll
-        for (int j = 0; j < kTenors; ++j) {
-            const double a = wave_sum_f(ga[j]), bb = wave_sum_f(gb[j]);
-            if (lane == 0) {
-                acc[wave][j] += a;
-                acc[wave][kTenors + j] += bb;
-            }
-            ga[j] = gb[j] = 0.f;
+        // FP32 partials cover at most two blocks, then go to the FP64 accumulators
+        if ((b & 1) == 1 || b == 1) {
+            flush(g0, 0);
+            flush(g1, 1);
+            flush(g2, 2);
         }
     }
-    if constexpr (kPass == 1) {
-        // n = 0: lam = T^T lambda(1) + mu(1); dJ/dspot = lam + (stride - 1) mu(1)
-        //        + (gx_1 + stride gA_1) e_1   (block 0 holds the curve for stride steps)
-        const float sx = active ? gx[P + i] : 0.f;
-        const float sa = active ? gA[P + i] : 0.f;
+    // n = 0: lam = T^T lambda(1) + mu(1); dJ/dspot = lam + (stride - 1) mu(1)
+    //        + (gx_1 + stride gA_1) e_1   (block 0 holds the curve for stride steps)
+    {
+        const float sx = active ? gx[P + ic] : 0.f;
+        const float sa = active ? gA[P + ic] : 0.f;
+        float d[kHalfSlots], la[kHalfSlots];
 #pragma unroll
-        for (int j = 0; j < kTenors; ++j) {
-            float d = __builtin_fmaf((float)(g.stride - 1), mu[j], lam[j]);
-            if (j == 1) d += sx + (float)g.stride * sa;
-            const double r = wave_sum_f(d);
-            if (lane == 0) acc[wave][2 * kTenors + j] += r;
+        for (int k = 0; k < kHalfSlots; ++k) {
+            d[k] = __builtin_fmaf((float)(S - 1), mu[k], lam[k]);
+            if (k == 1 && !upper) d[k] += sx + (float)S * sa;
+            la[k] = lacc[k][threadIdx.x];
         }
+        flush(la, 3);
+        flush(d, 4);
     }
     __syncthreads();
-    constexpr int nacc = (kPass == 0) ? 2 * kTenors : 3 * kTenors;
-    for (int k = threadIdx.x; k < nacc; k += kBlock) {
+    // output o = kind * 51 + tenor; tenor 25 comes from the lower half, whose slot 25 it is
+    for (int o = threadIdx.x; o < kAdjKinds * kTenors; o += kBlock) {
+        const int kind = o / kTenors, t = o % kTenors;
+        const int h = t > 25 ? 1 : 0, k = t - 25 * h;
         double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < kBlock / kWave; ++w) s += acc[w][k];
-        partials[(int64_t)blockIdx.x * nacc + k] = s;
+        for (int w = 0; w < kBlock / kWave; ++w) s += acc[w][h][kind][k];
+        partials[(int64_t)blockIdx.x * (kAdjKinds * kTenors) + o] = s;
     }
 }
 
@@ -243,15 +320,22 @@ hipError_t launch_ref_adjoint_seed(const ExposureParams& e, double dt, const dou
     return hipGetLastError();
 }
 
-hipError_t launch_ref_adjoint_sweep(const RefGrid& rg, double c, double sqrt_dt, int pass,
-                                    const double* normals, const float* zs, const float* gx,
-                                    const float* gA, double* partials, hipStream_t st)
+hipError_t launch_row_sum_f32(const float* x, int rows, int64_t n, double* out, hipStream_t st)
+{
+    row_sum_f32_kernel<<<rows, kBlock, 0, st>>>(x, n, out);
+    return hipGetLastError();
+}
+
+int64_t ref_adjoint_blocks(int64_t paths) { return (paths + kAdjPaths - 1) / kAdjPaths; }
+
+hipError_t launch_ref_adjoint(const RefGrid& rg, double c, double sqrt_dt, const double* normals,
+                              const float* zs, const float* gx, const float* gA, double* partials,
+                              hipStream_t st)
 {
     AdjGrid g{};
     g.stride = rg.stride;
     g.nsteps = rg.nsteps;
     g.ndates = rg.ndates;
-    g.pass = pass;
     g.paths = rg.paths;
     g.path0 = rg.path0;
     g.seed_lo = rg.seed_lo;
@@ -260,21 +344,14 @@ hipError_t launch_ref_adjoint_sweep(const RefGrid& rg, double c, double sqrt_dt,
     g.omc = 1.0f - g.c;
     g.opc = 1.0f + g.c;
     g.sqrt_dt = (float)sqrt_dt;
-    const dim3 grid((unsigned)((rg.paths + kBlock - 1) / kBlock));
-    if (zs && rg.stride % 10 != 0) return hipErrorInvalidValue;  // stored-normal prefetch chunks
-#define HJM_SWEEP(P_, S_) \
-    ref_adjoint_sweep_kernel<P_, S_><<<grid, kBlock, 0, st>>>(g, normals, zs, gx, gA, partials)
-    const int src = normals ? 1 : (zs ? 2 : 0);
-    if (pass == 0) {
-        if (src == 1) HJM_SWEEP(0, 1);
-        else if (src == 2) HJM_SWEEP(0, 2);
-        else HJM_SWEEP(0, 0);
-    } else {
-        if (src == 1) HJM_SWEEP(1, 1);
-        else if (src == 2) HJM_SWEEP(1, 2);
-        else HJM_SWEEP(1, 0);
-    }
-#undef HJM_SWEEP
+    const dim3 grid((unsigned)ref_adjoint_blocks(rg.paths));
+    if (zs && rg.stride % 5 != 0) return hipErrorInvalidValue;  // stored-normal read-ahead
+    if (normals)
+        ref_adjoint_kernel<1><<<grid, kBlock, 0, st>>>(g, normals, zs, gx, gA, partials);
+    else if (zs)
+        ref_adjoint_kernel<2><<<grid, kBlock, 0, st>>>(g, normals, zs, gx, gA, partials);
+    else
+        ref_adjoint_kernel<0><<<grid, kBlock, 0, st>>>(g, normals, zs, gx, gA, partials);
     return hipGetLastError();
 }
 

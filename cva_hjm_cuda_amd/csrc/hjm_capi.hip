@@ -424,20 +424,19 @@ int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
     if (!gr) return HJM_ERR_NULL;
     hjm_plan p = nullptr;
     HJM_TRY(hjm_plan_create(&p, market, swap, cfg));
+    constexpr int kOut = 5 * kTenors;  // g0, g1, g2, sum of lambda_e, d spot
     struct Cleanup {
         hjm_plan p;
         float *gx = nullptr, *gA = nullptr, *zs = nullptr;
-        double *pa = nullptr, *pb = nullptr, *sa = nullptr, *sb = nullptr;
+        double *part = nullptr, *sums = nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         ~Cleanup()
         {
             dev_free(gx);
             dev_free(zs);
             dev_free(gA);
-            dev_free(pa);
-            dev_free(pb);
-            dev_free(sa);
-            dev_free(sb);
+            dev_free(part);
+            dev_free(sums);
             if (e0) (void)hipEventDestroy(e0);
             if (e1) (void)hipEventDestroy(e1);
             hjm_plan_destroy(p);
@@ -445,18 +444,17 @@ int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
     } c{p};
     DeviceGuard guard;
     HJM_TRY(guard.set(p->device));
-    const int64_t P = p->paths, nb = p->nblocks;
+    const int64_t P = p->paths, nb = ref_adjoint_blocks(P);
+    const int D = p->grid.ndates, S = p->grid.stride;
     const size_t cells = (size_t)kTenors * P;
     HJM_TRY(dev_alloc(&c.gx, cells));
     HJM_TRY(dev_alloc(&c.gA, cells));
-    HJM_TRY(dev_alloc(&c.pa, (size_t)nb * 2 * kTenors));
-    HJM_TRY(dev_alloc(&c.pb, (size_t)nb * 3 * kTenors));
-    HJM_TRY(dev_alloc(&c.sa, 2 * kTenors));
-    HJM_TRY(dev_alloc(&c.sb, 3 * kTenors));
+    HJM_TRY(dev_alloc(&c.part, (size_t)nb * kOut));
+    HJM_TRY(dev_alloc(&c.sums, kOut + kTenors));  // then the per-slot sums of gA
     HJM_HIP(hipEventCreate(&c.e0), HJM_ERR_DEVICE);
     HJM_HIP(hipEventCreate(&c.e1), HJM_ERR_DEVICE);
-    // the matrix-core forward keeps its normals for the reverse sweeps (one FP32 per factor
-    // per evolved step and path); without the memory the sweeps regenerate them
+    // the matrix-core forward keeps its normals for the reverse sweep (one FP32 per factor
+    // per evolved step and path); without the memory the sweep regenerates them
     if (p->use_mfma && dev_alloc(&c.zs, (size_t)p->grid.nsteps * 3 * (size_t)P) != HJM_OK)
         c.zs = nullptr;
     hipStream_t st = p->stream;
@@ -465,29 +463,40 @@ int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
     const double dt = p->grid.dt;
     const double cc = dt / swap->dtau;
     HJM_HIP(launch_ref_adjoint_seed(p->ep, dt, p->d_nx, p->d_ny, c.gx, c.gA, st), HJM_ERR_LAUNCH);
-    HJM_HIP(launch_ref_adjoint_sweep(p->grid, cc, sqrt(dt), 0, p->d_normals, c.zs, c.gx, c.gA, c.pa, st),
+    // slots 1..D of gA summed over the paths (the drift gradient's mu terms)
+    HJM_HIP(launch_row_sum_f32(c.gA + P, D, P, c.sums + kOut, st), HJM_ERR_LAUNCH);
+    HJM_HIP(launch_ref_adjoint(p->grid, cc, sqrt(dt), p->d_normals, c.zs, c.gx, c.gA, c.part, st),
             HJM_ERR_LAUNCH);
-    HJM_HIP(launch_column_sum(c.pa, nb, 2 * kTenors, c.sa, st), HJM_ERR_LAUNCH);
-    HJM_HIP(launch_ref_adjoint_sweep(p->grid, cc, sqrt(dt), 1, p->d_normals, c.zs, c.gx, c.gA, c.pb, st),
-            HJM_ERR_LAUNCH);
-    HJM_HIP(launch_column_sum(c.pb, nb, 3 * kTenors, c.sb, st), HJM_ERR_LAUNCH);
+    HJM_HIP(launch_column_sum(c.part, nb, kOut, c.sums, st), HJM_ERR_LAUNCH);
     HJM_HIP(hipEventRecord(c.e1, st), HJM_ERR_DEVICE);
     hjm_results r{};
     r.cva = gr->cva;
     r.cva_stderr = gr->cva_stderr;
     HJM_TRY(plan_collect(p, &r));
-    double sa[2 * kTenors], sb[3 * kTenors];
-    HJM_HIP(hipMemcpy(sa, c.sa, sizeof(sa), hipMemcpyDeviceToHost), HJM_ERR_DEVICE);
-    HJM_HIP(hipMemcpy(sb, c.sb, sizeof(sb), hipMemcpyDeviceToHost), HJM_ERR_DEVICE);
+    double sums[kOut + kTenors];
+    HJM_HIP(hipMemcpy(sums, c.sums, sizeof(double) * (kOut + D), hipMemcpyDeviceToHost),
+            HJM_ERR_DEVICE);
     const double inv = 1.0 / (double)P;
     double dv[kFactors * kTenors], dm[kTenors];
-    for (int j = 0; j < kTenors; ++j) {
-        dv[j] = sa[j] * inv;
-        dv[kTenors + j] = sa[kTenors + j] * inv;
-        dv[2 * kTenors + j] = sb[j] * inv;
-        dm[j] = dt * sb[kTenors + j] * inv;
-        if (gr->d_spot) gr->d_spot[j] = sb[2 * kTenors + j] * inv;
+    for (int k = 0; k < kFactors * kTenors; ++k) dv[k] = sums[k] * inv;
+    // drift: dt (A Lambda + B M) / P, Lambda = sum over blocks and paths of lambda_e, M_s =
+    // (s - 1) x (sum over paths of gA at slot s): the injection at slot s is live in the
+    // s - 1 blocks 1 .. s - 1
+    {
+        std::vector<double> A((size_t)kTenors * kTenors), B((size_t)kTenors * kTenors);
+        adjoint_block_sums((double)step_ratio_f32(cc), S, A.data(), B.data());
+        double M[kTenors] = {0.0};
+        for (int slot = 2; slot <= D; ++slot) M[slot] = (double)(slot - 1) * sums[kOut + slot - 1];
+        const double* L = sums + 3 * kTenors;
+        for (int j = 0; j < kTenors; ++j) {
+            double a = 0.0;
+            for (int m = 0; m < kTenors; ++m)
+                a += A[(size_t)j * kTenors + m] * L[m] + B[(size_t)j * kTenors + m] * M[m];
+            dm[j] = dt * a * inv;
+        }
     }
+    if (gr->d_spot)
+        for (int j = 0; j < kTenors; ++j) gr->d_spot[j] = sums[4 * kTenors + j] * inv;
     if (gr->d_vol) memcpy(gr->d_vol, dv, sizeof(dv));
     if (gr->d_drift) memcpy(gr->d_drift, dm, sizeof(dm));
     if (gr->d_vol_total) {

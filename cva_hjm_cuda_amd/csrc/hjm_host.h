@@ -1,6 +1,8 @@
 // hjm_host.h -- host-side helpers shared by the C ABI translation units.
 #pragma once
 #include <math.h>
+
+#include <vector>
 #include <stdint.h>
 
 #include "hjm_common.h"
@@ -136,6 +138,36 @@ inline void cumulative_simpson(const double* y, double h, double* out)
 
 // out += (d m / d sigma)^T g for m_j = sum_k sigma_kj I_kj, I_k = W sigma_k:
 // d m_j / d sigma_kl = delta_jl I_kj + sigma_kj W_jl.
+// Sums over the S steps of a date block of the adjoint's reverse recursion
+// lambda(k + 1) = T^T lambda(k) + mu, lambda(0) = lambda_e (T = I + c Dfwd with the
+// backward difference on the last tenor, c as the FP32 step takes it):
+//   sum_{k<S} lambda(k) = A lambda_e + B mu,  A = sum_{k<S} (T^T)^k,
+//   B = sum_{r<S-1} (S-1-r) (T^T)^r   (row-major [51][51]).
+inline void adjoint_block_sums(double c, int S, double* A, double* B)
+{
+    const int n = kTenors;
+    std::vector<double> Pk((size_t)n * n, 0.0), next((size_t)n * n);
+    for (int j = 0; j < n; ++j) Pk[(size_t)j * n + j] = 1.0;
+    for (size_t e = 0; e < (size_t)n * n; ++e) A[e] = B[e] = 0.0;
+    for (int k = 0; k < S; ++k) {
+        for (size_t e = 0; e < (size_t)n * n; ++e) {
+            A[e] += Pk[e];
+            if (k < S - 1) B[e] += (double)(S - 1 - k) * Pk[e];
+        }
+        // next = T^T Pk: row j of T^T holds T[i][j] over i
+        for (int j = 0; j < n; ++j)
+            for (int m = 0; m < n; ++m) {
+                double v;
+                if (j == n - 1) v = (1.0 + c) * Pk[(size_t)j * n + m] + c * Pk[(size_t)(j - 1) * n + m];
+                else if (j == n - 2) v = (1.0 - c) * Pk[(size_t)j * n + m] + c * Pk[(size_t)(j - 1) * n + m] - c * Pk[(size_t)(j + 1) * n + m];
+                else if (j == 0) v = (1.0 - c) * Pk[(size_t)j * n + m];
+                else v = (1.0 - c) * Pk[(size_t)j * n + m] + c * Pk[(size_t)(j - 1) * n + m];
+                next[(size_t)j * n + m] = v;
+            }
+        Pk.swap(next);
+    }
+}
+
 inline void drift_vjp(const double* vols, double h, const double* g, double* out)
 {
     double W[kTenors][kTenors];  // W[l][j] = weight of y_l in the integral to node j

@@ -40,10 +40,15 @@ hipError_t launch_pf_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, float*
 // Pathwise adjoint of the reference-semantics CVA.
 hipError_t launch_ref_adjoint_seed(const ExposureParams& e, double dt, const double* nx,
                                    const double* ny, float* gx, float* gA, hipStream_t st);
-// normals: host FP64 [path][step][3]; else zs: stored FP32 [step][3][path]; else regenerate
-hipError_t launch_ref_adjoint_sweep(const RefGrid& rg, double c, double sqrt_dt, int pass,
-                                    const double* normals, const float* zs, const float* gx,
-                                    const float* gA, double* partials, hipStream_t st);
+// One-pass sweep, two lanes per path. normals: host FP64 [path][step][3]; else zs: stored
+// FP32 [step][3][path]; else regenerate. partials: [ref_adjoint_blocks(paths)][5 * 51]
+// (loading gradients of factors 0, 1, 2, sum of the block-end adjoints, d J / d spot).
+int64_t ref_adjoint_blocks(int64_t paths);
+hipError_t launch_ref_adjoint(const RefGrid& rg, double c, double sqrt_dt, const double* normals,
+                              const float* zs, const float* gx, const float* gA, double* partials,
+                              hipStream_t st);
+// out[r] = FP64 sum of row r of an FP32 [rows][n] array
+hipError_t launch_row_sum_f32(const float* x, int rows, int64_t n, double* out, hipStream_t st);
 
 // Portfolio semantics.
 hipError_t launch_pf_paths_f64(const RefStepConsts<double>& k, const PfGrid& g, const double* normals,

@@ -141,11 +141,19 @@ def test_gpu_fp32_forward_greeks_close_to_fp64():
     assert abs(b.cva / a.cva - 1) < 1e-4
     for x, y in ((b.d_spot, a.d_spot), (b.d_vol, a.d_vol), (b.d_drift, a.d_drift)):
         assert np.max(np.abs(x - y)) <= 1e-3 * np.max(np.abs(y))
-    ratio = b.timings_ms[2] / b.timings_ms[0]
-    _flush("  CVA+Greeks / forward device time = %.2f" % ratio)
-    # the FP32 forward runs on the matrix cores, the reverse sweeps on the VALU; measured
-    # 3.5-3.7 on MI355X boxes (target 4), bounded with room for box-to-box spread
-    assert ratio <= 4.5
+    # cost against a forward-only plan of the same size (path kernel without stored
+    # normals, exposure, reductions, no PFE); the median of a few runs of each
+    t = np.array([hjm.cva_greeks(m, swap, 65536, seed=7, precision=hjm.FP32_KAHAN).timings_ms
+                  for _ in range(3)])
+    plan = hjm.ReferencePlan(m, swap, 65536, precision=hjm.FP32_KAHAN, seed=7, pfe_quantile=0.0)
+    plan.execute()
+    fwd = float(np.median([plan.execute().timings_ms[4] for _ in range(3)]))
+    plan.close()
+    ratio = float(np.median(t[:, 2])) / fwd
+    _flush("  CVA+Greeks / forward-only device time = %.2f" % ratio)
+    # the FP32 forward runs on the matrix cores, the one-pass reverse sweep on the VALU;
+    # measured 2.97 on an MI355X; the target is 4
+    assert ratio <= 4.0
 
 
 @pytest.mark.gpu
