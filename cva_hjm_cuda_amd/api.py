@@ -57,12 +57,14 @@ class _Config(C.Structure):
     _fields_ = [("paths", C.c_int64), ("path_offset", C.c_int64), ("dt", C.c_double),
                 ("precision", C.c_int32), ("rng", C.c_int32), ("seed", C.c_uint64),
                 ("host_normals", _dp), ("hazard_rate", C.c_double), ("recovery", C.c_double),
-                ("pfe_quantile", C.c_double), ("device", C.c_int32), ("path_kernel", C.c_int32)]
+                ("pfe_quantile", C.c_double), ("device", C.c_int32), ("path_kernel", C.c_int32),
+                ("adjoint_normals", C.c_int32)]
 
 
 class _Greeks(C.Structure):
     _fields_ = [("cva", _dp), ("cva_stderr", _dp), ("d_spot", _dp), ("d_vol", _dp),
-                ("d_drift", _dp), ("d_vol_total", _dp), ("timings_ms", _dp)]
+                ("d_drift", _dp), ("d_vol_total", _dp), ("timings_ms", _dp),
+                ("stored_normals_bytes", C.POINTER(C.c_int64))]
 
 
 class _Portfolio(C.Structure):
@@ -258,10 +260,10 @@ KERNEL_AUTO, KERNEL_VECTOR, KERNEL_MATRIX = 0, 1, 2
 
 
 def _config(paths, dt, precision, rng, seed, normals, hazard, recovery, pfe_quantile, device,
-            path_offset, path_kernel=KERNEL_AUTO):
+            path_offset, path_kernel=KERNEL_AUTO, regenerate_normals=False):
     return _Config(int(paths), int(path_offset), float(dt), int(precision), int(rng), int(seed),
                    _ptr(normals), float(hazard), float(recovery), float(pfe_quantile),
-                   int(device), int(path_kernel))
+                   int(device), int(path_kernel), int(bool(regenerate_normals)))
 
 
 def _host_normals(normals, paths: int, horizon: float, dt: float, what: str) -> np.ndarray:
@@ -336,14 +338,17 @@ class Greeks:
     d_drift: np.ndarray      # dCVA / d m_j                              [51]
     d_vol_total: np.ndarray  # d_vol + (dm/dsigma)^T d_drift (Simpson HJM drift)  [153]
     timings_ms: np.ndarray   # forward, adjoint, total (device)
+    stored_normals_bytes: int = 0  # shocks kept by the forward for the sweep (0: regenerated)
 
 
 def cva_greeks(market: Market, swap: InterestRateSwap, paths: int, dt: float = 0.01,
                precision: int = FP32_KAHAN, seed: int = 1234,
                normals: Optional[np.ndarray] = None, hazard: float = 0.02,
                recovery: float = 0.4, device: int = 0, path_offset: int = 0,
-               path_kernel: int = KERNEL_AUTO) -> Greeks:
-    """CVA and its 51 + 153 (+ 51 drift) pathwise-adjoint sensitivities on the GPU."""
+               path_kernel: int = KERNEL_AUTO, regenerate_normals: bool = False) -> Greeks:
+    """CVA and its 51 + 153 (+ 51 drift) pathwise-adjoint sensitivities on the GPU.
+    ``regenerate_normals``: the reverse sweep regenerates the shocks instead of reading the
+    matrix-core forward's stored copy (no paths x steps x 3 FP32 buffer)."""
     rng = RNG_PHILOX
     if normals is not None:
         normals = _host_normals(normals, paths, swap.expiry, dt, "cva_greeks")
@@ -351,14 +356,16 @@ def cva_greeks(market: Market, swap: InterestRateSwap, paths: int, dt: float = 0
     mk = market._c()
     sw = _Swap(swap.notional, swap.K, swap.expiry, swap.dtau)
     cfg = _config(paths, dt, precision, rng, seed, normals, hazard, recovery, 0.0, device,
-                  path_offset, path_kernel)
+                  path_offset, path_kernel, regenerate_normals)
     cva, se = np.zeros(1), np.zeros(1)
     ds, dv, dm, dvt, tm = np.zeros(TENORS), np.zeros(3 * TENORS), np.zeros(TENORS), \
         np.zeros(3 * TENORS), np.zeros(3)
-    g = _Greeks(_ptr(cva), _ptr(se), _ptr(ds), _ptr(dv), _ptr(dm), _ptr(dvt), _ptr(tm))
+    nb = np.zeros(1, np.int64)
+    g = _Greeks(_ptr(cva), _ptr(se), _ptr(ds), _ptr(dv), _ptr(dm), _ptr(dvt), _ptr(tm),
+                nb.ctypes.data_as(C.POINTER(C.c_int64)))
     _check(library().hjm_cva_greeks_reference(C.byref(mk), C.byref(sw), C.byref(cfg),
                                               C.byref(g)), "hjm_cva_greeks_reference")
-    return Greeks(float(cva[0]), float(se[0]), ds, dv, dm, dvt, tm)
+    return Greeks(float(cva[0]), float(se[0]), ds, dv, dm, dvt, tm, int(nb[0]))
 
 
 class ReferencePlan:
@@ -572,25 +579,29 @@ def portfolio_cva_greeks(market: Market, portfolio, paths: int, horizon: float =
                          dt: float = 0.01, precision: int = FP32_KAHAN, seed: int = 2024,
                          normals: Optional[np.ndarray] = None, recovery: float = 0.4,
                          tuning: Optional[Tuning] = None, device: int = 0,
-                         path_kernel: int = KERNEL_AUTO) -> Greeks:
-    """Total portfolio CVA and its 51 + 153 (+ 51 drift) pathwise-adjoint sensitivities."""
+                         path_kernel: int = KERNEL_AUTO, regenerate_normals: bool = False,
+                         path_offset: int = 0) -> Greeks:
+    """Total portfolio CVA and its 51 + 153 (+ 51 drift) pathwise-adjoint sensitivities.
+    ``regenerate_normals`` as for cva_greeks."""
     rng = RNG_PHILOX
     if normals is not None:
         normals = _host_normals(normals, paths, horizon, dt, "portfolio_cva_greeks")
         rng = RNG_HOST
     mk = market._c()
     pfc = _pf_c(portfolio, horizon, recovery)
-    cfg = _config(paths, dt, precision, rng, seed, normals, 0.0, recovery, 0.0, device, 0,
-                  path_kernel)
+    cfg = _config(paths, dt, precision, rng, seed, normals, 0.0, recovery, 0.0, device,
+                  path_offset, path_kernel, regenerate_normals)
     tun = tuning._c() if tuning is not None else None
     cva, se = np.zeros(1), np.zeros(1)
     ds, dv, dm, dvt, tm = (np.zeros(TENORS), np.zeros(3 * TENORS), np.zeros(TENORS),
                            np.zeros(3 * TENORS), np.zeros(3))
-    g = _Greeks(_ptr(cva), _ptr(se), _ptr(ds), _ptr(dv), _ptr(dm), _ptr(dvt), _ptr(tm))
+    nb = np.zeros(1, np.int64)
+    g = _Greeks(_ptr(cva), _ptr(se), _ptr(ds), _ptr(dv), _ptr(dm), _ptr(dvt), _ptr(tm),
+                nb.ctypes.data_as(C.POINTER(C.c_int64)))
     _check(library().hjm_portfolio_cva_greeks(C.byref(mk), C.byref(pfc), C.byref(cfg),
                                               C.byref(tun) if tun is not None else None,
                                               C.byref(g)), "hjm_portfolio_cva_greeks")
-    return Greeks(float(cva[0]), float(se[0]), ds, dv, dm, dvt, tm)
+    return Greeks(float(cva[0]), float(se[0]), ds, dv, dm, dvt, tm, int(nb[0]))
 
 
 # ---- calibration upstream of the path -------------------------------------------------

@@ -117,6 +117,8 @@ int check_ref_config(const hjm_market* market, const hjm_swap* swap, const hjm_c
         return HJM_ERR_ARG;
     if (cfg->pfe_quantile > 1.0) return HJM_ERR_ARG;
     if (cfg->path_kernel < HJM_KERNEL_AUTO || cfg->path_kernel > HJM_KERNEL_MATRIX) return HJM_ERR_ARG;
+    if (cfg->adjoint_normals != HJM_ADJ_NORMALS_AUTO && cfg->adjoint_normals != HJM_ADJ_NORMALS_REGENERATE)
+        return HJM_ERR_ARG;
     HJM_TRY(ref_grid(swap->expiry, swap->dtau, cfg->dt, stride, S, D));
     if ((int64_t)cfg->paths * kTenors > (1ll << 40)) return HJM_ERR_ARG;
     return HJM_OK;
@@ -455,7 +457,8 @@ int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
     HJM_HIP(hipEventCreate(&c.e1), HJM_ERR_DEVICE);
     // the matrix-core forward keeps its normals for the reverse sweep (one FP32 per factor
     // per evolved step and path); without the memory the sweep regenerates them
-    if (p->use_mfma && dev_alloc(&c.zs, (size_t)p->grid.nsteps * 3 * (size_t)P) != HJM_OK)
+    if (p->use_mfma && cfg->adjoint_normals == HJM_ADJ_NORMALS_AUTO &&
+        dev_alloc(&c.zs, (size_t)p->grid.nsteps * 3 * (size_t)P) != HJM_OK)
         c.zs = nullptr;
     hipStream_t st = p->stream;
     HJM_TRY(plan_launch(p, false, c.zs));
@@ -503,6 +506,8 @@ int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
         memcpy(gr->d_vol_total, dv, sizeof(dv));
         drift_vjp(market->volatilities, swap->dtau, dm, gr->d_vol_total);
     }
+    if (gr->stored_normals_bytes)
+        *gr->stored_normals_bytes = c.zs ? (int64_t)p->grid.nsteps * 3 * P * (int64_t)sizeof(float) : 0;
     if (gr->timings_ms) {
         float fwd = 0.f, adj = 0.f;
         HJM_HIP(hipEventElapsedTime(&fwd, p->ev[0], p->ev[4]), HJM_ERR_DEVICE);

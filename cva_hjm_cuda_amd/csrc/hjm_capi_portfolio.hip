@@ -159,6 +159,8 @@ int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfol
     if (stride <= 0 || fabs(rs - stride) > 1e-6 * stride) return HJM_ERR_GRID;
     const int S = stride * (nd - 1);
     if (cfg->path_kernel < HJM_KERNEL_AUTO || cfg->path_kernel > HJM_KERNEL_MATRIX) return HJM_ERR_ARG;
+    if (cfg->adjoint_normals != HJM_ADJ_NORMALS_AUTO && cfg->adjoint_normals != HJM_ADJ_NORMALS_REGENERATE)
+        return HJM_ERR_ARG;
     const bool mfma_ok = cfg->precision == HJM_FP32_KAHAN && cfg->rng == HJM_RNG_PHILOX &&
                          stride == kMfmaStride;
     if (cfg->path_kernel == HJM_KERNEL_MATRIX && !mfma_ok) return HJM_ERR_UNSUPPORTED;
@@ -491,20 +493,19 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
     cfg.pfe_quantile = 0.0;
     hjm_portfolio_plan p = nullptr;
     HJM_TRY(hjm_portfolio_plan_create(&p, market, portfolio, &cfg, tuning));
+    constexpr int kOut = 5 * kTenors;  // g0, g1, g2, sum of lambda_e, d spot
     struct Cleanup {
         hjm_portfolio_plan p;
         float *gbar = nullptr, *beta = nullptr, *zs = nullptr;
-        double *pa = nullptr, *pb = nullptr, *sa = nullptr, *sb = nullptr;
+        double *part = nullptr, *sums = nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         ~Cleanup()
         {
             dev_free(zs);
             dev_free(gbar);
             dev_free(beta);
-            dev_free(pa);
-            dev_free(pb);
-            dev_free(sa);
-            dev_free(sb);
+            dev_free(part);
+            dev_free(sums);
             if (e0) (void)hipEventDestroy(e0);
             if (e1) (void)hipEventDestroy(e1);
             hjm_portfolio_plan_destroy(p);
@@ -513,19 +514,18 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
     DeviceGuard guard;
     HJM_TRY(guard.set(p->device));
     const int64_t P = p->paths;
-    const int nd = p->ndates;
-    const int64_t nb = (P + kBlock - 1) / kBlock;
+    const int nd = p->ndates, E = nd - 1;
+    const int64_t nb = pf_adjoint_blocks(P);
     HJM_TRY(dev_alloc(&c.gbar, (size_t)nd * kTenors * P));
     HJM_TRY(dev_alloc(&c.beta, (size_t)nd * P));
-    HJM_TRY(dev_alloc(&c.pa, (size_t)nb * 2 * kTenors));
-    HJM_TRY(dev_alloc(&c.pb, (size_t)nb * 3 * kTenors));
-    HJM_TRY(dev_alloc(&c.sa, 2 * kTenors));
-    HJM_TRY(dev_alloc(&c.sb, 3 * kTenors));
+    HJM_TRY(dev_alloc(&c.part, (size_t)nb * kOut));
+    HJM_TRY(dev_alloc(&c.sums, (size_t)kOut + kMaxDates));  // then the per-date sums of beta
     HJM_HIP(hipEventCreate(&c.e0), HJM_ERR_DEVICE);
     HJM_HIP(hipEventCreate(&c.e1), HJM_ERR_DEVICE);
-    // the matrix-core forward keeps its normals for the reverse sweeps; without the
-    // memory the sweeps regenerate them
-    if (p->use_mfma && dev_alloc(&c.zs, (size_t)p->grid.nsteps * 3 * (size_t)P) != HJM_OK)
+    // the matrix-core forward keeps its normals for the reverse sweep; without the memory
+    // the sweep regenerates them
+    if (p->use_mfma && config->adjoint_normals == HJM_ADJ_NORMALS_AUTO &&
+        dev_alloc(&c.zs, (size_t)p->grid.nsteps * 3 * (size_t)P) != HJM_OK)
         c.zs = nullptr;
     hipStream_t st = p->stream;
     HJM_TRY(pplan_enqueue(p, true, c.zs));
@@ -539,39 +539,52 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
                                                       p->d_kmax, p->d_w, c.gbar, c.beta, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
     const double cc = p->grid.dt / portfolio->dtau;
-    HJM_HIP(launch_pf_adjoint_sweep(p->grid, cc, 0, p->d_normals, c.zs, c.gbar, c.beta, c.pa, st),
+    if (E > 0)
+        HJM_HIP(launch_row_sum_f32(c.beta + P, E, P, c.sums + kOut, st), HJM_ERR_LAUNCH);
+    HJM_HIP(launch_pf_adjoint(p->grid, cc, p->d_normals, c.zs, c.gbar, c.beta, c.part, st),
             HJM_ERR_LAUNCH);
-    HJM_HIP(launch_column_sum(c.pa, nb, 2 * kTenors, c.sa, st), HJM_ERR_LAUNCH);
-    HJM_HIP(launch_pf_adjoint_sweep(p->grid, cc, 1, p->d_normals, c.zs, c.gbar, c.beta, c.pb, st),
-            HJM_ERR_LAUNCH);
-    HJM_HIP(launch_column_sum(c.pb, nb, 3 * kTenors, c.sb, st), HJM_ERR_LAUNCH);
+    HJM_HIP(launch_column_sum(c.part, nb, kOut, c.sums, st), HJM_ERR_LAUNCH);
     HJM_HIP(hipEventRecord(c.e1, st), HJM_ERR_DEVICE);
     std::vector<double> cvas(p->nsets);
     hjm_portfolio_results r{};
     r.cva = cvas.data();
     HJM_TRY(pplan_collect(p, &r));
-    double sa[2 * kTenors], sb[3 * kTenors];
-    HJM_HIP(hipMemcpy(sa, c.sa, sizeof(sa), hipMemcpyDeviceToHost), HJM_ERR_DEVICE);
-    HJM_HIP(hipMemcpy(sb, c.sb, sizeof(sb), hipMemcpyDeviceToHost), HJM_ERR_DEVICE);
+    std::vector<double> sums((size_t)kOut + kMaxDates, 0.0);
+    HJM_HIP(hipMemcpy(sums.data(), c.sums, sizeof(double) * (kOut + E), hipMemcpyDeviceToHost),
+            HJM_ERR_DEVICE);
     double total = 0.0;
     for (double v : cvas) total += v;
     if (gr->cva) *gr->cva = total;
     if (gr->cva_stderr) *gr->cva_stderr = NAN;
     const double inv = 1.0 / (double)P, dt = p->grid.dt;
     double dv[kFactors * kTenors], dm[kTenors];
-    for (int j = 0; j < kTenors; ++j) {
-        dv[j] = sa[j] * inv;
-        dv[kTenors + j] = sa[kTenors + j] * inv;
-        dv[2 * kTenors + j] = sb[j] * inv;
-        dm[j] = dt * sb[kTenors + j] * inv;
-        if (gr->d_spot) gr->d_spot[j] = sb[2 * kTenors + j] * inv;
+    for (int k = 0; k < kFactors * kTenors; ++k) dv[k] = sums[k] * inv;
+    // drift: dt (A Lambda + B mu_sum e_0) / P with Lambda the sum over dates and paths of
+    // the date-end adjoints and mu_sum = dt sum_e e (sum over paths of beta_e): beta_e
+    // feeds mu in the e intervals before date e
+    {
+        std::vector<double> A((size_t)kTenors * kTenors), B((size_t)kTenors * kTenors);
+        adjoint_block_sums((double)step_ratio_f32(cc), p->grid.stride, A.data(), B.data());
+        double musum = 0.0;
+        for (int e = 1; e <= E; ++e) musum += (double)e * sums[kOut + e - 1];
+        musum *= dt;
+        const double* L = sums.data() + 3 * kTenors;
+        for (int j = 0; j < kTenors; ++j) {
+            double a = B[(size_t)j * kTenors] * musum;
+            for (int m = 0; m < kTenors; ++m) a += A[(size_t)j * kTenors + m] * L[m];
+            dm[j] = dt * a * inv;
+        }
     }
+    if (gr->d_spot)
+        for (int j = 0; j < kTenors; ++j) gr->d_spot[j] = sums[4 * kTenors + j] * inv;
     if (gr->d_vol) memcpy(gr->d_vol, dv, sizeof(dv));
     if (gr->d_drift) memcpy(gr->d_drift, dm, sizeof(dm));
     if (gr->d_vol_total) {
         memcpy(gr->d_vol_total, dv, sizeof(dv));
         drift_vjp(market->volatilities, portfolio->dtau, dm, gr->d_vol_total);
     }
+    if (gr->stored_normals_bytes)
+        *gr->stored_normals_bytes = c.zs ? (int64_t)p->grid.nsteps * 3 * P * (int64_t)sizeof(float) : 0;
     if (gr->timings_ms) {
         float fwd = 0.f, adj = 0.f;
         HJM_HIP(hipEventElapsedTime(&fwd, p->ev[0], p->ev[4]), HJM_ERR_DEVICE);

@@ -94,9 +94,11 @@ hipError_t launch_pf_adjoint_seed_f32(const PfValueParams& v, const float* curve
                                       const double* logb, const float* coef, const float* a,
                                       const int* kmax, const double* w, float* gbar, float* beta,
                                       hipStream_t st);
-hipError_t launch_pf_adjoint_sweep(const PfGrid& g, double c, int pass, const double* normals,
-                                   const float* zs, const float* gbar, const float* beta,
-                                   double* partials, hipStream_t st);
+// One-pass sweep, two lanes per path; partials [pf_adjoint_blocks(paths)][5 * 51] as the
+// reference sweep's.
+int64_t pf_adjoint_blocks(int64_t paths);
+hipError_t launch_pf_adjoint(const PfGrid& g, double c, const double* normals, const float* zs,
+                             const float* gbar, const float* beta, double* partials, hipStream_t st);
 
 // The reference's own GPU design (baseline form).
 hipError_t launch_blockwise_paths(const RefStepConsts<double>& k, const RefGrid& g, int block_index,

@@ -14,6 +14,7 @@
 //      (host-aggregated coefficients staged in LDS), exposure max(V_g, 0) / B stored
 //      [date][set][path].
 //   3. pf_stats_kernel: per set, EE numerators and per-path CVA moments (FP64).
+#include "hjm_adjoint_lanes.h"
 #include "hjm_common.h"
 #include "hjm_kernels.h"
 #include "hjm_rng.h"
@@ -667,168 +668,176 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_seed_kernel(const PfValuePa
     beta[(int64_t)e * P + i] = (float)b;
 }
 
-// Reverse sweep over the whole horizon (n = S-1 .. 0): grads with lambda(n+1) and the
-// shocks of step n, lambda <- T^T lambda + mu e_0 (mu = dt sum of beta of later dates),
-// + gbar_e at date steps. Pass 0: factors 0, 1; pass 1: factor 2, drift, curve.
+// Reverse sweep over the whole horizon (n = S-1 .. 0) in one pass, two lanes per path
+// (the lower half-wave holds tenors 0..25, the upper half-wave tenors 25..50 with tenor 25
+// copied from the lower lane each step): loading gradients with lambda(n+1) and the shocks
+// of step n, lambda <- T^T lambda + mu e_0 (mu = dt sum of beta of later dates), + gbar_e at
+// the date steps. The drift gradient comes from the block-end adjoints lambda_e, summed
+// over dates and paths (the host applies sum_{k<S} (T^T)^k to them and the mu terms).
 // kSrc: 0 regenerate the in-kernel normals, 1 host-supplied FP64 [path][step][3],
 // 2 the forward pass's stored FP32 normals [step][3][path].
-template <int kPass, int kSrc>
-__global__ __launch_bounds__(kBlock) void pf_adjoint_sweep_kernel(const PfGrid g, float c,
-                                                                  const double* __restrict__ normals,
-                                                                  const float* __restrict__ zs,
-                                                                  const float* __restrict__ gbar,
-                                                                  const float* __restrict__ beta,
-                                                                  double* __restrict__ partials)
+// partials[workgroup][5 * 51]: g0, g1, g2, sum of lambda_e, d J / d spot.
+template <int kSrc>
+__global__ __launch_bounds__(kBlock) void pf_adjoint_kernel(const PfGrid g, float c,
+                                                            const double* __restrict__ normals,
+                                                            const float* __restrict__ zs,
+                                                            const float* __restrict__ gbar,
+                                                            const float* __restrict__ beta,
+                                                            double* __restrict__ partials)
 {
-    constexpr int kAcc = 3 * kTenors;
-    __shared__ double acc[kBlock / kWave][kAcc];
-    // each lane's column of regenerated shocks (16 steps at a time)
-    __shared__ float zbuf[kSrc == 0 ? kGroupShocks * kBlock : 1];
-    float* col = zbuf + (kSrc == 0 ? threadIdx.x : 0);
+    __shared__ double acc[kBlock / kWave][2][kAdjKinds][kHalfSlots];
+    __shared__ float lacc[kHalfSlots][kBlock];  // per lane: sum over dates of lambda_e
+    __shared__ float zbuf[kSrc == 0 ? kGroupShocks * kAdjPaths : 1];
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-    for (int k = lane; k < kAcc; k += kWave) acc[wave][k] = 0.0;
+    const int half = lane >> 5;
+    const int pslot = wave * 32 + (lane & 31);
+    for (int k = threadIdx.x; k < (kBlock / kWave) * 2 * kAdjKinds * kHalfSlots; k += kBlock)
+        (&acc[0][0][0][0])[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < kHalfSlots; ++k) lacc[k][threadIdx.x] = 0.f;
     __syncthreads();
     const int64_t P = g.paths;
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * kAdjPaths + pslot;
     const bool active = i < P;
-    const int64_t ii = active ? i : 0;
-    const uint32_t pid = (uint32_t)(g.path0 + ii);
+    const int64_t ic = active ? i : 0;
+    const uint32_t pid = (uint32_t)(g.path0 + ic);
+    float* col = zbuf + (kSrc == 0 ? pslot : 0);
+    const bool upper = half == 1;
     const float sqdt = (float)sqrt(g.dt), dtf = (float)g.dt;
-    float cv = c, mcv = -c, omc = 1.0f - c, opc = 1.0f + c;
-    asm volatile("" : "+v"(cv), "+v"(mcv), "+v"(omc), "+v"(opc));
-    float lam[kTenors], ga[kTenors], gb[kTenors];
+    float cv = c, omc = 1.0f - c, a25 = upper ? 1.0f + c : 1.0f - c, b24 = upper ? -c : 0.f;
+    asm volatile("" : "+v"(cv), "+v"(omc), "+v"(a25), "+v"(b24));
+    const int tbase = 25 * half;
+    // a lane adds the seeds of its own tenors (the upper lane's copy of tenor 25 takes none)
+    const bool seeded = active;
+    auto add_seeds = [&](int e, float (&l)[kHalfSlots]) {
+        const float* q = gbar + ((int64_t)e * kTenors + tbase) * P + ic;
+#pragma unroll
+        for (int k = 0; k < kHalfSlots; ++k)
+            l[k] += (seeded && !(upper && k == 0)) ? q[(int64_t)k * P] : 0.f;
+    };
+    float lam[kHalfSlots], g0[kHalfSlots], g1[kHalfSlots], g2[kHalfSlots];
+#pragma unroll
+    for (int k = 0; k < kHalfSlots; ++k) lam[k] = g0[k] = g1[k] = g2[k] = 0.f;
     const int E = g.ndates_m1;
-#pragma unroll
-    for (int j = 0; j < kTenors; ++j) {
-        lam[j] = active ? gbar[((int64_t)E * kTenors + j) * P + i] : 0.f;
-        ga[j] = gb[j] = 0.f;
-    }
+    add_seeds(E, lam);
     float mu = 0.f;
+
+    auto flush = [&](float (&v)[kHalfSlots], int kind) {
+#pragma unroll
+        for (int k = 0; k < kHalfSlots; ++k) v[k] = half_wave_sum(v[k]);
+        if ((lane & 31) == 31) {
+#pragma unroll
+            for (int k = 0; k < kHalfSlots; ++k) atomicAdd(&acc[wave][half][kind][k], (double)v[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kHalfSlots; ++k) v[k] = 0.f;
+    };
+    auto step = [&](float z0, float z1, float z2) {
+        const float s0 = sqdt * z0, s1 = sqdt * z1, s2 = sqdt * z2;
+#pragma unroll
+        for (int k = 0; k < kHalfSlots; ++k) {
+            op_fmac_vv(g0[k], lam[k], s0);
+            op_fmac_vv(g1[k], lam[k], s1);
+            op_fmac_vv(g2[k], lam[k], s2);
+        }
+        const uint32_t l25 = __builtin_bit_cast(uint32_t, lam[kHalfSlots - 1]);
+        const float recv = __builtin_bit_cast(float, __builtin_amdgcn_permlane32_swap(l25, l25, false, false)[0]);
+        lam[0] = upper ? recv : lam[0];
+        // lambda <- T^T lambda (+ mu on tenor 0), descending
+        float t25, t24;
+        op_mul_vv(t25, cv, lam[24]);
+        op_mul_vv(t24, cv, lam[23]);
+        op_fma_vvv(t24, b24, lam[25], t24);
+        op_fma_inplace(lam[25], a25, t25);
+        op_fma_inplace(lam[24], omc, t24);
+        constexpr int G = 4;
+#pragma unroll
+        for (int k0 = 23; k0 >= 1; k0 -= G) {
+            float t[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                if (k0 - u >= 1) op_mul_vv(t[u], cv, lam[k0 - u - 1]);
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                if (k0 - u >= 1) op_fma_inplace(lam[k0 - u], omc, t[u]);
+        }
+        op_fma_inplace(lam[0], omc, mu);
+    };
+
+    const int S = g.stride;
+    constexpr int kPre = 5;
+    float zq[kPre][3];
+    auto loadz = [&](int n, float (&d)[3]) {
+        const float* q = zs + (int64_t)n * 3 * P + ic;
+        d[0] = q[0];
+        d[1] = q[P];
+        d[2] = q[2 * P];
+    };
+    if constexpr (kSrc == 2) {
+#pragma unroll
+        for (int u = 0; u < kPre; ++u) loadz(E * S - 1 - u, zq[u]);
+    }
+    const int whole = S / kGroupSteps * kGroupSteps;
     for (int e = E - 1; e >= 0; --e) {
-        mu += active ? dtf * beta[(int64_t)(e + 1) * P + i] : 0.f;
-        // one reverse Euler step with the step's normals
-        auto step = [&](int n, float z0, float z1, float z2) {
-            (void)n;
-            (void)z0;
-            (void)z1;
-            (void)z2;
-            if constexpr (kPass == 0) {
-                const float s0 = sqdt * z0, s1 = sqdt * z1;
+        mu += (active && !upper) ? dtf * beta[(int64_t)(e + 1) * P + ic] : 0.f;
 #pragma unroll
-                for (int j = 0; j < kTenors; ++j) {
-                    op_fmac_vv(ga[j], lam[j], s0);
-                    op_fmac_vv(gb[j], lam[j], s1);
-                }
-            } else {
-                const float s2 = sqdt * z2;
-#pragma unroll
-                for (int j = 0; j < kTenors; ++j) {
-                    op_fmac_vv(ga[j], lam[j], s2);
-                    op_add_inplace(gb[j], lam[j]);
-                }
-            }
-            // lambda <- T^T lambda (+ mu on tenor 0), descending
-            float t50, t49;
-            op_fma_vvv(t50, cv, lam[49], 0.f);
-            op_fma_vvv(t49, cv, lam[48], 0.f);
-            op_fma_vvv(t49, mcv, lam[50], t49);
-            op_fma_inplace(lam[50], opc, t50);
-            op_fma_inplace(lam[49], omc, t49);
-            constexpr int G = 4;
-#pragma unroll
-            for (int j0 = 48; j0 >= 1; j0 -= G) {
-                float t[G];
-#pragma unroll
-                for (int u = 0; u < G; ++u)
-                    if (j0 - u >= 1) op_fma_vvv(t[u], cv, lam[j0 - u - 1], 0.f);
-#pragma unroll
-                for (int u = 0; u < G; ++u)
-                    if (j0 - u >= 1) op_fma_inplace(lam[j0 - u], omc, t[u]);
-            }
-            op_fma_inplace(lam[0], omc, mu);
-        };
+        for (int k = 0; k < kHalfSlots; ++k) lacc[k][threadIdx.x] += lam[k];
         if constexpr (kSrc == 2) {
-            // stored normals, read kPre steps ahead so the HBM latency hides behind the
-            // previous steps' arithmetic (the matrix-core forward runs 50 steps per date)
-            constexpr int kPre = 10;
-            float cur[kPre][2], nxt[kPre][2];
-            auto load = [&](int n_hi, float (&d)[kPre][2]) {
+            for (int s0 = S - 1; s0 >= 0; s0 -= kPre) {
 #pragma unroll
                 for (int u = 0; u < kPre; ++u) {
-                    const float* q = zs + (int64_t)(n_hi - u) * 3 * P + ii;
-                    if constexpr (kPass == 0) {
-                        d[u][0] = q[0];
-                        d[u][1] = q[P];
-                    } else {
-                        d[u][0] = q[2 * P];
-                        d[u][1] = 0.f;
-                    }
-                }
-            };
-            load(e * g.stride + g.stride - 1, cur);
-            for (int s0 = g.stride - 1; s0 >= 0; s0 -= kPre) {
-                if (s0 - kPre >= 0) load(e * g.stride + s0 - kPre, nxt);
-#pragma unroll
-                for (int u = 0; u < kPre; ++u) {
-                    const int n = e * g.stride + s0 - u;
-                    if constexpr (kPass == 0) step(n, cur[u][0], cur[u][1], 0.f);
-                    else step(n, 0.f, 0.f, cur[u][0]);
-                }
-#pragma unroll
-                for (int u = 0; u < kPre; ++u) {
-                    cur[u][0] = nxt[u][0];
-                    cur[u][1] = nxt[u][1];
+                    const float a0 = zq[u][0], a1 = zq[u][1], a2 = zq[u][2];
+                    const int nn = e * S + s0 - kPre - u;
+                    if (nn >= 0) loadz(nn, zq[u]);
+                    step(a0, a1, a2);
                 }
             }
         } else {
-            const int whole = g.stride / kGroupSteps * kGroupSteps;
-            for (int s = g.stride - 1; s >= 0; --s) {
-                const int n = e * g.stride + s;
+            for (int s = S - 1; s >= 0; --s) {
+                const int n = e * S + s;
                 if constexpr (kSrc == 1) {
-                    const double* q = normals + ((int64_t)ii * g.nsteps + n) * 3;
-                    step(n, (float)q[0], (float)q[1], (float)q[2]);
+                    const double* q = normals + (ic * g.nsteps + n) * 3;
+                    step((float)q[0], (float)q[1], (float)q[2]);
                 } else {
-                    // groups of 16 steps staged in LDS, entered from their last step
                     Normals3 z;
                     if (s < whole) {
-                        if ((s & (kGroupSteps - 1)) == kGroupSteps - 1)
-                            fill_group(g.seed_lo, g.seed_hi, pid, (uint32_t)e, s / kGroupSteps, col, kBlock);
-                        z = group_step(col, kBlock, s);
+                        if ((s & (kGroupSteps - 1)) == kGroupSteps - 1) {
+                            __syncthreads();
+                            fill_group_half(g.seed_lo, g.seed_hi, pid, (uint32_t)e, s / kGroupSteps,
+                                            half, col, kAdjPaths);
+                            __syncthreads();
+                        }
+                        z = group_step(col, kAdjPaths, s);
                     } else {
-                        z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n, (uint32_t)g.stride);
+                        z = step_normals(g.seed_lo, g.seed_hi, pid, (uint32_t)n, (uint32_t)S);
                     }
-                    step(n, z.z0, z.z1, z.z2);
+                    step(z.z0, z.z1, z.z2);
                 }
             }
         }
         // date e: the valuation at e depends on f(n_e)
-#pragma unroll
-        for (int j = 0; j < kTenors; ++j)
-            lam[j] += active ? gbar[((int64_t)e * kTenors + j) * P + i] : 0.f;
-#pragma unroll
-        for (int j = 0; j < kTenors; ++j) {
-            const double a = wave_sum((double)ga[j]), bb = wave_sum((double)gb[j]);
-            if (lane == 0) {
-                acc[wave][j] += a;
-                acc[wave][kTenors + j] += bb;
-            }
-            ga[j] = gb[j] = 0.f;
+        add_seeds(e, lam);
+        if ((e & 1) == 0) {
+            flush(g0, 0);
+            flush(g1, 1);
+            flush(g2, 2);
         }
     }
-    if constexpr (kPass == 1) {
+    {
+        float la[kHalfSlots];
 #pragma unroll
-        for (int j = 0; j < kTenors; ++j) {
-            const double r = wave_sum((double)lam[j]);
-            if (lane == 0) acc[wave][2 * kTenors + j] += r;
-        }
+        for (int k = 0; k < kHalfSlots; ++k) la[k] = lacc[k][threadIdx.x];
+        flush(la, 3);
+        flush(lam, 4);
     }
     __syncthreads();
-    constexpr int nacc = (kPass == 0) ? 2 * kTenors : 3 * kTenors;
-    for (int k = threadIdx.x; k < nacc; k += kBlock) {
+    for (int o = threadIdx.x; o < kAdjKinds * kTenors; o += kBlock) {
+        const int kind = o / kTenors, t = o % kTenors;
+        const int h = t > 25 ? 1 : 0, k = t - 25 * h;
         double s = 0.0;
 #pragma unroll
-        for (int ww = 0; ww < kBlock / kWave; ++ww) s += acc[ww][k];
-        partials[(int64_t)blockIdx.x * nacc + k] = s;
+        for (int w = 0; w < kBlock / kWave; ++w) s += acc[w][h][kind][k];
+        partials[(int64_t)blockIdx.x * (kAdjKinds * kTenors) + o] = s;
     }
 }
 
@@ -859,26 +868,20 @@ hipError_t launch_pf_adjoint_seed_f32(const PfValueParams& v, const float* curve
     return pf_adj_seed_launch<float>(v, curves, logb, coef, a, kmax, w, gbar, beta, st);
 }
 
-hipError_t launch_pf_adjoint_sweep(const PfGrid& g, double c, int pass, const double* normals,
-                                   const float* zs, const float* gbar, const float* beta,
-                                   double* partials, hipStream_t st)
+int64_t pf_adjoint_blocks(int64_t paths) { return (paths + kAdjPaths - 1) / kAdjPaths; }
+
+hipError_t launch_pf_adjoint(const PfGrid& g, double c, const double* normals, const float* zs,
+                             const float* gbar, const float* beta, double* partials, hipStream_t st)
 {
-    const dim3 grid((unsigned)((g.paths + kBlock - 1) / kBlock));
+    const dim3 grid((unsigned)pf_adjoint_blocks(g.paths));
     const float cf = step_ratio_f32(c);  // 1 - cf, 1 + cf exact, as in the FP32 forward
-    if (zs && g.stride % 10 != 0) return hipErrorInvalidValue;  // stored-normal prefetch chunks
-#define HJM_SWEEP(P_, S_) \
-    pf_adjoint_sweep_kernel<P_, S_><<<grid, kBlock, 0, st>>>(g, cf, normals, zs, gbar, beta, partials)
-    const int src = normals ? 1 : (zs ? 2 : 0);
-    if (pass == 0) {
-        if (src == 1) HJM_SWEEP(0, 1);
-        else if (src == 2) HJM_SWEEP(0, 2);
-        else HJM_SWEEP(0, 0);
-    } else {
-        if (src == 1) HJM_SWEEP(1, 1);
-        else if (src == 2) HJM_SWEEP(1, 2);
-        else HJM_SWEEP(1, 0);
-    }
-#undef HJM_SWEEP
+    if (zs && g.stride % 5 != 0) return hipErrorInvalidValue;  // stored-normal read-ahead
+    if (normals)
+        pf_adjoint_kernel<1><<<grid, kBlock, 0, st>>>(g, cf, normals, zs, gbar, beta, partials);
+    else if (zs)
+        pf_adjoint_kernel<2><<<grid, kBlock, 0, st>>>(g, cf, normals, zs, gbar, beta, partials);
+    else
+        pf_adjoint_kernel<0><<<grid, kBlock, 0, st>>>(g, cf, normals, zs, gbar, beta, partials);
     return hipGetLastError();
 }
 

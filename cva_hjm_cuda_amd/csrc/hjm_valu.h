@@ -51,6 +51,11 @@ __device__ __forceinline__ void op_fma_vvv(float& d, float a, float b, float c)
 {
     asm("v_fma_f32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
 }
+// d = a * b with a, b in VGPRs
+__device__ __forceinline__ void op_mul_vv(float& d, float a, float b)
+{
+    asm("v_mul_f32 %0, %1, %2" : "=v"(d) : "v"(a), "v"(b));
+}
 // d += a * b with a, b in VGPRs, in place
 __device__ __forceinline__ void op_fmac_vv(float& d, float a, float b)
 {

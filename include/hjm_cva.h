@@ -56,6 +56,14 @@ enum hjm_path_kernel {
     HJM_KERNEL_MATRIX = 2  /* FP32 only: 50-step blocks as MFMA GEMMs (else HJM_ERR_UNSUPPORTED) */
 };
 
+/* How the adjoint's reverse sweep gets the forward pass's shocks. */
+enum hjm_adjoint_normals {
+    HJM_ADJ_NORMALS_AUTO = 0,       /* the matrix-core forward keeps them (paths x steps x 3
+                                       FP32) when that much device memory is free; otherwise,
+                                       and after a VALU forward, the sweep regenerates them */
+    HJM_ADJ_NORMALS_REGENERATE = 1  /* always regenerate (no stored-shock buffer) */
+};
+
 /* Market data (simulation_gpu.h:10 arguments accrual, spot_rates, drift, volatilities). */
 typedef struct {
     const double* spot_rates;   /* [51] initial forward curve f(0, tau_j) */
@@ -86,6 +94,7 @@ typedef struct {
     double pfe_quantile;    /* e.g. 0.975; <= 0 skips the PFE selection */
     int32_t device;         /* HIP device ordinal */
     int32_t path_kernel;    /* enum hjm_path_kernel (reference and portfolio path kernels) */
+    int32_t adjoint_normals; /* enum hjm_adjoint_normals (the CVA Greeks calls only) */
 } hjm_config;
 
 /* Outputs; every pointer may be NULL (then that output is not produced / copied). */
@@ -160,6 +169,8 @@ typedef struct {
     double* d_vol_total;  /* [153] d_vol + (d m / d sigma)^T d_drift, m the cumulative-
                              Simpson HJM drift of the loadings (hjm_hjm_drift) */
     double* timings_ms;   /* [3] forward (all kernels), adjoint (all kernels), total */
+    int64_t* stored_normals_bytes; /* [1] device bytes of the shocks the matrix-core forward
+                                      kept for the sweep; 0 when the sweep regenerated them */
 } hjm_greeks;
 int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
                              const hjm_config* config, hjm_greeks* greeks);

@@ -157,6 +157,21 @@ def test_gpu_fp32_forward_greeks_close_to_fp64():
 
 
 @pytest.mark.gpu
+def test_gpu_stored_and_regenerated_shocks_give_the_same_greeks():
+    """After the matrix-core FP32 forward the sweep reads the forward's stored shocks; asked
+    to, it regenerates them from the generator instead: the same Greeks."""
+    _flush("[gpu] adjoint: stored vs regenerated shocks, 8192 paths 10y FP32")
+    m = hjm.benchmark_market()
+    swap = hjm.InterestRateSwap(1e6, 0.025, 10.0)
+    a = hjm.cva_greeks(m, swap, 8192, seed=3, precision=hjm.FP32_KAHAN)
+    b = hjm.cva_greeks(m, swap, 8192, seed=3, precision=hjm.FP32_KAHAN, regenerate_normals=True)
+    assert a.stored_normals_bytes == 8192 * 1000 * 3 * 4 and b.stored_normals_bytes == 0
+    assert a.cva == b.cva
+    for x, y in ((a.d_spot, b.d_spot), (a.d_vol, b.d_vol), (a.d_drift, b.d_drift)):
+        assert np.allclose(x, y, rtol=1e-12, atol=0)
+
+
+@pytest.mark.gpu
 def test_gpu_autograd_wrapper():
     _flush("[gpu] torch.autograd wrapper")
     from cva_hjm_cuda_amd.autograd import cva as cva_fn
