@@ -103,6 +103,21 @@ def run_portfolio(hjm, args, nswaps, nsets, paths):
     wall = time.perf_counter() - t0
     plan.close()
     kp = float(np.mean(times[:, 0]))
+    adj = None
+    if args.config == "full":
+        # CVA + 204 Greeks over the whole book (forward keeping its shocks, one-pass reverse
+        # sweep) against a forward-only plan of the same size (no PFE)
+        log("[bench] full configuration: CVA + 204 Greeks")
+        g = hjm.portfolio_cva_greeks(market, pf, paths, 30.0, DT, **mk)
+        fo = hjm.PortfolioPlan(market, pf, paths, 30.0, DT, pfe_quantile=0.0,
+                               tuning=hjm.Tuning(256, 0, False), **mk)
+        fo.execute()
+        only = float(np.median([fo.execute().timings_ms[4] for _ in range(3)]))
+        fo.close()
+        adj = {"paths": paths, "steps_per_path": S, "forward_only_ms": only,
+               "greeks_forward_ms": float(g.timings_ms[0]), "adjoint_ms": float(g.timings_ms[1]),
+               "cva_plus_greeks_over_forward": float(g.timings_ms[2]) / only,
+               "stored_normals_bytes": g.stored_normals_bytes, "greeks": 204}
     flops = PF_FLOPS_PER_PATH_STEP * paths * S
     achieved = flops / (kp * 1e-3) / 1e12
     value = paths * S * args.steps / wall
@@ -124,6 +139,7 @@ def run_portfolio(hjm, args, nswaps, nsets, paths):
                                 "statistics": float(np.mean(times[:, 2])),
                                 "pfe_select": float(np.mean(times[:, 3])),
                                 "device_total_eager": float(np.mean(times[:, 4]))},
+        "adjoint": adj,
         "cpu_baseline": None,
     }
 

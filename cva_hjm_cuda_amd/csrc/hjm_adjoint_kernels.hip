@@ -30,55 +30,56 @@
 namespace hjm {
 
 // Seeds of the sweep from the forward pass's numeraire pairs (FP64): CF_i, V_t, the
-// survival-weighted indicator sums u_i and gx_i, gA_i, stored [slot][path] in FP32.
+// survival-weighted indicator sums u_i and gx_i, gA_i, stored [slot][path] in FP32; and
+// per workgroup the sums over its paths of gA at every slot, sa_partials[block][51]
+// (slot 0 and slots past the last date 0), for the drift gradient's mu terms.
 __global__ __launch_bounds__(kBlock) void ref_adjoint_seed_kernel(const ExposureParams e,
                                                                   double dt,
                                                                   const double* __restrict__ nx,
                                                                   const double* __restrict__ ny,
                                                                   float* __restrict__ gx,
-                                                                  float* __restrict__ gA)
+                                                                  float* __restrict__ gA,
+                                                                  double* __restrict__ sa_partials)
 {
+    __shared__ double red[kBlock / kWave][kTenors];
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= e.paths) return;
-    const int64_t P = e.paths;
+    const bool active = i < e.paths;
+    const int64_t P = e.paths, ic = active ? i : 0;
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
     // V_t = sum_{i > t} CF_i, built backwards; u_i needs V_0..V_{i-1}, so first collect
     // the signs of V_t in a bit mask (D <= 50 dates).
     uint64_t pos = 0;
     double V = 0.0;
     for (int t = e.ndates - 1; t >= 0; --t) {
         const int s = t + 1;
-        const double x = nx[(int64_t)s * P + i], y = ny[(int64_t)s * P + i];
+        const double x = nx[(int64_t)s * P + ic], y = ny[(int64_t)s * P + ic];
         V += y * e.notional * e.accrual[s] * (e.m * (exp(x / e.m) - 1.0) - e.K);
         if (V > 0.0) pos |= (1ull << t);
     }
     double u = 0.0;
     for (int s = 1; s <= e.ndates; ++s) {
         if (pos & (1ull << (s - 1))) u += e.w[s - 1];
-        const double x = nx[(int64_t)s * P + i], y = ny[(int64_t)s * P + i];
+        const double x = nx[(int64_t)s * P + ic], y = ny[(int64_t)s * P + ic];
         const double base = y * e.notional * e.accrual[s];
         const double cf = base * (e.m * (exp(x / e.m) - 1.0) - e.K);
-        gx[(int64_t)s * P + i] = (float)(u * base * exp(x / e.m));
-        gA[(int64_t)s * P + i] = (float)(-u * dt * cf);
+        const float a = active ? (float)(-u * dt * cf) : 0.f;
+        if (active) {
+            gx[(int64_t)s * P + i] = (float)(u * base * exp(x / e.m));
+            gA[(int64_t)s * P + i] = a;
+        }
+        const double ws = wave_sum((double)a);
+        if (lane == 0) red[wave][s] = ws;
     }
-}
-
-// out[r] = sum of row r of an FP32 [rows][n] array, in FP64 and a fixed order (one
-// workgroup per row).
-__global__ __launch_bounds__(kBlock) void row_sum_f32_kernel(const float* __restrict__ x,
-                                                             int64_t n, double* __restrict__ out)
-{
-    __shared__ double red[kBlock / kWave];
-    const float* row = x + (int64_t)blockIdx.x * n;
-    double s = 0.0;
-    for (int64_t k = threadIdx.x; k < n; k += kBlock) s += (double)row[k];
-    s = wave_sum(s);
-    if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = s;
+    if (lane == 0) {
+        red[wave][0] = 0.0;
+        for (int s = e.ndates + 1; s < kTenors; ++s) red[wave][s] = 0.0;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < kTenors) {
         double t = 0.0;
 #pragma unroll
-        for (int w = 0; w < kBlock / kWave; ++w) t += red[w];
-        out[blockIdx.x] = t;
+        for (int w = 0; w < kBlock / kWave; ++w) t += red[w][threadIdx.x];
+        sa_partials[(int64_t)blockIdx.x * kTenors + threadIdx.x] = t;
     }
 }
 
@@ -275,16 +276,11 @@ __global__ __launch_bounds__(kBlock) void ref_adjoint_kernel(const AdjGrid g,
 }
 
 hipError_t launch_ref_adjoint_seed(const ExposureParams& e, double dt, const double* nx,
-                                   const double* ny, float* gx, float* gA, hipStream_t st)
+                                   const double* ny, float* gx, float* gA, double* sa_partials,
+                                   hipStream_t st)
 {
     const dim3 grid((unsigned)((e.paths + kBlock - 1) / kBlock));
-    ref_adjoint_seed_kernel<<<grid, kBlock, 0, st>>>(e, dt, nx, ny, gx, gA);
-    return hipGetLastError();
-}
-
-hipError_t launch_row_sum_f32(const float* x, int rows, int64_t n, double* out, hipStream_t st)
-{
-    row_sum_f32_kernel<<<rows, kBlock, 0, st>>>(x, n, out);
+    ref_adjoint_seed_kernel<<<grid, kBlock, 0, st>>>(e, dt, nx, ny, gx, gA, sa_partials);
     return hipGetLastError();
 }
 

@@ -430,7 +430,7 @@ int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
     struct Cleanup {
         hjm_plan p;
         float *gx = nullptr, *gA = nullptr, *zs = nullptr;
-        double *part = nullptr, *sums = nullptr;
+        double *part = nullptr, *sums = nullptr, *sap = nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         ~Cleanup()
         {
@@ -439,6 +439,7 @@ int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
             dev_free(gA);
             dev_free(part);
             dev_free(sums);
+            dev_free(sap);
             if (e0) (void)hipEventDestroy(e0);
             if (e1) (void)hipEventDestroy(e1);
             hjm_plan_destroy(p);
@@ -453,6 +454,7 @@ int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
     HJM_TRY(dev_alloc(&c.gA, cells));
     HJM_TRY(dev_alloc(&c.part, (size_t)nb * kOut));
     HJM_TRY(dev_alloc(&c.sums, kOut + kTenors));  // then the per-slot sums of gA
+    HJM_TRY(dev_alloc(&c.sap, (size_t)p->nblocks * kTenors));
     HJM_HIP(hipEventCreate(&c.e0), HJM_ERR_DEVICE);
     HJM_HIP(hipEventCreate(&c.e1), HJM_ERR_DEVICE);
     // the matrix-core forward keeps its normals for the reverse sweep (one FP32 per factor
@@ -465,9 +467,10 @@ int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
     HJM_HIP(hipEventRecord(c.e0, st), HJM_ERR_DEVICE);
     const double dt = p->grid.dt;
     const double cc = dt / swap->dtau;
-    HJM_HIP(launch_ref_adjoint_seed(p->ep, dt, p->d_nx, p->d_ny, c.gx, c.gA, st), HJM_ERR_LAUNCH);
-    // slots 1..D of gA summed over the paths (the drift gradient's mu terms)
-    HJM_HIP(launch_row_sum_f32(c.gA + P, D, P, c.sums + kOut, st), HJM_ERR_LAUNCH);
+    HJM_HIP(launch_ref_adjoint_seed(p->ep, dt, p->d_nx, p->d_ny, c.gx, c.gA, c.sap, st),
+            HJM_ERR_LAUNCH);
+    // gA summed over the paths at every slot (the drift gradient's mu terms)
+    HJM_HIP(launch_column_sum(c.sap, p->nblocks, kTenors, c.sums + kOut, st), HJM_ERR_LAUNCH);
     HJM_HIP(launch_ref_adjoint(p->grid, cc, sqrt(dt), p->d_normals, c.zs, c.gx, c.gA, c.part, st),
             HJM_ERR_LAUNCH);
     HJM_HIP(launch_column_sum(c.part, nb, kOut, c.sums, st), HJM_ERR_LAUNCH);
@@ -477,8 +480,7 @@ int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
     r.cva_stderr = gr->cva_stderr;
     HJM_TRY(plan_collect(p, &r));
     double sums[kOut + kTenors];
-    HJM_HIP(hipMemcpy(sums, c.sums, sizeof(double) * (kOut + D), hipMemcpyDeviceToHost),
-            HJM_ERR_DEVICE);
+    HJM_HIP(hipMemcpy(sums, c.sums, sizeof(sums), hipMemcpyDeviceToHost), HJM_ERR_DEVICE);
     const double inv = 1.0 / (double)P;
     double dv[kFactors * kTenors], dm[kTenors];
     for (int k = 0; k < kFactors * kTenors; ++k) dv[k] = sums[k] * inv;
@@ -489,7 +491,7 @@ int hjm_cva_greeks_reference(const hjm_market* market, const hjm_swap* swap,
         std::vector<double> A((size_t)kTenors * kTenors), B((size_t)kTenors * kTenors);
         adjoint_block_sums((double)step_ratio_f32(cc), S, A.data(), B.data());
         double M[kTenors] = {0.0};
-        for (int slot = 2; slot <= D; ++slot) M[slot] = (double)(slot - 1) * sums[kOut + slot - 1];
+        for (int slot = 2; slot <= D; ++slot) M[slot] = (double)(slot - 1) * sums[kOut + slot];
         const double* L = sums + 3 * kTenors;
         for (int j = 0; j < kTenors; ++j) {
             double a = 0.0;

@@ -497,7 +497,7 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
     struct Cleanup {
         hjm_portfolio_plan p;
         float *gbar = nullptr, *beta = nullptr, *zs = nullptr;
-        double *part = nullptr, *sums = nullptr;
+        double *part = nullptr, *sums = nullptr, *bp = nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         ~Cleanup()
         {
@@ -506,6 +506,7 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
             dev_free(beta);
             dev_free(part);
             dev_free(sums);
+            dev_free(bp);
             if (e0) (void)hipEventDestroy(e0);
             if (e1) (void)hipEventDestroy(e1);
             hjm_portfolio_plan_destroy(p);
@@ -520,6 +521,8 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
     HJM_TRY(dev_alloc(&c.beta, (size_t)nd * P));
     HJM_TRY(dev_alloc(&c.part, (size_t)nb * kOut));
     HJM_TRY(dev_alloc(&c.sums, (size_t)kOut + kMaxDates));  // then the per-date sums of beta
+    const int64_t tiles = (P + kBlock - 1) / kBlock;
+    HJM_TRY(dev_alloc(&c.bp, (size_t)tiles * nd));
     HJM_HIP(hipEventCreate(&c.e0), HJM_ERR_DEVICE);
     HJM_HIP(hipEventCreate(&c.e1), HJM_ERR_DEVICE);
     // the matrix-core forward keeps its normals for the reverse sweep; without the memory
@@ -533,14 +536,14 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
     const bool f64 = p->precision == HJM_FP64;
     hipError_t err = f64 ? launch_pf_adjoint_seed_f64(p->vp, (const double*)p->d_curves, p->d_logb,
                                                       (const double*)p->d_coef, (const double*)p->d_a,
-                                                      p->d_kmax, p->d_w, c.gbar, c.beta, st)
+                                                      p->d_kmax, p->d_w, c.gbar, c.beta, c.bp, st)
                          : launch_pf_adjoint_seed_f32(p->vp, (const float*)p->d_curves, p->d_logb,
                                                       (const float*)p->d_coef, (const float*)p->d_a,
-                                                      p->d_kmax, p->d_w, c.gbar, c.beta, st);
+                                                      p->d_kmax, p->d_w, c.gbar, c.beta, c.bp, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
     const double cc = p->grid.dt / portfolio->dtau;
-    if (E > 0)
-        HJM_HIP(launch_row_sum_f32(c.beta + P, E, P, c.sums + kOut, st), HJM_ERR_LAUNCH);
+    // beta summed over the paths at every date (the drift gradient's bank-account terms)
+    HJM_HIP(launch_column_sum(c.bp, tiles, nd, c.sums + kOut, st), HJM_ERR_LAUNCH);
     HJM_HIP(launch_pf_adjoint(p->grid, cc, p->d_normals, c.zs, c.gbar, c.beta, c.part, st),
             HJM_ERR_LAUNCH);
     HJM_HIP(launch_column_sum(c.part, nb, kOut, c.sums, st), HJM_ERR_LAUNCH);
@@ -550,7 +553,7 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
     r.cva = cvas.data();
     HJM_TRY(pplan_collect(p, &r));
     std::vector<double> sums((size_t)kOut + kMaxDates, 0.0);
-    HJM_HIP(hipMemcpy(sums.data(), c.sums, sizeof(double) * (kOut + E), hipMemcpyDeviceToHost),
+    HJM_HIP(hipMemcpy(sums.data(), c.sums, sizeof(double) * (kOut + nd), hipMemcpyDeviceToHost),
             HJM_ERR_DEVICE);
     double total = 0.0;
     for (double v : cvas) total += v;
@@ -566,7 +569,7 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
         std::vector<double> A((size_t)kTenors * kTenors), B((size_t)kTenors * kTenors);
         adjoint_block_sums((double)step_ratio_f32(cc), p->grid.stride, A.data(), B.data());
         double musum = 0.0;
-        for (int e = 1; e <= E; ++e) musum += (double)e * sums[kOut + e - 1];
+        for (int e = 1; e <= E; ++e) musum += (double)e * sums[kOut + e];
         musum *= dt;
         const double* L = sums.data() + 3 * kTenors;
         for (int j = 0; j < kTenors; ++j) {

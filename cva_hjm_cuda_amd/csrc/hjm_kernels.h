@@ -38,8 +38,10 @@ hipError_t launch_pf_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, float*
                                 double* logb, float* zs, hipStream_t st);
 
 // Pathwise adjoint of the reference-semantics CVA.
+// sa_partials: [blocks of 256 paths][51] per-slot sums of gA over the block's paths
 hipError_t launch_ref_adjoint_seed(const ExposureParams& e, double dt, const double* nx,
-                                   const double* ny, float* gx, float* gA, hipStream_t st);
+                                   const double* ny, float* gx, float* gA, double* sa_partials,
+                                   hipStream_t st);
 // One-pass sweep, two lanes per path. normals: host FP64 [path][step][3]; else zs: stored
 // FP32 [step][3][path]; else regenerate. partials: [ref_adjoint_blocks(paths)][5 * 51]
 // (loading gradients of factors 0, 1, 2, sum of the block-end adjoints, d J / d spot).
@@ -47,8 +49,6 @@ int64_t ref_adjoint_blocks(int64_t paths);
 hipError_t launch_ref_adjoint(const RefGrid& rg, double c, double sqrt_dt, const double* normals,
                               const float* zs, const float* gx, const float* gA, double* partials,
                               hipStream_t st);
-// out[r] = FP64 sum of row r of an FP32 [rows][n] array
-hipError_t launch_row_sum_f32(const float* x, int rows, int64_t n, double* out, hipStream_t st);
 
 // Portfolio semantics.
 hipError_t launch_pf_paths_f64(const RefStepConsts<double>& k, const PfGrid& g, const double* normals,
@@ -86,14 +86,15 @@ hipError_t launch_bracket_finish(const double* x, int rows, int64_t n, int64_t k
 hipError_t launch_bracket_finish_f32(const float* x, int rows, int64_t n, int64_t k, void* ws,
                                      double* out, hipStream_t st);
 
+// beta_partials: [blocks of 256 paths][ndates] sums of beta over the block's paths
 hipError_t launch_pf_adjoint_seed_f64(const PfValueParams& v, const double* curves,
                                       const double* logb, const double* coef, const double* a,
                                       const int* kmax, const double* w, float* gbar, float* beta,
-                                      hipStream_t st);
+                                      double* beta_partials, hipStream_t st);
 hipError_t launch_pf_adjoint_seed_f32(const PfValueParams& v, const float* curves,
                                       const double* logb, const float* coef, const float* a,
                                       const int* kmax, const double* w, float* gbar, float* beta,
-                                      hipStream_t st);
+                                      double* beta_partials, hipStream_t st);
 // One-pass sweep, two lanes per path; partials [pf_adjoint_blocks(paths)][5 * 51] as the
 // reference sweep's.
 int64_t pf_adjoint_blocks(int64_t paths);

@@ -554,7 +554,8 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_seed_kernel(const PfValuePa
                                                                  const int* __restrict__ kmax,
                                                                  const double* __restrict__ w,
                                                                  float* __restrict__ gbar,
-                                                                 float* __restrict__ beta)
+                                                                 float* __restrict__ beta,
+                                                                 double* __restrict__ beta_partials)
 {
     const int64_t P = v.paths;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -654,6 +655,19 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_seed_kernel(const PfValuePa
         S += iota;
         Sk[kk] = S;
         if (kk > kTenors - 1) ext += (Real)(kk - (kTenors - 1)) * iota;
+    }
+    // the workgroup's sum of beta at this date (the drift gradient's bank-account terms)
+    {
+        __shared__ double bred[kBlock / kWave];
+        const double bs = wave_sum(live ? (double)(float)b : 0.0);
+        if (threadIdx.x % kWave == 0) bred[threadIdx.x / kWave] = bs;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+#pragma unroll
+            for (int q = 0; q < kBlock / kWave; ++q) t += bred[q];
+            beta_partials[(int64_t)blockIdx.x * v.ndates + e] = t;
+        }
     }
     if (!live) return;
     float* gb = gbar + (int64_t)e * kTenors * P + i;
@@ -844,28 +858,31 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_kernel(const PfGrid g, floa
 template <typename Real>
 static hipError_t pf_adj_seed_launch(const PfValueParams& v, const Real* curves, const double* logb,
                                      const Real* coef, const Real* a, const int* kmax,
-                                     const double* w, float* gbar, float* beta, hipStream_t st)
+                                     const double* w, float* gbar, float* beta,
+                                     double* beta_partials, hipStream_t st)
 {
     const dim3 grid((unsigned)((v.paths + kBlock - 1) / kBlock), (unsigned)v.ndates);
     pf_adjoint_seed_kernel<Real><<<grid, kBlock, 0, st>>>(v, curves, logb, coef, a, kmax, w, gbar,
-                                                          beta);
+                                                          beta, beta_partials);
     return hipGetLastError();
 }
 
 hipError_t launch_pf_adjoint_seed_f64(const PfValueParams& v, const double* curves,
                                       const double* logb, const double* coef, const double* a,
                                       const int* kmax, const double* w, float* gbar, float* beta,
-                                      hipStream_t st)
+                                      double* beta_partials, hipStream_t st)
 {
-    return pf_adj_seed_launch<double>(v, curves, logb, coef, a, kmax, w, gbar, beta, st);
+    return pf_adj_seed_launch<double>(v, curves, logb, coef, a, kmax, w, gbar, beta, beta_partials,
+                                      st);
 }
 
 hipError_t launch_pf_adjoint_seed_f32(const PfValueParams& v, const float* curves,
                                       const double* logb, const float* coef, const float* a,
                                       const int* kmax, const double* w, float* gbar, float* beta,
-                                      hipStream_t st)
+                                      double* beta_partials, hipStream_t st)
 {
-    return pf_adj_seed_launch<float>(v, curves, logb, coef, a, kmax, w, gbar, beta, st);
+    return pf_adj_seed_launch<float>(v, curves, logb, coef, a, kmax, w, gbar, beta, beta_partials,
+                                     st);
 }
 
 int64_t pf_adjoint_blocks(int64_t paths) { return (paths + kAdjPaths - 1) / kAdjPaths; }
