@@ -15,7 +15,8 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config precision|b
 
 --config batched / full run the portfolio workloads instead (256 swaps in 32 netting sets
 x 262144 paths, resp. 1024 swaps in 128 sets x 1048576 paths, 30y, FP32 + compensated,
-autotuned persistent path kernel, one hipGraph replay per step).
+launch shapes of the path and valuation kernels chosen by the autotuner before the timed
+region, one hipGraph replay per step).
 """
 import argparse
 import json
@@ -81,20 +82,33 @@ def adjoint_ratio(hjm, market, reps=5):
             "cva_plus_greeks_over_own_forward": (fwd + adj) / fwd, "greeks": 204}
 
 
+def _eager_timings(hjm, market, pf, paths, tuning, mk, reps=3):
+    plan = hjm.PortfolioPlan(market, pf, paths, 30.0, DT, tuning=tuning, **mk)
+    plan.execute()
+    times = np.array([plan.execute().timings_ms.copy() for _ in range(reps)])
+    plan.close()
+    return times
+
+
 def run_portfolio(hjm, args, nswaps, nsets, paths):
     """Portfolio workload: the matrix-core FP32 path kernel, valuation, statistics and PFE
-    of every netting set, replayed as one hipGraph per step (timed on the wall clock);
-    the per-kernel breakdown comes from eager runs of the same plan."""
+    of every netting set at the launch shapes the autotuner picks for this problem
+    (hjm_portfolio_autotune, run before the timed region), replayed as one hipGraph per
+    step (timed on the wall clock); the per-kernel breakdown comes from eager runs of the
+    same tuning, beside the eager total at the default shapes."""
     market = hjm.benchmark_market()
     pf = hjm.synthetic_portfolio(nswaps, nsets, seed=2024)
     S = 3000
     mk = dict(precision=hjm.FP32_KAHAN, seed=2024)
-    eager = hjm.PortfolioPlan(market, pf, paths, 30.0, DT, tuning=hjm.Tuning(256, 0, False),
-                              **mk)
-    eager.execute()
-    times = np.array([eager.execute().timings_ms.copy() for _ in range(3)])
-    eager.close()
-    plan = hjm.PortfolioPlan(market, pf, paths, 30.0, DT, tuning=hjm.Tuning(256, 0, True), **mk)
+    t0 = time.perf_counter()
+    tune, _ = hjm.autotune_portfolio(market, pf, paths, 30.0, DT, precision=hjm.FP32_KAHAN,
+                                     seed=2024)
+    tune_s = time.perf_counter() - t0
+    log("[bench] autotuned shapes: %s (%.1f s)" % (tune.as_dict(), tune_s))
+    default = _eager_timings(hjm, market, pf, paths, hjm.Tuning(), mk)
+    eager_t = hjm.Tuning(**{**tune.as_dict(), "use_graph": False})
+    times = _eager_timings(hjm, market, pf, paths, eager_t, mk)
+    plan = hjm.PortfolioPlan(market, pf, paths, 30.0, DT, tuning=tune, **mk)
     for _ in range(max(1, args.warmup)):
         plan.execute()
     t0 = time.perf_counter()
@@ -108,9 +122,9 @@ def run_portfolio(hjm, args, nswaps, nsets, paths):
         # CVA + 204 Greeks over the whole book (forward keeping its shocks, one-pass reverse
         # sweep) against a forward-only plan of the same size (no PFE)
         log("[bench] full configuration: CVA + 204 Greeks")
-        g = hjm.portfolio_cva_greeks(market, pf, paths, 30.0, DT, **mk)
-        fo = hjm.PortfolioPlan(market, pf, paths, 30.0, DT, pfe_quantile=0.0,
-                               tuning=hjm.Tuning(256, 0, False), **mk)
+        g = hjm.portfolio_cva_greeks(market, pf, paths, 30.0, DT, tuning=eager_t, **mk)
+        fo = hjm.PortfolioPlan(market, pf, paths, 30.0, DT, pfe_quantile=0.0, tuning=eager_t,
+                               **mk)
         fo.execute()
         only = float(np.median([fo.execute().timings_ms[4] for _ in range(3)]))
         fo.close()
@@ -130,15 +144,20 @@ def run_portfolio(hjm, args, nswaps, nsets, paths):
         "config": {"workload": "portfolio: %d swaps in %d netting sets, %d paths x %d steps "
                                "(30y, dt=0.01), EE/PFE97.5/CVA per set" % (nswaps, nsets, paths, S),
                    "paths": paths, "steps_per_path": S, "swaps": nswaps, "netting_sets": nsets,
-                   "hip_graph": True},
+                   "hip_graph": True, "tuning": tune.as_dict(), "autotune_s": tune_s},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["f32"],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f32"], "traffic": None,
                      "kernel": "pf_paths_mfma_kernel", "kernel_ms": kp,
                      "flops_per_launch": flops},
+        # fused valuation-statistics pass (if tuned so): [1] is the valuation with the
+        # reductions, [2] the column sums
         "kernel_ms_breakdown": {"paths": kp, "valuation": float(np.mean(times[:, 1])),
                                 "statistics": float(np.mean(times[:, 2])),
                                 "pfe_select": float(np.mean(times[:, 3])),
-                                "device_total_eager": float(np.mean(times[:, 4]))},
+                                "device_total_eager": float(np.mean(times[:, 4])),
+                                "device_total_eager_default_shapes": float(np.mean(default[:, 4])),
+                                "paths_default_shapes": float(np.mean(default[:, 0])),
+                                "valuation_default_shapes": float(np.mean(default[:, 1]))},
         "adjoint": adj,
         "cpu_baseline": None,
     }

@@ -76,7 +76,10 @@ class _Portfolio(C.Structure):
 
 class _Tuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("use_graph", C.c_int32), ("reserved", C.c_int32)]
+                ("use_graph", C.c_int32), ("fuse_statistics", C.c_int32),
+                ("pfe_stage_slots", C.c_int32), ("path_tiles", C.c_int32),
+                ("path_waves", C.c_int32), ("value_tiles", C.c_int32),
+                ("value_waves", C.c_int32), ("value_stage", C.c_int32)]
 
 
 class _PfResults(C.Structure):
@@ -162,6 +165,9 @@ def _declare(lib):
     lib.hjm_calibrate_pca.argtypes = [C.POINTER(_Calibration), C.POINTER(_CalibResults)]
     lib.hjm_generate_normals.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_int32, C.c_int32,
                                          C.c_int32, _dp]
+    _fp32 = C.POINTER(C.c_float)
+    lib.hjm_box_muller_parts.argtypes = [C.POINTER(C.c_uint32), C.c_int64, C.c_int32, _fp32,
+                                         _fp32, _fp32]
 
 
 def _ptr(a: Optional[np.ndarray]):
@@ -440,6 +446,17 @@ def generate_normals(seed: int, paths: int, nsteps: int, path_offset: int = 0,
     return out
 
 
+def box_muller_parts(words: np.ndarray, device: int = 0):
+    """The generator's Box-Muller factors of 32-bit Philox words, on the GPU: (radius of
+    each word as the first of a pair, cosine and sine of each as the second), float32."""
+    w = np.ascontiguousarray(words, np.uint32)
+    out = [np.zeros(w.size, np.float32) for _ in range(3)]
+    fp = [o.ctypes.data_as(C.POINTER(C.c_float)) for o in out]
+    _check(library().hjm_box_muller_parts(w.ctypes.data_as(C.POINTER(C.c_uint32)), w.size,
+                                          int(device), *fp), "hjm_box_muller_parts")
+    return tuple(out)
+
+
 # ---- drop-in for calculateExposureMultiGPU (simulation_gpu.h:10,12) ---------------
 def calculate_exposure_multi_gpu(expected_exposure: np.ndarray, payOff: InterestRateSwap,
                                  accrual, spot_rates, drift, volatilities, scale: float,
@@ -474,12 +491,31 @@ def calculate_exposure_multi_gpu(expected_exposure: np.ndarray, payOff: Interest
 # ---- portfolio semantics ------------------------------------------------------------
 @dataclass
 class Tuning:
-    block_size: int = 256
-    blocks_per_cu: int = 0
+    """Launch shapes of a portfolio plan (0: the kernel's default; see hjm_tuning)."""
+    block_size: int = 256          # VALU path kernel
+    blocks_per_cu: int = 0         # persistent path kernel (matrix-core: 0 or 1)
     use_graph: bool = False
+    fuse_statistics: bool = False  # FP32: valuation and statistics in one pass
+    pfe_stage_slots: int = 0       # fused pass: 0 = 32
+    path_tiles: int = 0            # matrix-core path kernel: N-tiles per wave (0 = 2)
+    path_waves: int = 0            # ... waves per workgroup (0 = 8)
+    value_tiles: int = 0           # matrix-core valuation: N-tiles per wave (0 = 4)
+    value_waves: int = 0           # ... waves per workgroup (0 = 4)
+    value_stage: int = 0           # ... M-tiles staged in LDS at a time (0 = 8)
 
     def _c(self):
-        return _Tuning(self.block_size, self.blocks_per_cu, int(self.use_graph), 0)
+        return _Tuning(self.block_size, self.blocks_per_cu, int(self.use_graph),
+                       int(self.fuse_statistics), int(self.pfe_stage_slots), self.path_tiles,
+                       self.path_waves, self.value_tiles, self.value_waves, self.value_stage)
+
+    @staticmethod
+    def _from_c(t):
+        return Tuning(t.block_size, t.blocks_per_cu, bool(t.use_graph), bool(t.fuse_statistics),
+                      t.pfe_stage_slots, t.path_tiles, t.path_waves, t.value_tiles,
+                      t.value_waves, t.value_stage)
+
+    def as_dict(self):
+        return {k: (int(v) if isinstance(v, bool) else v) for k, v in self.__dict__.items()}
 
 
 @dataclass
@@ -561,18 +597,22 @@ def simulate_portfolio(market: Market, portfolio, paths: int, keep_paths: bool =
 
 def autotune_portfolio(market: Market, portfolio, paths: int, horizon: float = 30.0,
                        dt: float = 0.01, precision: int = FP32_KAHAN, seed: int = 2024,
-                       device: int = 0):
-    """Fastest (block size, persistent grid) for the VALU path kernel; returns (Tuning, ms).
-    The matrix-core path kernel (FP32, in-kernel normals, AUTO or MATRIX) has a fixed launch
-    shape and ignores the tuning."""
+                       recovery: float = 0.4, pfe_quantile: float = 0.975, device: int = 0,
+                       path_kernel: int = KERNEL_AUTO):
+    """Fastest launch shapes for this problem, measured on the device: returns (Tuning
+    with use_graph set, eager ms of one execute with it). Matrix-core FP32 plans tune the
+    path kernel (N-tiles, waves, persistent grid), the valuation (N-tiles, waves, LDS
+    stage) and the fused against the exposure pass; VALU path kernels their block size
+    and persistent grid (hjm_portfolio_autotune)."""
     mk = market._c()
-    pfc = _pf_c(portfolio, horizon, 0.4)
-    cfg = _config(paths, dt, precision, RNG_PHILOX, seed, None, 0.0, 0.4, 0.0, device, 0)
+    pfc = _pf_c(portfolio, horizon, recovery)
+    cfg = _config(paths, dt, precision, RNG_PHILOX, seed, None, 0.0, recovery, pfe_quantile,
+                  device, 0, path_kernel)
     t = _Tuning()
     ms = C.c_double()
     _check(library().hjm_portfolio_autotune(C.byref(mk), C.byref(pfc), C.byref(cfg), C.byref(t),
                                             C.byref(ms)), "hjm_portfolio_autotune")
-    return Tuning(t.block_size, t.blocks_per_cu, bool(t.use_graph)), ms.value
+    return Tuning._from_c(t), ms.value
 
 
 def portfolio_cva_greeks(market: Market, portfolio, paths: int, horizon: float = 30.0,

@@ -679,6 +679,34 @@ int hjm_generate_normals(uint64_t seed, int64_t path_offset, int64_t paths, int3
     return rc;
 }
 
+int hjm_box_muller_parts(const uint32_t* words, int64_t n, int32_t device, float* radius,
+                         float* cosine, float* sine)
+{
+    if (!words || !radius || !cosine || !sine) return HJM_ERR_NULL;
+    if (n <= 0 || n > (1ll << 28)) return HJM_ERR_ARG;
+    DeviceGuard guard;
+    HJM_TRY(guard.set(device));
+    uint32_t* dw = nullptr;
+    float* d = nullptr;
+    HJM_TRY(dev_alloc(&dw, (size_t)n));
+    int rc = dev_alloc(&d, 3 * (size_t)n);
+    if (rc == HJM_OK) {
+        const size_t b = (size_t)n * sizeof(float);
+        if (hipMemcpy(dw, words, (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+            rc = HJM_ERR_DEVICE;
+        else if (launch_box_muller_parts(dw, n, d, d + n, d + 2 * n, nullptr) != hipSuccess ||
+                 hipDeviceSynchronize() != hipSuccess)
+            rc = HJM_ERR_LAUNCH;
+        else if (hipMemcpy(radius, d, b, hipMemcpyDeviceToHost) != hipSuccess ||
+                 hipMemcpy(cosine, d + n, b, hipMemcpyDeviceToHost) != hipSuccess ||
+                 hipMemcpy(sine, d + 2 * n, b, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = HJM_ERR_DEVICE;
+        dev_free(d);
+    }
+    dev_free(dw);
+    return rc;
+}
+
 int hjm_select_kth(const double* values, int32_t rows, int64_t n, int64_t k, int32_t device,
                    double* out)
 {

@@ -20,7 +20,9 @@ struct hjm_pplan_s {
     hipEvent_t ev[5] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hjm_tuning tuning{256, 0, 0, 0};
+    hjm_tuning tuning{256, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // block size 256, the rest defaults
+    MfmaShape path_shape{}, value_shape{};             // matrix-core kernels, from tuning
+    int cus = 0;
     int precision = HJM_FP64;
     bool do_pfe = false;
     int64_t paths = 0;
@@ -134,6 +136,33 @@ void build_tables(const hjm_portfolio* pf, int ndates, std::vector<double>& coef
     }
 }
 
+int check_tuning(const hjm_tuning& t)
+{
+    if (t.block_size != 64 && t.block_size != 128 && t.block_size != 256) return HJM_ERR_ARG;
+    if (t.blocks_per_cu < 0 || t.blocks_per_cu > 64) return HJM_ERR_ARG;
+    if (t.use_graph != 0 && t.use_graph != 1) return HJM_ERR_ARG;
+    if (t.fuse_statistics != 0 && t.fuse_statistics != 1) return HJM_ERR_ARG;
+    if (t.pfe_stage_slots < 0 || t.pfe_stage_slots > 32) return HJM_ERR_ARG;
+    if (!pf_path_shape_ok(t.path_tiles ? t.path_tiles : 2, t.path_waves ? t.path_waves : 8))
+        return HJM_ERR_ARG;
+    if (!pf_value_shape_ok(t.value_tiles ? t.value_tiles : 4, t.value_waves ? t.value_waves : 4))
+        return HJM_ERR_ARG;
+    if (t.value_stage < 0 || t.value_stage > 8) return HJM_ERR_ARG;
+    return HJM_OK;
+}
+
+// the kernels' launch shapes for a (checked) tuning
+void pplan_apply_tuning(hjm_portfolio_plan p, const hjm_tuning& t)
+{
+    p->tuning = t;
+    const int64_t tiles = (p->paths + t.block_size - 1) / t.block_size;
+    p->grid_blocks = (int)std::min<int64_t>(
+        tiles, t.blocks_per_cu > 0 ? (int64_t)t.blocks_per_cu * p->cus : tiles);
+    p->path_shape = MfmaShape{t.path_tiles, t.path_waves, 0,
+                              t.blocks_per_cu > 0 ? (int64_t)t.blocks_per_cu * p->cus : 0};
+    p->value_shape = MfmaShape{t.value_tiles, t.value_waves, t.value_stage, 0};
+}
+
 int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfolio* pf,
                const hjm_config* cfg, const hjm_tuning* tuning)
 {
@@ -148,9 +177,7 @@ int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfol
     if (cfg->rng == HJM_RNG_HOST && !cfg->host_normals) return HJM_ERR_NULL;
     if (cfg->pfe_quantile > 1.0) return HJM_ERR_ARG;
     if (tuning) {
-        if (tuning->block_size != 64 && tuning->block_size != 128 && tuning->block_size != 256)
-            return HJM_ERR_ARG;
-        if (tuning->blocks_per_cu < 0 || tuning->blocks_per_cu > 64) return HJM_ERR_ARG;
+        HJM_TRY(check_tuning(*tuning));
         p->tuning = *tuning;
     }
     if (!(cfg->dt > 0.0)) return HJM_ERR_ARG;
@@ -197,17 +224,15 @@ int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfol
 
     DeviceGuard guard;
     HJM_TRY(guard.set(cfg->device));
-    int cus = 0;
-    HJM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, cfg->device),
+    HJM_HIP(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, cfg->device),
             HJM_ERR_DEVICE);
-    const int64_t tiles = (cfg->paths + p->tuning.block_size - 1) / p->tuning.block_size;
-    p->grid_blocks = (int)std::min<int64_t>(
-        tiles, p->tuning.blocks_per_cu > 0 ? (int64_t)p->tuning.blocks_per_cu * cus : tiles);
+    pplan_apply_tuning(p, p->tuning);
     HJM_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), HJM_ERR_DEVICE);
     for (auto& ev : p->ev) HJM_HIP(hipEventCreate(&ev), HJM_ERR_DEVICE);
     const size_t rsz = (p->precision == HJM_FP64) ? sizeof(double) : sizeof(float);
     const size_t P = (size_t)cfg->paths;
-    const size_t nb = (P + kBlock - 1) / kBlock;
+    // workgroups of the statistics: 256 paths (exposure pass) or the fused pass's blocks
+    const size_t nb = std::max<size_t>((P + kBlock - 1) / kBlock, (size_t)pf_fused_blocks(cfg->paths));
     char* raw = nullptr;
     HJM_TRY(dev_alloc(&raw, (size_t)nd * kTenors * P * rsz));
     p->d_curves = raw;
@@ -290,9 +315,24 @@ int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfol
     return HJM_OK;
 }
 
-// Enqueue the four passes on the plan's stream, with timing events between them unless
-// the sequence is being captured into a graph (a captured replay is timed as a whole).
-int pplan_enqueue(hjm_portfolio_plan p, bool events = true, float* zs = nullptr)
+// FP32 plans value on the matrix cores. With tuning.fuse_statistics, unless the caller
+// wants the per-path exposures (or the PFE needs whole rows: fewer than 4 x 16384 paths),
+// valuation and statistics run fused, without writing the exposures: the PFE's bracket
+// comes from the first 16384 paths valued on their own, the fused pass counts and
+// collects every row's candidates, and only rows the counting leaves unresolved are
+// valued again, whole, for the selection.
+bool pplan_fused(hjm_portfolio_plan p, bool keep_exposures)
+{
+    if (p->precision == HJM_FP64 || keep_exposures || !p->tuning.fuse_statistics) return false;
+    return !p->do_pfe || (p->d_selws && bracket_plan(p->paths, p->pfe_rank, (int)sizeof(float)).use);
+}
+
+// Enqueue the passes on the plan's stream, with timing events between them unless the
+// sequence is being captured into a graph (a captured replay is timed as a whole).
+// Timings: [0] paths, [1] valuation (+ statistics when fused), [2] statistics (the column
+// sums when fused), [3] PFE selection.
+int pplan_enqueue(hjm_portfolio_plan p, bool events = true, float* zs = nullptr,
+                  bool keep_exposures = false)
 {
     hipStream_t st = p->stream;
     const bool f64 = p->precision == HJM_FP64;
@@ -300,18 +340,55 @@ int pplan_enqueue(hjm_portfolio_plan p, bool events = true, float* zs = nullptr)
     auto mark = [&](int i) { return events ? hipEventRecord(p->ev[i], st) : hipSuccess; };
     HJM_HIP(mark(0), HJM_ERR_DEVICE);
     hipError_t err =
-        p->use_mfma ? launch_pf_paths_mfma(p->mg, p->d_wfrag, (float*)p->d_curves, p->d_logb, zs, st)
+        p->use_mfma ? launch_pf_paths_mfma(p->mg, p->d_wfrag, (float*)p->d_curves, p->d_logb, zs,
+                                           p->path_shape, st)
         : f64 ? launch_pf_paths_f64(p->kd, p->grid, p->d_normals, (double*)p->d_curves, p->d_logb,
                                     p->tuning.block_size, p->grid_blocks, st)
               : launch_pf_paths_f32(p->kf, p->grid, p->d_normals, (float*)p->d_curves, p->d_logb,
                                     p->tuning.block_size, p->grid_blocks, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
     HJM_HIP(mark(1), HJM_ERR_DEVICE);
+    if (pplan_fused(p, keep_exposures)) {
+        const float* curves = (const float*)p->d_curves;
+        float* ex = (float*)p->d_ex;
+        SelRowState* state = nullptr;
+        float* cand = nullptr;
+        BracketPlan bp{};
+        if (p->do_pfe) {
+            bp = bracket_plan(p->paths, p->pfe_rank, (int)sizeof(float));
+            bracket_ws_split<float>(p->d_selws, rows, &state, &cand);
+            // the sample: exposures of the first bp.m paths, then the rows' brackets
+            HJM_HIP(launch_pf_value_mfma(p->vp, curves, p->d_logb, p->d_vfrag, p->d_apad, p->d_tilek,
+                                         ex, p->value_shape, st, bp.m),
+                    HJM_ERR_LAUNCH);
+            HJM_HIP(launch_bracket_sample_f32(ex, rows, p->paths, p->pfe_rank, p->d_selws, st),
+                    HJM_ERR_LAUNCH);
+        }
+        HJM_HIP(launch_pf_value_stats_mfma(p->vp, curves, p->d_logb, p->d_vfrag, p->d_apad,
+                                           p->d_tilek, p->d_w, p->d_partials, state, cand, bp.cap,
+                                           p->tuning.pfe_stage_slots, st),
+                HJM_ERR_LAUNCH);
+        HJM_HIP(mark(2), HJM_ERR_DEVICE);
+        HJM_HIP(launch_grouped_column_sum(p->d_partials, p->nsets, pf_fused_blocks(p->paths),
+                                          p->ndates + 2, p->d_sums, st),
+                HJM_ERR_LAUNCH);
+        HJM_HIP(mark(3), HJM_ERR_DEVICE);
+        if (p->do_pfe) {
+            // whole rows only where the counting did not resolve the rank
+            HJM_HIP(launch_pf_value_mfma(p->vp, curves, p->d_logb, p->d_vfrag, p->d_apad, p->d_tilek,
+                                         ex, p->value_shape, st, -1, state, p->pfe_rank, bp.cap),
+                    HJM_ERR_LAUNCH);
+            HJM_HIP(launch_bracket_finish_f32(ex, rows, p->paths, p->pfe_rank, p->d_selws, p->d_pfe, st),
+                    HJM_ERR_LAUNCH);
+        }
+        HJM_HIP(mark(4), HJM_ERR_DEVICE);
+        return HJM_OK;
+    }
     err = f64 ? launch_pf_value_f64(p->vp, (const double*)p->d_curves, p->d_logb,
                                     (const double*)p->d_coef, (const double*)p->d_a, p->d_kmax,
                                     (double*)p->d_ex, st)
               : launch_pf_value_mfma(p->vp, (const float*)p->d_curves, p->d_logb, p->d_vfrag,
-                                     p->d_apad, p->d_tilek, (float*)p->d_ex, st);
+                                     p->d_apad, p->d_tilek, (float*)p->d_ex, p->value_shape, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
     HJM_HIP(mark(2), HJM_ERR_DEVICE);
     // With a bracket-selection workspace the statistics pass also does the PFE's counting
@@ -467,7 +544,10 @@ int hjm_portfolio_plan_execute(hjm_portfolio_plan plan, hjm_portfolio_results* r
     if (!plan || !results) return HJM_ERR_NULL;
     DeviceGuard guard;
     HJM_TRY(guard.set(plan->device));
-    HJM_TRY(pplan_run(plan));
+    // per-path exposures asked for: an eager run that writes them (the graph, if any, is
+    // the fused form)
+    if (results->exposures) HJM_TRY(pplan_enqueue(plan, true, nullptr, true));
+    else HJM_TRY(pplan_run(plan));
     return pplan_collect(plan, results);
 }
 
@@ -603,45 +683,115 @@ int hjm_portfolio_autotune(const hjm_market* market, const hjm_portfolio* portfo
                            const hjm_config* config, hjm_tuning* best, double* best_ms)
 {
     if (!best || !config || !portfolio) return HJM_ERR_NULL;
-    // the path kernel's cost is linear in the horizon: tune on two dates
-    hjm_portfolio short_pf = *portfolio;
-    short_pf.horizon = 2.0 * portfolio->dtau;
-    std::vector<int32_t> per(portfolio->periods, portfolio->periods + portfolio->nswaps);
-    for (auto& x : per) x = std::min(x, 2);
-    short_pf.periods = per.data();
     hjm_config cfg = *config;
-    cfg.pfe_quantile = 0.0;
     if (cfg.rng == HJM_RNG_HOST) cfg.rng = HJM_RNG_PHILOX;
-    cfg.path_kernel = HJM_KERNEL_VECTOR;  // the matrix-core kernel has a fixed launch shape
-    double bestv = 1e300;
-    hjm_tuning bt{256, 0, 1, 0};
-    for (int bs : {64, 128, 256}) {
-        for (int bpc : {0, 1, 2, 4, 8}) {
-            hjm_tuning t{bs, bpc, 0, 0};
-            hjm_portfolio_plan p = nullptr;
-            int rc = hjm_portfolio_plan_create(&p, market, &short_pf, &cfg, &t);
-            if (rc != HJM_OK) return rc;
-            double tot = 0.0;
-            hjm_portfolio_results r{};
-            double tm[5];
-            r.timings_ms = tm;
-            for (int rep = 0; rep < 4 && rc == HJM_OK; ++rep) {
-                DeviceGuard guard;
-                rc = guard.set(p->device);
-                if (rc == HJM_OK) rc = pplan_enqueue(p);
-                if (rc == HJM_OK) rc = pplan_collect(p, &r);
-                if (rep > 0) tot += tm[0];
-            }
-            hjm_portfolio_plan_destroy(p);
-            if (rc != HJM_OK) return rc;
-            if (tot < bestv) {
-                bestv = tot;
-                bt = hjm_tuning{bs, bpc, 1, 0};
-            }
-        }
+    // the VALU path kernel's cost is linear in the horizon: it is tuned on two dates
+    hjm_portfolio short_pf = *portfolio;
+    std::vector<int32_t> per;
+    // the plan's own choice of path kernel (pplan_init)
+    const bool mfma = cfg.precision == HJM_FP32_KAHAN && cfg.path_kernel != HJM_KERNEL_VECTOR &&
+                      cfg.dt > 0.0 && llround(portfolio->dtau / cfg.dt) == kMfmaStride;
+    int nd = 0;
+    HJM_TRY(check_portfolio(portfolio, &nd));
+    if (!mfma) {
+        short_pf.horizon = 2.0 * portfolio->dtau;
+        per.assign(portfolio->periods, portfolio->periods + portfolio->nswaps);
+        for (auto& x : per) x = std::min(x, 2);
+        short_pf.periods = per.data();
+        cfg.pfe_quantile = 0.0;
     }
+    hjm_portfolio_plan p = nullptr;
+    HJM_TRY(hjm_portfolio_plan_create(&p, market, &short_pf, &cfg, nullptr));
+    DeviceGuard guard;
+    int rc = guard.set(p->device);
+    // mean of 3 eager executes after one untimed, per timing slot
+    auto measure = [&](const hjm_tuning& t, double (&ms)[5]) {
+        pplan_apply_tuning(p, t);
+        for (double& m : ms) m = 0.0;
+        double tm[5];
+        hjm_portfolio_results r{};
+        r.timings_ms = tm;
+        for (int rep = 0; rep < 4 && rc == HJM_OK; ++rep) {
+            rc = pplan_enqueue(p);
+            if (rc == HJM_OK) rc = pplan_collect(p, &r);
+            if (rep > 0)
+                for (int i = 0; i < 5; ++i) ms[i] += tm[i] / 3.0;
+        }
+    };
+    hjm_tuning bt{256, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double ms[5];
+    if (p->use_mfma) {
+        // path kernel: shape x {one workgroup per group, persistent (one per CU)}
+        const int shapes[4][2] = {{2, 8}, {2, 12}, {1, 12}, {1, 16}};
+        double bv = 1e300;
+        hjm_tuning t = bt;
+        for (const auto& sh : shapes)
+            for (int bpc : {0, 1}) {
+                t.path_tiles = sh[0];
+                t.path_waves = sh[1];
+                t.blocks_per_cu = bpc;
+                measure(t, ms);
+                if (rc == HJM_OK && ms[0] < bv) {
+                    bv = ms[0];
+                    bt.path_tiles = sh[0];
+                    bt.path_waves = sh[1];
+                    bt.blocks_per_cu = bpc;
+                }
+            }
+        // valuation: tiles x waves x LDS stage (stages below the M-tile count only)
+        const int mtiles = value_mtiles(p->nsets);
+        bv = 1e300;
+        t = bt;
+        for (int tiles : {4, 2})
+            for (int waves : {4, 8})
+                for (int stage : {8, 4, 2}) {
+                    if (stage != 8 && stage >= mtiles) continue;
+                    t.value_tiles = tiles;
+                    t.value_waves = waves;
+                    t.value_stage = stage;
+                    measure(t, ms);
+                    if (rc == HJM_OK && ms[1] < bv) {
+                        bv = ms[1];
+                        bt.value_tiles = tiles;
+                        bt.value_waves = waves;
+                        bt.value_stage = stage;
+                    }
+                }
+        // the exposure pass against the fused valuation-statistics pass (which keeps its
+        // own shape): everything after the path kernel
+        double passes[2] = {1e300, 1e300};
+        for (int fuse : {0, 1}) {
+            t = bt;
+            t.fuse_statistics = fuse;
+            measure(t, ms);
+            if (rc == HJM_OK) passes[fuse] = ms[4] - ms[0];
+        }
+        bt.fuse_statistics = passes[1] < passes[0] ? 1 : 0;
+    } else {
+        double bv = 1e300;
+        for (int bs : {64, 128, 256})
+            for (int bpc : {0, 1, 2, 4, 8}) {
+                hjm_tuning t = bt;
+                t.block_size = bs;
+                t.blocks_per_cu = bpc;
+                measure(t, ms);
+                if (rc == HJM_OK && ms[0] < bv) {
+                    bv = ms[0];
+                    bt.block_size = bs;
+                    bt.blocks_per_cu = bpc;
+                }
+            }
+    }
+    double total = 0.0;
+    if (rc == HJM_OK) {
+        measure(bt, ms);
+        total = ms[4];
+    }
+    hjm_portfolio_plan_destroy(p);
+    if (rc != HJM_OK) return rc;
+    bt.use_graph = 1;
     *best = bt;
-    if (best_ms) *best_ms = bestv / 3.0;
+    if (best_ms) *best_ms = total;
     return HJM_OK;
 }
 

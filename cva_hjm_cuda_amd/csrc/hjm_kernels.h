@@ -4,6 +4,8 @@
 
 namespace hjm {
 
+struct SelRowState;  // hjm_select.h
+
 struct ExposureParams {
     int64_t paths;
     int ndates;
@@ -27,6 +29,8 @@ hipError_t launch_radix_select(const double* x, int rows, int64_t n, int64_t k, 
                                double* out, hipStream_t st);
 hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nsteps, int stride,
                           double* out, hipStream_t st);
+hipError_t launch_box_muller_parts(const uint32_t* words, int64_t n, float* r, float* c, float* s,
+                                   hipStream_t st);
 
 // Matrix-core FP32 path kernels (50 steps per date); prepare once per plan (sets the
 // kernels' dynamic LDS limits) before launching or capturing them.
@@ -34,8 +38,20 @@ hipError_t mfma_kernels_prepare();
 // zs (optional): the forward's FP32 normals [step][factor][path] for a later reverse sweep
 hipError_t launch_ref_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, double* nx, double* ny,
                                  float* zs, hipStream_t st);
+// Launch shape of a matrix-core portfolio kernel (0 in a field: the kernel's default).
+//   tiles: N-tiles of 16 paths per wave; waves: per workgroup;
+//   stage (valuation): M-tiles of -gamma staged in LDS at a time;
+//   max_blocks (paths): a persistent grid of at most this many workgroups.
+struct MfmaShape {
+    int tiles = 0, waves = 0, stage = 0;
+    int64_t max_blocks = 0;
+};
+// path kernel: {tiles, waves} in {2, 8} (default), {2, 12}, {1, 12}, {1, 16};
+// valuation: tiles 2 or 4 (default), waves 4 (default) or 8, stage 1..8 (default 8)
+bool pf_path_shape_ok(int tiles, int waves);
+bool pf_value_shape_ok(int tiles, int waves);
 hipError_t launch_pf_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, float* curves,
-                                double* logb, float* zs, hipStream_t st);
+                                double* logb, float* zs, const MfmaShape& shape, hipStream_t st);
 
 // Pathwise adjoint of the reference-semantics CVA.
 // sa_partials: [blocks of 256 paths][51] per-slot sums of gA over the block's paths
@@ -58,10 +74,25 @@ hipError_t launch_pf_paths_f32(const RefStepConsts<float>& k, const PfGrid& g, c
 hipError_t launch_pf_value_f64(const PfValueParams& v, const double* curves, const double* logb,
                                const double* coef, const double* a, const int* kmax, double* ex,
                                hipStream_t st);
-// FP32 plans: the matrix-core valuation over build_value_fragments' tables
+// FP32 plans: the matrix-core valuation over build_value_fragments' tables, of paths
+// [0, count) (count < 0: all); filter (or null): only the rows (date, set) whose PFE
+// bracket selection is unresolved for rank fk with fcap candidate slots
 hipError_t launch_pf_value_mfma(const PfValueParams& v, const float* curves, const double* logb,
                                 const uint16_t* gfrag, const float* apad, const int* tilek,
-                                float* ex, hipStream_t st);
+                                float* ex, const MfmaShape& shape, hipStream_t st, int64_t count = -1,
+                                const SelRowState* filter = nullptr, int64_t fk = 0,
+                                int64_t fcap = 0);
+// FP32 plans with at most 128 sets: valuation, EE and per-path CVA moments (partials
+// [set][pf_fused_blocks(paths)][ndates + 2], for the grouped column sums) and, with sel,
+// the PFE bracket counting and candidates of every row, in one pass without exposures
+int64_t pf_fused_blocks(int64_t paths);
+hipError_t launch_pf_value_stats_mfma(const PfValueParams& v, const float* curves, const double* logb,
+                                      const uint16_t* gfrag, const float* apad, const int* tilek,
+                                      const double* w, double* partials, SelRowState* sel,
+                                      float* cand, int64_t cap, int stage_slots, hipStream_t st);
+// sums[set][c] = sum over blocks of partials[set][block][c] (fixed order)
+hipError_t launch_grouped_column_sum(const double* partials, int nsets, int64_t blocks, int cols,
+                                     double* sums, hipStream_t st);
 // selws (or null): bracket-selection workspace whose counting this pass does, for rank
 // pfe_rank of each (date, set) row, after launch_bracket_sample*; needs
 // pf_stats_stage(...) > 0 (LDS staging slice per wave, in entries; *flush_dates, if given,

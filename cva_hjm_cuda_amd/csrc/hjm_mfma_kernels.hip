@@ -17,10 +17,14 @@
 //     every date and the short rate's left-point sum (row 51) extends the log bank account.
 //   pf_value_mfma_kernel: the portfolio's netting-set values at one date, C = a - gamma P
 //     over 16-set x 16-path tiles (sets as M, paths as N, periods as K).
+//   pf_value_stats_mfma_kernel: the same values at every date of a workgroup's paths,
+//     reduced to EE sums, per-path CVA moments and the PFE's bracket counting where they
+//     are produced (no exposures in HBM).
 #include "hjm_common.h"
 #include "hjm_kernels.h"
 #include "hjm_mfma_plan.h"
 #include "hjm_rng.h"
+#include "hjm_select.h"
 
 #include <utility>
 
@@ -66,8 +70,10 @@ __device__ __forceinline__ Split8 split8(const float* x)
     return s;
 }
 
-constexpr int kNT = 2;                       // N-tiles (16 paths each) per wave
-constexpr int kMfmaBlock = 512;              // 8 waves
+// reference kernel: 8 waves of 2 N-tiles (16 paths each); the portfolio kernel's shape
+// is a template choice (MfmaShape)
+constexpr int kNT = 2;
+constexpr int kMfmaBlock = 512;
 constexpr int kMfmaPathsPerBlock = kMfmaBlock / kWave * kNT * 16;
 
 // last quad whose values complete shock chunk s (values 8 s .. 8 s + 7, three per quad)
@@ -114,27 +120,27 @@ __device__ __forceinline__ void for_each_index(F& f, std::integer_sequence<int, 
 
 // One 50-step block: acc = W x [fin; shocks of steps b*50 .. b*50+49; 1]. kStore: also
 // write the block's normals to zs[step][factor][path] (FP32) for a later reverse sweep.
-template <int MT, bool kStore>
-__device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const float (&fin)[kNT][16],
-                                           f4_t (&acc)[MT][kNT], const MfmaGrid& g,
-                                           const int64_t (&path)[kNT], const bool (&live)[kNT],
+template <int MT, int NT, bool kStore>
+__device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const float (&fin)[NT][16],
+                                           f4_t (&acc)[MT][NT], const MfmaGrid& g,
+                                           const int64_t (&path)[NT], const bool (&live)[NT],
                                            int b, int grp, int lane, float* __restrict__ zs)
 {
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
-        for (int n = 0; n < kNT; ++n) acc[m][n] = f4_t{0.f, 0.f, 0.f, 0.f};
+        for (int n = 0; n < NT; ++n) acc[m][n] = f4_t{0.f, 0.f, 0.f, 0.f};
     // W's fragments are the same in every block: an opaque offset keeps their LDS reads
     // inside the block instead of hoisted (hundreds of registers) out of the block loop
     uint32_t off = (uint32_t)lane;
     asm volatile("" : "+v"(off));
-    auto tiles = [&](int c, int m0, int m1, const Split8 (&xs)[kNT]) {
+    auto tiles = [&](int c, int m0, int m1, const Split8 (&xs)[NT]) {
 #pragma unroll
         for (int m = m0; m < m1; ++m) {
             const u4_t* a = lds_w + ((m * kMfmaChunks + c) * kMfmaParts) * kWave + off;
             const u4_t ah = a[0], am = a[kWave], al = a[2 * kWave];
 #pragma unroll
-            for (int n = 0; n < kNT; ++n) {
+            for (int n = 0; n < NT; ++n) {
                 f4_t v = acc[m][n];
                 v = mfma_bf16(ah, xs[n].h, v);
                 v = mfma_bf16(ah, xs[n].m, v);
@@ -148,16 +154,16 @@ __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const
     };
     // split operands of the 7 chunks: curve chunks from fin (registers 8 c .. 8 c + 7),
     // shock chunk s from values 8 s .. 8 s + 7 of z
-    Split8 xs[kMfmaChunks][kNT];
+    Split8 xs[kMfmaChunks][NT];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
-        for (int n = 0; n < kNT; ++n) xs[c][n] = split8(&fin[n][8 * c]);
+        for (int n = 0; n < NT; ++n) xs[c][n] = split8(&fin[n][8 * c]);
     // shocks: value v = 3 q + k of lane group grp is factor k of step 4 q + grp; step 50
     // (lane group 2, q = 12) carries the constant 1 of the drift in factor 0.
-    float z[kNT][8 * (kMfmaChunks - 2)];
+    float z[NT][8 * (kMfmaChunks - 2)];
 #pragma unroll
-    for (int n = 0; n < kNT; ++n) z[n][3 * kMfmaQuads] = 0.f;
+    for (int n = 0; n < NT; ++n) z[n][3 * kMfmaQuads] = 0.f;
     auto quad = [&](auto qc) {
         constexpr int q = decltype(qc)::value;
         // steps 4 q + grp < 50 for every lane group up to q = 11; only the last quad pads
@@ -170,7 +176,7 @@ __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const
         constexpr int chi = (3 * q + 2) / 4;
         constexpr int clo = q == 0 ? 0 : (3 * q - 1) / 4 + 1;
 #pragma unroll
-        for (int n = 0; n < kNT; ++n) {
+        for (int n = 0; n < NT; ++n) {
 #pragma unroll
             for (int c = clo; c <= chi; ++c) {
                 float v[4];
@@ -198,7 +204,7 @@ __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const
         for (int s = 0; s < kMfmaChunks - 2; ++s)
             if (q == shock_chunk_last_quad(s))
 #pragma unroll
-                for (int n = 0; n < kNT; ++n) xs[2 + s][n] = split8(&z[n][8 * s]);
+                for (int n = 0; n < NT; ++n) xs[2 + s][n] = split8(&z[n][8 * s]);
         // this quad's share of the matrix work (its operands are complete); it runs on
         // the matrix cores beside the RNG of this and the other waves
         constexpr IssueSlot is = issue_slot<MT>(q, 0);
@@ -207,7 +213,7 @@ __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const
             if constexpr (MT == kMfmaMTiles) {
                 // within the quad: one MFMA, then a share of the RNG's VALU instructions
 #pragma unroll
-                for (int i = 0; i < (is.m1 - is.m0) * kNT * 6; ++i) {
+                for (int i = 0; i < (is.m1 - is.m0) * NT * 6; ++i) {
                     __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
                     __builtin_amdgcn_sched_group_barrier(0x002, 7, 0);
                 }
@@ -278,7 +284,7 @@ __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGr
 
     for (int b = 1; b < g.ndates; ++b) {
         f4_t acc[kMfmaMTiles][kNT];
-        mfma_block<kMfmaMTiles, kStore>(lds_w, fin, acc, g, path, live, b, grp, lane, zs);
+        mfma_block<kMfmaMTiles, kNT, kStore>(lds_w, fin, acc, g, path, live, b, grp, lane, zs);
         // integrals: Kahan-add the block's dA (tiles 3..6); curve rows become next input
 #pragma unroll
         for (int n = 0; n < kNT; ++n) {
@@ -325,75 +331,114 @@ __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGr
 // Portfolio semantics: dates e = 0 .. ndates-1 every 50 steps from t = 0. At each date the
 // curve goes to curves[e][tenor][path] and the log bank account dt sum_{n < 50 e} f_n(0)
 // to logb[e][path]; the block's row 51 (left-point sum of the short rate) extends it.
-template <bool kStore>
-__global__ __launch_bounds__(kMfmaBlock) void pf_paths_mfma_kernel(const MfmaGrid g,
-                                                               const u4_t* __restrict__ wfrag,
-                                                               float* __restrict__ curves,
-                                                               double* __restrict__ logb,
-                                                               float* __restrict__ zs)
+// A workgroup of WAVES waves of NT N-tiles simulates path group blockIdx.x of WAVES x NT x
+// 16 paths; kPersist: then every gridDim.x-th group after it, with a grid smaller than the
+// number of groups (the block map is staged in LDS once per workgroup). The group loop is
+// a separate instantiation: around the date loop it costs the compiler's allocation
+// about 50 VGPRs.
+template <bool kStore, int NT, int WAVES, bool kPersist>
+__global__ __launch_bounds__(WAVES * kWave) void pf_paths_mfma_kernel(const MfmaGrid g,
+                                                                  const u4_t* __restrict__ wfrag,
+                                                                  float* __restrict__ curves,
+                                                                  double* __restrict__ logb,
+                                                                  float* __restrict__ zs)
 {
+    constexpr int kThreads = WAVES * kWave, kGroup = WAVES * NT * 16;
     extern __shared__ u4_t lds_w[];
-    for (int t = threadIdx.x; t < kMfmaPfFragElems / 8; t += kMfmaBlock) lds_w[t] = wfrag[t];
+    for (int t = threadIdx.x; t < kMfmaPfFragElems / 8; t += kThreads) lds_w[t] = wfrag[t];
     __syncthreads();
 
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int grp = lane >> 4, col = lane & 15;
     const int64_t P = g.paths;
-    const int64_t base = (int64_t)blockIdx.x * kMfmaPathsPerBlock + (int64_t)wv * kNT * 16;
-    int64_t path[kNT];
-    bool live[kNT];
-    float fin[kNT][16];
-    double lnB[kNT];
-#pragma unroll
-    for (int n = 0; n < kNT; ++n) {
-        path[n] = base + 16 * n + col;
-        live[n] = path[n] < P;
-        lnB[n] = 0.0;
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int R = 16 * m + 4 * grp + r;
-                fin[n][4 * m + r] = (R < kTenors) ? g.spot[R < kTenors ? R : 0] : 0.f;
-            }
-    }
+    const int64_t groups = (P + kGroup - 1) / kGroup;
     // row 51 (the short-rate sum) is register 4 * 3 + 3 of lane group 0
     constexpr int kSumReg = 4 * (kTenors / 16) + kTenors % 4;
     const bool sum_lane = grp == (kTenors % 16) / 4;
-    for (int e = 0;; ++e) {
+    for (int64_t gi = blockIdx.x; gi < groups; gi += kPersist ? gridDim.x : groups) {
+        const int64_t base = gi * kGroup + (int64_t)wv * NT * 16;
+        int64_t path[NT];
+        bool live[NT];
+        float fin[NT][16];
+        double lnB[NT];
+        // opaque per group: the initial curve is selected again in every group rather than
+        // held in 16 registers across the loop
+        int lg = grp;
+        asm volatile("" : "+v"(lg));
 #pragma unroll
-        for (int n = 0; n < kNT; ++n) {
-            if (!live[n]) continue;
+        for (int n = 0; n < NT; ++n) {
+            path[n] = base + 16 * n + col;
+            live[n] = path[n] < P;
+            lnB[n] = 0.0;
 #pragma unroll
             for (int m = 0; m < 4; ++m)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int R = 16 * m + 4 * grp + r;
-                    if (R < kTenors) curves[((int64_t)e * kTenors + R) * P + path[n]] = fin[n][4 * m + r];
+                    const int R = 16 * m + 4 * lg + r;
+                    fin[n][4 * m + r] = (R < kTenors) ? g.spot[R < kTenors ? R : 0] : 0.f;
                 }
-            if (sum_lane) logb[(int64_t)e * P + path[n]] = lnB[n];
         }
-        if (e + 1 >= g.ndates) break;
-        f4_t acc[kMfmaPfMTiles][kNT];
-        mfma_block<kMfmaPfMTiles, kStore>(lds_w, fin, acc, g, path, live, e, grp, lane, zs);
+        for (int e = 0;; ++e) {
 #pragma unroll
-        for (int n = 0; n < kNT; ++n) {
+            for (int n = 0; n < NT; ++n) {
+                if (!live[n]) continue;
 #pragma unroll
-            for (int m = 0; m < 4; ++m)
+                for (int m = 0; m < 4; ++m)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) fin[n][4 * m + r] = acc[m][n][r];
-            lnB[n] += (double)fin[n][kSumReg] * g.dt;
+                    for (int r = 0; r < 4; ++r) {
+                        const int R = 16 * m + 4 * grp + r;
+                        if (R < kTenors)
+                            curves[((int64_t)e * kTenors + R) * P + path[n]] = fin[n][4 * m + r];
+                    }
+                if (sum_lane) logb[(int64_t)e * P + path[n]] = lnB[n];
+            }
+            if (e + 1 >= g.ndates) break;
+            f4_t acc[kMfmaPfMTiles][NT];
+            mfma_block<kMfmaPfMTiles, NT, kStore>(lds_w, fin, acc, g, path, live, e, grp, lane, zs);
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) fin[n][4 * m + r] = acc[m][n][r];
+                lnB[n] += (double)fin[n][kSumReg] * g.dt;
+            }
         }
     }
 }
 
+// The portfolio path kernel's instantiated launch shapes {N-tiles per wave, waves}: {2, 8},
+// {2, 12}, {1, 12}, {1, 16} (the block map's 84 KB of LDS allow one workgroup per CU;
+// 2 N-tiles take about 164 VGPRs, at most 3 waves per SIMD).
+constexpr int kPathShapes = 4;
+template <bool kStore, bool kPersist>
+struct PfPathsKernels {
+    template <int NT, int W>
+    static const void* fn() { return (const void*)pf_paths_mfma_kernel<kStore, NT, W, kPersist>; }
+};
+
 hipError_t mfma_kernels_prepare()
 {
-    const void* fns[4] = {(const void*)ref_paths_mfma_kernel<false>, (const void*)ref_paths_mfma_kernel<true>,
-                          (const void*)pf_paths_mfma_kernel<false>, (const void*)pf_paths_mfma_kernel<true>};
-    const int lds[4] = {2 * kMfmaFragElems, 2 * kMfmaFragElems, 2 * kMfmaPfFragElems,
-                        2 * kMfmaPfFragElems};
-    for (int i = 0; i < 4; ++i) {
+    const void* fns[2 + 4 * kPathShapes] = {(const void*)ref_paths_mfma_kernel<false>,
+                                            (const void*)ref_paths_mfma_kernel<true>};
+    int lds[2 + 4 * kPathShapes] = {2 * kMfmaFragElems, 2 * kMfmaFragElems};
+    int k = 2;
+    auto add = [&](const void* f) {
+        fns[k] = f;
+        lds[k++] = 2 * kMfmaPfFragElems;
+    };
+    auto add_shapes = [&](auto kernels) {
+        using K = decltype(kernels);
+        add(K::template fn<2, 8>());
+        add(K::template fn<2, 12>());
+        add(K::template fn<1, 12>());
+        add(K::template fn<1, 16>());
+    };
+    add_shapes(PfPathsKernels<false, false>{});
+    add_shapes(PfPathsKernels<true, false>{});
+    add_shapes(PfPathsKernels<false, true>{});
+    add_shapes(PfPathsKernels<true, true>{});
+    for (int i = 0; i < k; ++i) {
         const hipError_t e = hipFuncSetAttribute(fns[i], hipFuncAttributeMaxDynamicSharedMemorySize, lds[i]);
         if (e != hipSuccess) return e;
     }
@@ -412,16 +457,45 @@ hipError_t launch_ref_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, doubl
     return hipGetLastError();
 }
 
-hipError_t launch_pf_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, float* curves,
-                                double* logb, float* zs, hipStream_t st)
+bool pf_path_shape_ok(int tiles, int waves)
 {
-    const dim3 grid((unsigned)((g.paths + kMfmaPathsPerBlock - 1) / kMfmaPathsPerBlock));
-    const u4_t* w = reinterpret_cast<const u4_t*>(wfrag);
-    if (zs)
-        pf_paths_mfma_kernel<true><<<grid, kMfmaBlock, 2 * kMfmaPfFragElems, st>>>(g, w, curves, logb, zs);
+    return (tiles == 2 && (waves == 8 || waves == 12)) || (tiles == 1 && (waves == 12 || waves == 16));
+}
+
+template <bool kStore, bool kPersist>
+static hipError_t launch_pf_paths_shape(const MfmaGrid& g, const u4_t* w, float* curves, double* logb,
+                                        float* zs, int tiles, int waves, int64_t blocks, hipStream_t st)
+{
+    const dim3 grid((unsigned)blocks), block((unsigned)(waves * kWave));
+    const size_t lds = 2 * kMfmaPfFragElems;
+    if (tiles == 2 && waves == 8)
+        pf_paths_mfma_kernel<kStore, 2, 8, kPersist><<<grid, block, lds, st>>>(g, w, curves, logb, zs);
+    else if (tiles == 2 && waves == 12)
+        pf_paths_mfma_kernel<kStore, 2, 12, kPersist><<<grid, block, lds, st>>>(g, w, curves, logb, zs);
+    else if (tiles == 1 && waves == 12)
+        pf_paths_mfma_kernel<kStore, 1, 12, kPersist><<<grid, block, lds, st>>>(g, w, curves, logb, zs);
+    else if (tiles == 1 && waves == 16)
+        pf_paths_mfma_kernel<kStore, 1, 16, kPersist><<<grid, block, lds, st>>>(g, w, curves, logb, zs);
     else
-        pf_paths_mfma_kernel<false><<<grid, kMfmaBlock, 2 * kMfmaPfFragElems, st>>>(g, w, curves, logb, zs);
+        return hipErrorInvalidValue;
     return hipGetLastError();
+}
+
+hipError_t launch_pf_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, float* curves,
+                                double* logb, float* zs, const MfmaShape& shape, hipStream_t st)
+{
+    const int tiles = shape.tiles ? shape.tiles : 2, waves = shape.waves ? shape.waves : 8;
+    if (!pf_path_shape_ok(tiles, waves) || shape.max_blocks < 0) return hipErrorInvalidValue;
+    const u4_t* w = reinterpret_cast<const u4_t*>(wfrag);
+    const int64_t group = (int64_t)waves * tiles * 16;
+    const int64_t groups = (g.paths + group - 1) / group;
+    if (shape.max_blocks > 0 && shape.max_blocks < groups) {
+        const int64_t b = shape.max_blocks;
+        return zs ? launch_pf_paths_shape<true, true>(g, w, curves, logb, zs, tiles, waves, b, st)
+                  : launch_pf_paths_shape<false, true>(g, w, curves, logb, zs, tiles, waves, b, st);
+    }
+    return zs ? launch_pf_paths_shape<true, false>(g, w, curves, logb, zs, tiles, waves, groups, st)
+              : launch_pf_paths_shape<false, false>(g, w, curves, logb, zs, tiles, waves, groups, st);
 }
 
 // Netting-set exposures of one date on the matrix cores (FP32 plans):
@@ -433,128 +507,175 @@ hipError_t launch_pf_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, float*
 // the trapezoid sums of its two 8-period segments from the curve checkpoint and takes the
 // sums of the segments before them from the other lane groups (shuffles), then P_k =
 // exp(-I_k). The accumulator layout (lane l: sets 4 (l / 16) + r of path l % 16) is stored
-// directly: per store, 16 consecutive paths of one set. A workgroup of 4 waves x 64 paths
-// stages the date's A fragments in LDS, 8 M-tiles (128 sets) at a time; an M-tile whose
-// sets have no live period past 32 skips the second chunk.
-constexpr int kValBlock = 256;
-constexpr int kValNT = 4;                                  // N-tiles (16 paths) per wave
-constexpr int kValPaths = kValBlock / kWave * kValNT * 16; // paths per workgroup
-constexpr int kValLdsTiles = 8;                            // M-tiles staged at a time
+// directly: per store, 16 consecutive paths of one set. An M-tile whose sets have no live
+// period past 32 skips the second chunk. The valuation and the fused valuation-statistics
+// kernel below share these helpers, so both produce the same exposures bit for bit.
 
-__global__ __launch_bounds__(kValBlock) void pf_value_mfma_kernel(const PfValueParams v, int mtiles,
-                                                                const float* __restrict__ curves,
-                                                                const double* __restrict__ logb,
-                                                                const u4_t* __restrict__ gfrag,
-                                                                const float* __restrict__ apad,
-                                                                const int* __restrict__ tilek,
-                                                                float* __restrict__ exposures)
+// The bond prices of one N-tile's path (column l % 16) at date e, split for the B operand
+// of the two 32-period chunks, and its discount factor (from lane group `src_grp`'s load,
+// shuffled to the others by the caller).
+__device__ __forceinline__ void value_bonds(const PfValueParams& v, int e, int64_t p,
+                                            const float* __restrict__ curves, int grp, int col,
+                                            Split8 (&bs)[2])
 {
-    __shared__ u4_t sfrag[kValLdsTiles * kValueFragsPerTile * kWave];
+    const int64_t P = v.paths;
+    const float hw = (float)(0.5 * v.dtau), dtf = (float)v.dtau;
+    const float* fc = curves + (int64_t)e * kTenors * P + p;
+    // opaque per call: a caller's date loop does not hold the rows' addresses in VGPRs
+    asm volatile("" : "+v"(fc));
+    // segment 0: periods k = 8 grp + 1 .. 8 grp + 8 (tenors 8 grp .. 8 grp + 8 <= 32);
+    // segment 1: k = 33 + 8 grp .. 40 + 8 grp (tenors capped at the last, 50)
+    float fa[9], fb[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) fa[i] = fc[(int64_t)(8 * grp + i) * P];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int j = 32 + 8 * grp + i;
+        fb[i] = fc[(int64_t)(j < kTenors - 1 ? j : kTenors - 1) * P];
+    }
+    float ca[8], cb[8];
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        s += hw * (fa[i] + fa[i + 1]);
+        ca[i] = s;
+    }
+    s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int k = 33 + 8 * grp + i;
+        s += k <= kTenors - 1 ? hw * (fb[i] + fb[i + 1]) : dtf * fb[i];
+        cb[i] = s;
+    }
+    float t0[4], t1[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        t0[g] = __shfl(ca[7], col + 16 * g, kWave);
+        t1[g] = __shfl(cb[7], col + 16 * g, kWave);
+    }
+    float offa = 0.f, offb = ((t0[0] + t0[1]) + t0[2]) + t0[3];
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+        offa += g < grp ? t0[g] : 0.f;
+        offb += g < grp ? t1[g] : 0.f;
+    }
+    float pa[8], pb[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        pa[i] = expf(-(offa + ca[i]));
+        pb[i] = 32 + 8 * grp + i < kMaxPeriods ? expf(-(offb + cb[i])) : 0.f;
+    }
+    bs[0] = split8(pa);
+    bs[1] = split8(pb);
+}
+
+// One M-tile's values for NT N-tiles: acc[n] = a + (-gamma) P over the chunks the tile uses.
+template <int NT>
+__device__ __forceinline__ void value_tile(const u4_t* __restrict__ fr0, int lane, int tk, f4_t a4,
+                                           const Split8 (&bs)[NT][2], f4_t (&acc)[NT])
+{
+#pragma unroll
+    for (int n = 0; n < NT; ++n) acc[n] = a4;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        if (tk > 32 * c) {
+            const u4_t* fr = fr0 + (c * kMfmaParts) * kWave + lane;
+            const u4_t ah = fr[0], am = fr[kWave], al = fr[2 * kWave];
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+                f4_t x = acc[n];
+                x = mfma_bf16(ah, bs[n][c].h, x);
+                x = mfma_bf16(ah, bs[n][c].m, x);
+                x = mfma_bf16(am, bs[n][c].h, x);
+                x = mfma_bf16(ah, bs[n][c].l, x);
+                x = mfma_bf16(al, bs[n][c].h, x);
+                x = mfma_bf16(am, bs[n][c].m, x);
+                acc[n] = x;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ float exposure_of(float V, float disc) { return V > 0.f ? V * disc : 0.f; }
+
+// Exposures of paths [0, count) at the date blockIdx.y (rows [date][set], row stride
+// v.paths). A workgroup of WAVES waves of NT N-tiles values WAVES x NT x 16 paths; the
+// date's -gamma fragments pass through LDS `stage` M-tiles at a time (dynamic LDS).
+// filter (or null): value only the M-tiles holding a set whose PFE row at this date the
+// counting pass left unresolved (bracket_row_resolved false for rank fk and fcap
+// candidate slots); a workgroup with none returns at once.
+constexpr int kValMaxStage = 8;  // M-tiles staged at a time, at most
+
+template <int NT, int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void pf_value_mfma_kernel(const PfValueParams v, int mtiles,
+                                                                    int stage, int64_t count,
+                                                                    const float* __restrict__ curves,
+                                                                    const double* __restrict__ logb,
+                                                                    const u4_t* __restrict__ gfrag,
+                                                                    const float* __restrict__ apad,
+                                                                    const int* __restrict__ tilek,
+                                                                    const SelRowState* __restrict__ filter,
+                                                                    int64_t fk, int64_t fcap,
+                                                                    float* __restrict__ exposures)
+{
+    constexpr int kThreads = WAVES * kWave, kPaths = WAVES * NT * 16;
+    extern __shared__ u4_t sfrag[];
+    __shared__ uint32_t s_need[kMaxPfMTiles];
+    __shared__ int s_any;
     const int e = blockIdx.y;
     const int64_t P = v.paths;
     const int G = v.nsets;
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
     const int col = lane % 16, grp = lane / 16;
-    const int64_t pbase = (int64_t)blockIdx.x * kValPaths + wave * (kValNT * 16);
-    const float hw = (float)(0.5 * v.dtau), dtf = (float)v.dtau;
-    // discount factor of N-tile grp's path, shared with the other lane groups below
-    float disc[kValNT];
+    const int64_t pbase = (int64_t)blockIdx.x * kPaths + wave * (NT * 16);
+    if (filter) {
+        if (threadIdx.x == 0) s_any = 0;
+        for (int m = threadIdx.x; m < mtiles; m += kThreads) s_need[m] = 0;
+        __syncthreads();
+        for (int g = threadIdx.x; g < G; g += kThreads)
+            if (!bracket_row_resolved(filter[(int64_t)e * G + g], fk, fcap)) {
+                s_need[g / 16] = 1;
+                s_any = 1;
+            }
+        __syncthreads();
+        if (!s_any) return;
+    }
+    // discount factors of the wave's N-tiles: lane group j loads N-tile j's (j < NT) and
+    // shares it with the other lane groups
+    float disc[NT];
     {
-        const int64_t p = pbase + 16 * grp + col;
-        const float d = (float)exp(-logb[(int64_t)e * P + (p < P ? p : 0)]);
+        const int64_t p = pbase + 16 * (grp < NT ? grp : 0) + col;
+        const float d = (float)exp(-logb[(int64_t)e * P + (p < count ? p : 0)]);
 #pragma unroll
-        for (int n = 0; n < kValNT; ++n) disc[n] = __shfl(d, col + 16 * n, kWave);
+        for (int n = 0; n < NT; ++n) disc[n] = __shfl(d, col + 16 * n, kWave);
     }
-    Split8 bs[kValNT][2];
+    Split8 bs[NT][2];
 #pragma unroll
-    for (int n = 0; n < kValNT; ++n) {
+    for (int n = 0; n < NT; ++n) {
         const int64_t p = pbase + 16 * n + col;
-        const float* fc = curves + (int64_t)e * kTenors * P + (p < P ? p : 0);
-        // segment 0: periods k = 8 grp + 1 .. 8 grp + 8 (tenors 8 grp .. 8 grp + 8 <= 32);
-        // segment 1: k = 33 + 8 grp .. 40 + 8 grp (tenors capped at the last, 50)
-        float fa[9], fb[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) fa[i] = fc[(int64_t)(8 * grp + i) * P];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            const int j = 32 + 8 * grp + i;
-            fb[i] = fc[(int64_t)(j < kTenors - 1 ? j : kTenors - 1) * P];
-        }
-        float ca[8], cb[8];
-        float s = 0.f;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            s += hw * (fa[i] + fa[i + 1]);
-            ca[i] = s;
-        }
-        s = 0.f;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int k = 33 + 8 * grp + i;
-            s += k <= kTenors - 1 ? hw * (fb[i] + fb[i + 1]) : dtf * fb[i];
-            cb[i] = s;
-        }
-        float t0[4], t1[4];
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            t0[g] = __shfl(ca[7], col + 16 * g, kWave);
-            t1[g] = __shfl(cb[7], col + 16 * g, kWave);
-        }
-        float offa = 0.f, offb = ((t0[0] + t0[1]) + t0[2]) + t0[3];
-#pragma unroll
-        for (int g = 0; g < 3; ++g) {
-            offa += g < grp ? t0[g] : 0.f;
-            offb += g < grp ? t1[g] : 0.f;
-        }
-        float pa[8], pb[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            pa[i] = expf(-(offa + ca[i]));
-            pb[i] = 32 + 8 * grp + i < kMaxPeriods ? expf(-(offb + cb[i])) : 0.f;
-        }
-        bs[n][0] = split8(pa);
-        bs[n][1] = split8(pb);
+        value_bonds(v, e, p < count ? p : 0, curves, grp, col, bs[n]);
     }
-    for (int m0 = 0; m0 < mtiles; m0 += kValLdsTiles) {
-        const int mt = mtiles - m0 < kValLdsTiles ? mtiles - m0 : kValLdsTiles;
+    for (int m0 = 0; m0 < mtiles; m0 += stage) {
+        const int mt = mtiles - m0 < stage ? mtiles - m0 : stage;
         __syncthreads();
         const u4_t* src = gfrag + ((int64_t)e * mtiles + m0) * kValueFragsPerTile * kWave;
-        for (int t = threadIdx.x; t < mt * kValueFragsPerTile * kWave; t += kValBlock) sfrag[t] = src[t];
+        for (int t = threadIdx.x; t < mt * kValueFragsPerTile * kWave; t += kThreads) sfrag[t] = src[t];
         __syncthreads();
         for (int mm = 0; mm < mt; ++mm) {
             const int m = m0 + mm;
+            if (filter && !s_need[m]) continue;
             const int tk = tilek[(int64_t)e * mtiles + m];
             const f4_t a4 = *reinterpret_cast<const f4_t*>(apad + ((int64_t)e * mtiles + m) * 16 + 4 * grp);
-            f4_t acc[kValNT];
+            f4_t acc[NT];
+            value_tile<NT>(sfrag + (mm * kValueFragsPerTile) * kWave, lane, tk, a4, bs, acc);
 #pragma unroll
-            for (int n = 0; n < kValNT; ++n) acc[n] = a4;
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                if (tk > 32 * c) {
-                    const u4_t* fr = sfrag + (mm * kValueFragsPerTile + c * kMfmaParts) * kWave + lane;
-                    const u4_t ah = fr[0], am = fr[kWave], al = fr[2 * kWave];
-#pragma unroll
-                    for (int n = 0; n < kValNT; ++n) {
-                        f4_t x = acc[n];
-                        x = mfma_bf16(ah, bs[n][c].h, x);
-                        x = mfma_bf16(ah, bs[n][c].m, x);
-                        x = mfma_bf16(am, bs[n][c].h, x);
-                        x = mfma_bf16(ah, bs[n][c].l, x);
-                        x = mfma_bf16(al, bs[n][c].h, x);
-                        x = mfma_bf16(am, bs[n][c].m, x);
-                        acc[n] = x;
-                    }
-                }
-            }
-#pragma unroll
-            for (int n = 0; n < kValNT; ++n) {
+            for (int n = 0; n < NT; ++n) {
                 const int64_t p = pbase + 16 * n + col;
-                if (p < P) {
+                if (p < count) {
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int g = 16 * m + 4 * grp + r;
-                        const float V = acc[n][r];
-                        if (g < G) exposures[((int64_t)e * G + g) * P + p] = V > 0.f ? V * disc[n] : 0.f;
+                        if (g < G) exposures[((int64_t)e * G + g) * P + p] = exposure_of(acc[n][r], disc[n]);
                     }
                 }
             }
@@ -562,16 +683,292 @@ __global__ __launch_bounds__(kValBlock) void pf_value_mfma_kernel(const PfValueP
     }
 }
 
+bool pf_value_shape_ok(int tiles, int waves)
+{
+    return (tiles == 2 || tiles == 4) && (waves == 4 || waves == 8);
+}
+
 hipError_t launch_pf_value_mfma(const PfValueParams& v, const float* curves, const double* logb,
                                 const uint16_t* gfrag, const float* apad, const int* tilek,
-                                float* ex, hipStream_t st)
+                                float* ex, const MfmaShape& shape, hipStream_t st, int64_t count,
+                                const SelRowState* filter, int64_t fk, int64_t fcap)
 {
     if (v.paths <= 0 || v.nsets <= 0 || v.ndates <= 0) return hipErrorInvalidValue;
     const int mtiles = value_mtiles(v.nsets);
-    const dim3 grid((unsigned)((v.paths + kValPaths - 1) / kValPaths), (unsigned)v.ndates);
-    pf_value_mfma_kernel<<<grid, kValBlock, 0, st>>>(v, mtiles, curves, logb,
-                                                     reinterpret_cast<const u4_t*>(gfrag), apad,
-                                                     tilek, ex);
+    if (mtiles > kMaxPfMTiles) return hipErrorInvalidValue;
+    const int tiles = shape.tiles ? shape.tiles : 4, waves = shape.waves ? shape.waves : 4;
+    if (!pf_value_shape_ok(tiles, waves) || shape.stage < 0 || shape.stage > kValMaxStage)
+        return hipErrorInvalidValue;
+    const int stage = std::min(shape.stage ? shape.stage : kValMaxStage, mtiles);
+    if (count < 0 || count > v.paths) count = v.paths;
+    if (count == 0) return hipSuccess;
+    const int64_t paths_per_block = (int64_t)waves * tiles * 16;
+    const dim3 grid((unsigned)((count + paths_per_block - 1) / paths_per_block), (unsigned)v.ndates);
+    const dim3 block((unsigned)(waves * kWave));
+    const size_t lds = (size_t)stage * kValueFragsPerTile * kWave * sizeof(u4_t);
+    const u4_t* gf = reinterpret_cast<const u4_t*>(gfrag);
+#define HJM_VALUE_LAUNCH(NT, W)                                                                  \
+    pf_value_mfma_kernel<NT, W><<<grid, block, lds, st>>>(v, mtiles, stage, count, curves, logb, gf, \
+                                                          apad, tilek, filter, fk, fcap, ex)
+    if (tiles == 4 && waves == 4) HJM_VALUE_LAUNCH(4, 4);
+    else if (tiles == 4 && waves == 8) HJM_VALUE_LAUNCH(4, 8);
+    else if (tiles == 2 && waves == 4) HJM_VALUE_LAUNCH(2, 4);
+    else HJM_VALUE_LAUNCH(2, 8);
+#undef HJM_VALUE_LAUNCH
+    return hipGetLastError();
+}
+
+// ---- fused valuation and statistics (FP32 plans) -------------------------------------
+// One workgroup of 4 waves values its 128 paths at every date in turn for one group of up
+// to 64 netting sets (blockIdx.y; each wave 2 N-tiles of 16 paths, the date's -gamma
+// fragments of the group's 4 M-tiles staged in LDS) and reduces the exposures where they
+// are produced, so they never go to HBM:
+//   * EE numerators: per date and set the workgroup's sum (FP32 sums of a wave's 32 paths
+//     by row reductions of the accumulator layout, then the waves in order in FP64),
+//     partials[set][block][date];
+//   * the per-path CVA sample sum_e w_ge E_pge in registers (FP32) across the dates, its
+//     sum and square sum per set at the end, partials[set][block][ndates + 0, 1];
+//   * with sel: the counting pass of the PFE's sampled-bracket selection for every row
+//     (date, set) -- entries below the bracket (or the zeros), entries inside it counted
+//     and staged in LDS (kFuseSlots per set and date; more flags the row overflowed, so
+//     its selection revalues the whole row), then one claim per row and workgroup and the
+//     copy of the staged entries to the row's candidate buffer.
+// The per-path CVA of 64 sets x 128 paths takes 32 VGPRs per lane; a group of 128 would
+// not fit beside the valuation, so set groups re-derive their paths' bond prices; at
+// 2 waves per SIMD two workgroups share a CU, one computing while the other reduces. The
+// exposures are those of pf_value_mfma_kernel bit for bit (same helpers).
+constexpr int kFuseWaves = 4;
+constexpr int kFuseBlock = kFuseWaves * kWave;
+constexpr int kFuseNT = 2;
+constexpr int kFusePaths = kFuseWaves * kFuseNT * 16;   // 128
+constexpr int kFuseMTiles = 4;                          // M-tiles per set group
+constexpr int kFuseSets = 16 * kFuseMTiles;             // 64 sets per group
+constexpr int kFuseSlots = 32;                          // staged inside entries per set and date
+
+// v + (v of the lane moved by the DPP control), within rows of 16 lanes (FP64 and u32)
+template <int kCtrl>
+__device__ __forceinline__ double dpp_row_add_f64(double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, kCtrl, 0xf, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), kCtrl, 0xf, 0xf, false);
+    return v + __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+template <int kCtrl>
+__device__ __forceinline__ uint32_t dpp_row_add_u32(uint32_t v)
+{
+    return v + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, 0xf, 0xf, false);
+}
+// sums over each row of 16 lanes, in lane 15 of the row
+__device__ __forceinline__ double row_sum_f64(double v)
+{
+    v = dpp_row_add_f64<0xb1>(v);   // quad_perm [1, 0, 3, 2]
+    v = dpp_row_add_f64<0x4e>(v);   // quad_perm [2, 3, 0, 1]
+    v = dpp_row_add_f64<0x114>(v);  // row_shr:4
+    return dpp_row_add_f64<0x118>(v);  // row_shr:8
+}
+template <int kCtrl>
+__device__ __forceinline__ float dpp_row_add_f32(float v)
+{
+    return v + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), kCtrl,
+                                                                      0xf, 0xf, false));
+}
+__device__ __forceinline__ float row_sum_f32(float v)
+{
+    v = dpp_row_add_f32<0xb1>(v);
+    v = dpp_row_add_f32<0x4e>(v);
+    v = dpp_row_add_f32<0x114>(v);
+    return dpp_row_add_f32<0x118>(v);
+}
+__device__ __forceinline__ uint32_t row_sum_u32(uint32_t v)
+{
+    v = dpp_row_add_u32<0xb1>(v);
+    v = dpp_row_add_u32<0x4e>(v);
+    v = dpp_row_add_u32<0x114>(v);
+    return dpp_row_add_u32<0x118>(v);
+}
+
+__global__ __launch_bounds__(kFuseBlock) void pf_value_stats_mfma_kernel(
+    const PfValueParams v, int mtiles, const float* __restrict__ curves,
+    const double* __restrict__ logb, const u4_t* __restrict__ gfrag, const float* __restrict__ apad,
+    const int* __restrict__ tilek, const double* __restrict__ w, double* __restrict__ partials,
+    SelRowState* __restrict__ sel, float* __restrict__ cand, int64_t cap, int slots)
+{
+    // the date's fragments of the group's M-tiles; at the end, the per-wave CVA moments
+    __shared__ u4_t sfrag[kFuseMTiles * kValueFragsPerTile * kWave];
+    __shared__ double s_ee[kFuseWaves][kFuseSets];
+    __shared__ float s_w[kFuseSets];
+    __shared__ uint32_t s_lo[kFuseSets], s_hi[kFuseSets];
+    __shared__ uint32_t s_cnt[kFuseSets], s_nin[kFuseSets], s_base[kFuseSets];
+    __shared__ float s_cand[kFuseSets][kFuseSlots];
+    const int64_t P = v.paths;
+    const int G = v.nsets, nd = v.ndates, ncol = nd + 2;
+    const int m0 = blockIdx.y * kFuseMTiles, g0 = 16 * m0;   // the group's first M-tile, set
+    const int mt = mtiles - m0 < kFuseMTiles ? mtiles - m0 : kFuseMTiles;
+    const int ng = G - g0 < kFuseSets ? G - g0 : kFuseSets;   // sets of the group
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const int col = lane % 16, grp = lane / 16;
+    const int tiles = (int)gridDim.x;
+    const int64_t pbase = (int64_t)blockIdx.x * kFusePaths + wave * (kFuseNT * 16);
+    bool live[kFuseNT];
+#pragma unroll
+    for (int n = 0; n < kFuseNT; ++n) live[n] = pbase + 16 * n + col < P;
+    float cvap[kFuseMTiles][kFuseNT][4];
+#pragma unroll
+    for (int m = 0; m < kFuseMTiles; ++m)
+#pragma unroll
+        for (int n = 0; n < kFuseNT; ++n)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) cvap[m][n][r] = 0.f;
+
+    for (int e = 0; e < nd; ++e) {
+        __syncthreads();  // the previous date's LDS is no longer read
+        const u4_t* src = gfrag + ((int64_t)e * mtiles + m0) * kValueFragsPerTile * kWave;
+        for (int t = threadIdx.x; t < mt * kValueFragsPerTile * kWave; t += kFuseBlock) sfrag[t] = src[t];
+        for (int gl = threadIdx.x; gl < kFuseSets; gl += kFuseBlock) {
+            const bool in = gl < ng;
+            const int64_t rid = (int64_t)e * G + g0 + gl;
+            s_w[gl] = in ? (float)w[(int64_t)(g0 + gl) * nd + e] : 0.f;
+            s_lo[gl] = (in && sel) ? (uint32_t)sel[rid].lo : 0u;
+            s_hi[gl] = (in && sel) ? (uint32_t)sel[rid].hi : 0u;
+            s_cnt[gl] = 0u;
+            s_nin[gl] = 0u;
+        }
+        __syncthreads();
+        float disc[kFuseNT];
+        {
+            // lane groups 0 and 1 load the discount factors of N-tiles 0 and 1
+            const int64_t p = pbase + 16 * (grp & 1) + col;
+            const float d = (float)exp(-logb[(int64_t)e * P + (p < P ? p : 0)]);
+#pragma unroll
+            for (int n = 0; n < kFuseNT; ++n) disc[n] = __shfl(d, col + 16 * n, kWave);
+        }
+        Split8 bs[kFuseNT][2];
+#pragma unroll
+        for (int n = 0; n < kFuseNT; ++n) {
+            const int64_t p = pbase + 16 * n + col;
+            value_bonds(v, e, p < P ? p : 0, curves, grp, col, bs[n]);
+        }
+#pragma unroll
+        for (int m = 0; m < kFuseMTiles; ++m) {
+            if (m >= mt) break;
+            const int tk = tilek[(int64_t)e * mtiles + m0 + m];
+            const f4_t a4 = *reinterpret_cast<const f4_t*>(apad + ((int64_t)e * mtiles + m0 + m) * 16 + 4 * grp);
+            f4_t acc[kFuseNT];
+            value_tile<kFuseNT>(sfrag + (m * kValueFragsPerTile) * kWave, lane, tk, a4, bs, acc);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gl = 16 * m + 4 * grp + r;   // set g0 + gl; weight 0 past the last
+                const float wg = s_w[gl];
+                const uint32_t lo = s_lo[gl], hi = s_hi[gl];
+                const uint32_t lo1 = lo ? lo : 1u;     // inside: non-zero and >= lo
+                const bool any_in = sel && gl < ng && hi >= lo1;
+                const bool collect = any_in && lo != hi;
+                float es = 0.f;
+                uint32_t nlo = 0;
+#pragma unroll
+                for (int n = 0; n < kFuseNT; ++n) {
+                    const float E = live[n] ? exposure_of(acc[n][r], disc[n]) : 0.f;
+                    cvap[m][n][r] = __builtin_fmaf(wg, E, cvap[m][n][r]);
+                    es += E;
+                    const uint32_t b = __builtin_bit_cast(uint32_t, E);
+                    nlo += (live[n] && b < lo1) ? 1u : 0u;
+                    if (any_in && live[n] && b - lo1 <= hi - lo1) {
+                        const uint32_t pos = atomicAdd(&s_nin[gl], 1u);
+                        if (collect && pos < (uint32_t)slots) s_cand[gl][pos] = E;
+                    }
+                }
+                es = row_sum_f32(es);
+                if (col == 15) s_ee[wave][gl] = (double)es;
+                if (sel) {
+                    nlo = row_sum_u32(nlo);
+                    if (col == 15 && nlo) atomicAdd(&s_cnt[gl], nlo);
+                }
+            }
+        }
+        __syncthreads();
+        for (int gl = threadIdx.x; gl < ng; gl += kFuseBlock) {
+            const int g = g0 + gl;
+            double t = 0.0;
+#pragma unroll
+            for (int ww = 0; ww < kFuseWaves; ++ww) t += s_ee[ww][gl];
+            partials[((int64_t)g * tiles + blockIdx.x) * ncol + e] = t;
+            if (sel) {
+                const int64_t rid = (int64_t)e * G + g;
+                const uint32_t lo = s_lo[gl], hi = s_hi[gl], nlo = s_cnt[gl], nin = s_nin[gl];
+                const uint32_t lo1 = lo ? lo : 1u;
+                const bool collect = hi >= lo1 && lo != hi;
+                if (nlo) atomicAdd(lo ? &sel[rid].below : &sel[rid].zin, nlo);
+                if (nin) atomicAdd(&sel[rid].nin, nin);
+                uint32_t base = 0;
+                if (collect && nin > (uint32_t)slots) atomicOr(&sel[rid].claim, kSelOverflow);
+                else if (collect && nin) base = atomicAdd(&sel[rid].claim, nin);
+                s_base[gl] = base;
+            }
+        }
+        __syncthreads();
+        if (sel) {
+            for (int t = threadIdx.x; t < ng * kFuseSlots; t += kFuseBlock) {
+                const int gl = t / kFuseSlots, i = t % kFuseSlots;
+                const uint32_t lo = s_lo[gl], hi = s_hi[gl], nin = s_nin[gl];
+                const uint32_t lo1 = lo ? lo : 1u;
+                const bool collect = hi >= lo1 && lo != hi;
+                if (collect && nin <= (uint32_t)slots && (uint32_t)i < nin) {
+                    const int64_t slot = (int64_t)s_base[gl] + i;
+                    if (slot < cap) cand[((int64_t)e * G + g0 + gl) * cap + slot] = s_cand[gl][i];
+                }
+            }
+        }
+    }
+    // per-path CVA moments per set: row sums, then the waves in order
+    __syncthreads();
+    double* s_cva = reinterpret_cast<double*>(sfrag);   // [wave][set][2]
+#pragma unroll
+    for (int m = 0; m < kFuseMTiles; ++m) {
+        if (m >= mt) break;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            double c1 = 0.0, c2 = 0.0;
+#pragma unroll
+            for (int n = 0; n < kFuseNT; ++n) {
+                const double c = (double)cvap[m][n][r];
+                c1 += c;
+                c2 += c * c;
+            }
+            c1 = row_sum_f64(c1);
+            c2 = row_sum_f64(c2);
+            const int gl = 16 * m + 4 * grp + r;
+            if (col == 15) {
+                s_cva[(wave * kFuseSets + gl) * 2] = c1;
+                s_cva[(wave * kFuseSets + gl) * 2 + 1] = c2;
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 2 * ng; t += kFuseBlock) {
+        const int gl = t / 2, q = t % 2;
+        double s = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < kFuseWaves; ++ww) s += s_cva[(ww * kFuseSets + gl) * 2 + q];
+        partials[((int64_t)(g0 + gl) * tiles + blockIdx.x) * ncol + nd + q] = s;
+    }
+}
+
+int64_t pf_fused_blocks(int64_t paths) { return (paths + kFusePaths - 1) / kFusePaths; }
+
+hipError_t launch_pf_value_stats_mfma(const PfValueParams& v, const float* curves, const double* logb,
+                                      const uint16_t* gfrag, const float* apad, const int* tilek,
+                                      const double* w, double* partials, SelRowState* sel,
+                                      float* cand, int64_t cap, int stage_slots, hipStream_t st)
+{
+    const int slots = stage_slots > 0 && stage_slots < kFuseSlots ? stage_slots : kFuseSlots;
+    const int mtiles = value_mtiles(v.nsets);
+    if (v.paths <= 0 || v.nsets <= 0 || mtiles > kMaxPfMTiles) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)pf_fused_blocks(v.paths), (unsigned)((mtiles + kFuseMTiles - 1) / kFuseMTiles));
+    pf_value_stats_mfma_kernel<<<grid, kFuseBlock, 0, st>>>(v, mtiles, curves, logb,
+                                                            reinterpret_cast<const u4_t*>(gfrag),
+                                                            apad, tilek, w, partials, sel, cand, cap,
+                                                            slots);
     return hipGetLastError();
 }
 

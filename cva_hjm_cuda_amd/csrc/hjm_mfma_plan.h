@@ -210,6 +210,7 @@ inline void build_mfma_block_map(const double* drift, const double* vols, double
 constexpr int kValueFragsPerTile = 2 * kMfmaParts;  // u4 fragments per lane and M-tile
 
 inline int value_mtiles(int nsets) { return (nsets + 15) / 16; }
+constexpr int kMaxPfMTiles = (HJM_MAX_SETS + 15) / 16;
 
 inline void build_value_fragments(const std::vector<double>& coef, const std::vector<double>& a,
                                   const std::vector<int>& kmax, int ndates, int nsets, int kpad,

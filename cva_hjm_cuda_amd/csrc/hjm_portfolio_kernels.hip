@@ -521,6 +521,14 @@ static hipError_t pf_stats_launch(const PfValueParams& v, const Real* ex, const 
     return hipGetLastError();
 }
 
+hipError_t launch_grouped_column_sum(const double* partials, int nsets, int64_t blocks, int cols,
+                                     double* sums, hipStream_t st)
+{
+    grouped_column_sum_kernel<<<dim3((unsigned)cols, (unsigned)nsets), kBlock, 0, st>>>(partials, blocks,
+                                                                                      cols, sums);
+    return hipGetLastError();
+}
+
 hipError_t launch_pf_stats_f64(const PfValueParams& v, const double* ex, const double* w,
                                double* partials, double* sums, void* selws, int64_t pfe_rank,
                                hipStream_t st)

@@ -340,6 +340,28 @@ hipError_t launch_column_sum(const double* partials, int64_t rows, int cols, dou
     return hipGetLastError();
 }
 
+// The Box-Muller factors of n words: r[i] = radius of words[i] as the first word of a pair,
+// c[i], s[i] = cosine and sine of words[i] as the second.
+__global__ __launch_bounds__(kBlock) void box_muller_parts_kernel(const uint32_t* __restrict__ words,
+                                                                  int64_t n, float* __restrict__ r,
+                                                                  float* __restrict__ c,
+                                                                  float* __restrict__ s)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t w = words[i];
+    r[i] = bm_radius(w);
+    bm_angle(w, c[i], s[i]);
+}
+
+hipError_t launch_box_muller_parts(const uint32_t* words, int64_t n, float* r, float* c, float* s,
+                                   hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    box_muller_parts_kernel<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(words, n, r, c, s);
+    return hipGetLastError();
+}
+
 hipError_t launch_normals(uint64_t seed, int64_t path0, int64_t paths, int nsteps, int stride,
                           double* out, hipStream_t st)
 {

@@ -76,13 +76,26 @@ HJM_HD void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, ui
 // distinct u and t: |r - r_exact| <= 1.4e-7 r, |cos - cos_exact|, |sin - sin_exact| <=
 // 1.3e-7, so |z - z_exact| <= 2.6e-7 |z| + 8.8e-7 (r <= sqrt(66 ln 2) = 6.77). Four
 // transcendental and seven plain VALU instructions per pair.
-__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float& z0, float& z1)
+// The radius of word a and the cosine and sine of word b (the factors of box_muller).
+__device__ __forceinline__ float bm_radius(uint32_t a)
 {
     const float u = __builtin_fmaf((float)a, 0x1p-32f, 0x1p-33f);
-    const float r = __builtin_amdgcn_sqrtf(-0x1.62e430p+0f * __builtin_amdgcn_logf(u));
+    return __builtin_amdgcn_sqrtf(-0x1.62e430p+0f * __builtin_amdgcn_logf(u));
+}
+__device__ __forceinline__ void bm_angle(uint32_t b, float& c, float& s)
+{
     const float t = (float)b * 0x1p-32f;
-    z0 = r * __builtin_amdgcn_cosf(t);
-    z1 = r * __builtin_amdgcn_sinf(t);
+    c = __builtin_amdgcn_cosf(t);
+    s = __builtin_amdgcn_sinf(t);
+}
+
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float& z0, float& z1)
+{
+    const float r = bm_radius(a);
+    float c, s;
+    bm_angle(b, c, s);
+    z0 = r * c;
+    z1 = r * s;
 }
 
 // Shocks 4 call .. 4 call + 3 of the stream (path, block, cls).

@@ -469,6 +469,18 @@ __global__ __launch_bounds__(kCountBlock) void bracket_count_kernel(const T* __r
     }
 }
 
+// True when the counting pass's results answer the row (bracket_finish_kernel then reads
+// only the counts and the candidates); false when the sample missed rank k or the
+// candidates did not all fit, and the selection must see the whole row.
+__host__ __device__ inline bool bracket_row_resolved(const SelRowState& s, int64_t k, int64_t cap)
+{
+    const int64_t below = s.below, zin = s.zin, nin = s.nin;
+    if (!(below < k && k <= below + zin + nin)) return false;
+    if (k - below <= zin || s.lo == s.hi) return true;
+    // (the top bit of claim: a counting pass could not stage every candidate)
+    return nin <= cap && !(s.claim & kSelOverflow);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kSelBlock) void bracket_finish_kernel(const T* __restrict__ x, int64_t n,
                                                                    int64_t k,
@@ -479,23 +491,16 @@ __global__ __launch_bounds__(kSelBlock) void bracket_finish_kernel(const T* __re
     typedef typename SelectBits<T>::U U;
     const int row = blockIdx.x;
     const SelRowState s = state[row];
-    const int64_t below = s.below, zin = s.zin, nin = s.nin;
-    if (below < k && k <= below + zin + nin) {
-        int64_t rk = k - below;
-        if (rk <= zin) {
+    if (bracket_row_resolved(s, k, cap)) {
+        const int64_t rk = k - (int64_t)s.below;
+        if (rk <= (int64_t)s.zin) {
             if (threadIdx.x == 0) out[row] = 0.0;
-            return;
-        }
-        rk -= zin;
-        if ((U)s.lo == (U)s.hi) {
+        } else if ((U)s.lo == (U)s.hi) {
             if (threadIdx.x == 0) out[row] = (double)__builtin_bit_cast(T, (U)s.lo);
-            return;
+        } else {
+            select_kth_row<T>(cand + (int64_t)row * cap, s.nin, rk - (int64_t)s.zin, out + row, true);
         }
-        if (nin <= cap && !(s.claim & kSelOverflow)) {
-            // (the top bit of claim: a counting pass could not stage every candidate)
-            select_kth_row<T>(cand + (int64_t)row * cap, nin, rk, out + row, true);
-            return;
-        }
+        return;
     }
     select_kth_row<T>(x + (int64_t)row * n, n, k, out + row);
 }

@@ -203,15 +203,30 @@ typedef struct {
     double dtau;                  /* date spacing = payment period (0.5) */
 } hjm_portfolio;
 
-/* Launch shape of the VALU path kernel (FP64, host normals, or path_kernel = VECTOR); the
- * matrix-core FP32 path kernel has a fixed shape (512 threads, 256 paths per workgroup)
- * and uses only use_graph. */
+/* Launch shapes of the portfolio plan's kernels (0 in a field: its default). The VALU
+ * path kernel (FP64, host normals, or path_kernel = VECTOR) uses block_size and
+ * blocks_per_cu; the matrix-core FP32 path kernel path_tiles, path_waves and
+ * blocks_per_cu (0 or 1: its block map fills 84 KB of LDS, one workgroup per CU); the
+ * matrix-core valuation value_tiles, value_waves and value_stage. Results do not depend
+ * on the shapes. hjm_portfolio_autotune measures them. */
 typedef struct {
-    int32_t block_size;     /* threads per workgroup of the path kernel: 64, 128 or 256 */
-    int32_t blocks_per_cu;  /* persistent path kernel: grid = blocks_per_cu x CUs;
-                               0 = one workgroup per tile of paths */
+    int32_t block_size;     /* threads per workgroup of the VALU path kernel: 64, 128 or 256 */
+    int32_t blocks_per_cu;  /* persistent path kernel: grid = blocks_per_cu x CUs workgroups
+                               looping over the groups of paths; 0 = one workgroup per group */
     int32_t use_graph;      /* capture all passes in one hipGraph, replay it per execute */
-    int32_t reserved;
+    int32_t fuse_statistics; /* FP32 plans: 1 = value and reduce in one pass that keeps the
+                                exposures out of HBM (EE, per-path CVA moments, the PFE's
+                                bracket counting); 0 = valuation writes the exposures, a
+                                statistics pass reads them */
+    int32_t pfe_stage_slots; /* fused pass: PFE candidates a workgroup stages per netting
+                                set and date (1..32; 0: 32); a row with more is valued again,
+                                whole, for its selection (exact either way) */
+    int32_t path_tiles;     /* matrix-core path kernel: N-tiles of 16 paths per wave, and */
+    int32_t path_waves;     /* waves per workgroup: {2, 8} (default), {2, 12}, {1, 12}, {1, 16} */
+    int32_t value_tiles;    /* matrix-core valuation: N-tiles per wave, 2 or 4 (default) */
+    int32_t value_waves;    /* waves per workgroup, 4 (default) or 8 */
+    int32_t value_stage;    /* M-tiles (16 sets) of -gamma staged in LDS at a time, 1..8
+                               (default 8) */
 } hjm_tuning;
 
 typedef struct {
@@ -240,9 +255,12 @@ int hjm_simulate_portfolio(const hjm_market* market, const hjm_portfolio* portfo
 int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* portfolio,
                              const hjm_config* config, const hjm_tuning* tuning,
                              hjm_greeks* greeks);
-/* Times the path kernel over block sizes {64, 128, 256} x persistent grids {0, 1, 2, 4, 8
- * workgroups per CU} on the problem's own path count and a 2-date horizon; writes the
- * fastest into *best (use_graph set) and its time into *best_ms. */
+/* Measures launch shapes on the caller's problem and writes the fastest into *best
+ * (use_graph set) and the eager time of one execute with it into *best_ms. Matrix-core
+ * FP32 plans: the path kernel over its four shapes x {one workgroup per group, persistent
+ * one per CU}, the valuation over tiles x waves x LDS stage, then the exposure pass
+ * against the fused valuation-statistics pass. VALU path kernels: block sizes {64, 128,
+ * 256} x persistent grids {0, 1, 2, 4, 8 workgroups per CU} on a 2-date horizon. */
 int hjm_portfolio_autotune(const hjm_market* market, const hjm_portfolio* portfolio,
                            const hjm_config* config, hjm_tuning* best, double* best_ms);
 
@@ -307,6 +325,12 @@ int hjm_calibrate_pca(const hjm_calibration* calibration, hjm_calibration_result
  * date blocks), written as [paths][nsteps][3] doubles. Runs on the GPU.              */
 int hjm_generate_normals(uint64_t seed, int64_t path_offset, int64_t paths, int32_t nsteps,
                          int32_t stride, int32_t device, double* out);
+/* The generator's Box-Muller factors on caller words (no counterpart in the reference;
+ * for tests of the transform's error bound): radius[i] = sqrt(-2 ln u) of words[i] as the
+ * first word of a pair, cosine[i] / sine[i] = cos / sin(2 pi t) of words[i] as the second,
+ * exactly as the path kernels evaluate them. 1 <= n <= 2^28.                          */
+int hjm_box_muller_parts(const uint32_t* words, int64_t n, int32_t device, float* radius,
+                         float* cosine, float* sine);
 
 /* ---- the PFE selection on caller data (no counterpart in the reference): out[r] = the
  * k-th smallest (1-based) of row r of values [rows][n], all >= 0 (exact, by the GPU

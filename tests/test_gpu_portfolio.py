@@ -3,9 +3,11 @@
 FP64 on identical normals (the generator's own, hjm_generate_normals): per-path
 netting-set exposures, EE, CVA and SE within 1e-10 relative of the oracle; PFE the exact
 order statistic; in-kernel Philox bit-identical to the same normals supplied from the host; hipGraph replays bit-identical to eager runs (several replays with eager
-work in between); tuned persistent grids bit-identical to the default grid; FP32 versus
+work in between); tuned persistent grids and every launch shape of the matrix-core
+kernels bit-identical to the defaults, the autotuner's choice replayed bit for bit; FP32 versus
 FP64 at the batched configuration's size; shard additivity.
 """
+import dataclasses
 import math
 
 import numpy as np
@@ -100,6 +102,41 @@ def test_portfolio_fp32_pfe_exact_on_long_rows():
         assert np.array_equal(r.pfe, np.sort(r.exposures, axis=0)[k - 1]), (horizon, q)
 
 
+def test_fused_statistics_equal_the_exposure_pass():
+    """FP32 plans can reduce the exposures inside the valuation (EE, per-path CVA moments,
+    the PFE bracket counting; tuning.fuse_statistics) unless the per-path exposures are
+    asked for. On one plan, 72 sets
+    (two set groups of the fused pass) x 70000 paths x 30y: the fused run's PFE equals the
+    exposure pass's (the exact order statistic of the same exposures), EE and CVA to 1e-6,
+    the standard error to FP32 accumulation of the per-path CVA; and with
+    one candidate slot per set and date the rows overflow, are valued again whole, and the
+    PFE is still exact. At the 97.5 % and 50 % quantiles. (EE and CVA: FP32 partial sums of
+    32 paths in the fused pass, FP64 per path in the exposure pass.)"""
+    m = hjm.benchmark_market()
+    pf = hjm.synthetic_portfolio(144, 72, seed=5)
+    P = 70000
+    for q in (0.975, 0.5):
+        _flush("[gpu] fused statistics vs exposure pass, q = %g" % q)
+        plan = hjm.PortfolioPlan(m, pf, P, horizon=30.0, seed=9, pfe_quantile=q,
+                                 tuning=hjm.Tuning(fuse_statistics=True))
+        a = plan.execute(keep_paths=True)
+        b = plan.execute()
+        plan.close()
+        k = math.ceil(q * P)
+        want = np.sort(a.exposures.astype(np.float32), axis=0)[k - 1].astype(np.float64)
+        assert np.array_equal(a.pfe, want)
+        assert np.array_equal(b.pfe, a.pfe)
+        assert np.allclose(b.expected_exposure, a.expected_exposure, rtol=1e-6, atol=0)
+        assert np.allclose(b.cva, a.cva, rtol=1e-6, atol=0)
+        assert np.allclose(b.cva_stderr, a.cva_stderr, rtol=1e-5, atol=0)
+        plan = hjm.PortfolioPlan(m, pf, P, horizon=30.0, seed=9, pfe_quantile=q,
+                                 tuning=hjm.Tuning(fuse_statistics=True, pfe_stage_slots=1))
+        c = plan.execute()
+        plan.close()
+        assert np.array_equal(c.pfe, a.pfe)
+        assert np.array_equal(c.expected_exposure, b.expected_exposure)
+
+
 def test_portfolio_in_kernel_rng_equals_host_normals(small):
     m, pf, z, _ = small
     a = hjm.simulate_portfolio(m, pf, 128, horizon=30.0, precision=hjm.FP64, normals=z)
@@ -146,8 +183,62 @@ def test_graph_replays_and_tuned_grids_are_bit_identical():
                                    path_kernel=hjm.KERNEL_VECTOR)
         assert np.array_equal(r.expected_exposure, eager_v.expected_exposure), t
     assert np.max(np.abs(eager_v.cva / eager.cva - 1)) < 1e-4
-    tune, ms = hjm.autotune_portfolio(m, pf, 70000, horizon=30.0, seed=9)
-    assert tune.block_size in (64, 128, 256) and ms > 0
+    tune, ms = hjm.autotune_portfolio(m, pf, 70000, horizon=30.0, seed=9,
+                                      path_kernel=hjm.KERNEL_VECTOR)
+    assert tune.block_size in (64, 128, 256) and tune.blocks_per_cu in (0, 1, 2, 4, 8)
+    assert tune.use_graph and ms > 0
+
+
+def _matrix_shapes():
+    """Every launch shape of the matrix-core path kernel (x persistent grid) and of the
+    valuation (x LDS stage) that hjm_tuning accepts."""
+    for tiles, waves in ((2, 8), (2, 12), (1, 12), (1, 16)):
+        for bpc in (0, 1):
+            yield hjm.Tuning(blocks_per_cu=bpc, path_tiles=tiles, path_waves=waves)
+    for tiles in (2, 4):
+        for waves in (4, 8):
+            for stage in (0, 1, 3):
+                yield hjm.Tuning(value_tiles=tiles, value_waves=waves, value_stage=stage)
+
+
+def test_matrix_shapes_autotune_and_tuned_graph_are_bit_identical():
+    """The matrix-core kernels' launch shapes change no result bit (each path and each
+    exposure is computed by the same instructions whatever the tiling), the autotuner
+    returns one of them, and the hipGraph captured at the tuned shape replays bit for bit
+    what an eager run at that shape gives. 40 sets (3 M-tiles: a stage of 1 or 3 splits
+    them), 70000 paths (not a multiple of any group: ragged last groups)."""
+    _flush("[gpu] matrix-core launch shapes, autotuner, graph at the tuned shape")
+    m = hjm.benchmark_market()
+    pf = hjm.synthetic_portfolio(120, 40, seed=5)
+    P = 70000
+    base = hjm.simulate_portfolio(m, pf, P, horizon=30.0, seed=9)
+    for t in _matrix_shapes():
+        r = hjm.simulate_portfolio(m, pf, P, horizon=30.0, seed=9, tuning=t)
+        assert np.array_equal(r.expected_exposure, base.expected_exposure), t
+        assert np.array_equal(r.pfe, base.pfe) and np.array_equal(r.cva, base.cva), t
+        assert np.array_equal(r.cva_stderr, base.cva_stderr), t
+    with pytest.raises(hjm.HJMError):
+        hjm.simulate_portfolio(m, pf, 1024, horizon=30.0, seed=9,
+                               tuning=hjm.Tuning(path_tiles=1, path_waves=8))
+    with pytest.raises(hjm.HJMError):
+        hjm.simulate_portfolio(m, pf, 1024, horizon=30.0, seed=9,
+                               tuning=hjm.Tuning(value_tiles=3))
+    tune, ms = hjm.autotune_portfolio(m, pf, P, horizon=30.0, seed=9)
+    assert tune.use_graph and ms > 0
+    assert (tune.path_tiles, tune.path_waves) in ((2, 8), (2, 12), (1, 12), (1, 16)), tune
+    assert tune.blocks_per_cu in (0, 1) and tune.value_tiles in (2, 4), tune
+    assert tune.value_waves in (4, 8) and tune.value_stage in (2, 4, 8), tune
+    eager_t = dataclasses.replace(tune, use_graph=False)
+    ref = hjm.simulate_portfolio(m, pf, P, horizon=30.0, seed=9, tuning=eager_t)
+    if not tune.fuse_statistics:
+        assert np.array_equal(ref.expected_exposure, base.expected_exposure)
+    plan = hjm.PortfolioPlan(m, pf, P, horizon=30.0, seed=9, tuning=tune)
+    for rep in range(3):
+        r = plan.execute()
+        assert np.array_equal(r.expected_exposure, ref.expected_exposure), rep
+        assert np.array_equal(r.pfe, ref.pfe) and np.array_equal(r.cva, ref.cva), rep
+        hjm.simulate_reference(m, hjm.InterestRateSwap(1e6, 0.02, 5.0), 512, seed=rep)
+    plan.close()
 
 
 def test_batched_config_fp32_vs_fp64_and_shards():

@@ -67,6 +67,44 @@ def test_gpu_normals_within_bound_of_oracle(oracle):
         assert np.median(np.abs(g - o)) < 1e-7, (seed, stride)
 
 
+def _distinct_float_words():
+    """Chunks of 32-bit words covering every distinct float32(word): all words below 2^24
+    (exact), then each octave [2^k, 2^(k+1)), k = 24..31, at its float spacing 2^(k-23),
+    and 2^32 - 1 (which rounds to 2^32)."""
+    yield np.arange(0, 1 << 23, dtype=np.uint64)
+    yield np.arange(1 << 23, 1 << 24, dtype=np.uint64)
+    for k in range(24, 32):
+        yield (1 << k) + (np.arange(1 << 23, dtype=np.uint64) << (k - 23))
+    yield np.array([(1 << 32) - 1], dtype=np.uint64)
+
+
+def test_box_muller_factors_within_their_bounds_exhaustive():
+    """The hardware transcendentals of the generator's Box-Muller (hjm_rng.h) against the
+    exact transform in FP64, over every distinct uniform: radius within 1.4e-7 relative,
+    cosine and sine within 1.3e-7 absolute. These two bounds give the normals' bound
+    3.3e-7 |z| + 9e-7 that the other normals tests apply (r <= 6.77)."""
+    _flush("[gpu] Box-Muller factors, every distinct uniform (84 M words)")
+    worst_r = worst_cs = 0.0
+    for a in _distinct_float_words():
+        r, c, s = hjm.box_muller_parts(a.astype(np.uint32))
+        af = a.astype(np.float32).astype(np.float64)    # float(word), as the kernels round it
+        u = (af * 2.0 ** -32 + 2.0 ** -33).astype(np.float32).astype(np.float64)  # one rounding
+        re = np.sqrt(-2.0 * np.log(u))
+        t = 2.0 * np.pi * (af * 2.0 ** -32)
+        pos = re > 0
+        assert np.all(r[~pos] == 0)
+        if pos.any():
+            worst_r = max(worst_r, float(np.max(np.abs(r[pos] - re[pos]) / re[pos])))
+        worst_cs = max(worst_cs, float(np.max(np.abs(c - np.cos(t)))),
+                       float(np.max(np.abs(s - np.sin(t)))))
+    print("  radius max rel %.4e, cos/sin max abs %.4e" % (worst_r, worst_cs))
+    assert worst_r <= 1.4e-7 and worst_cs <= 1.3e-7
+    # the normals' bound from the factors': |r c - re ce| <= |r - re| |c| + re |c - ce|
+    # plus the product's rounding (2^-24 relative), with re <= sqrt(66 ln 2)
+    rmax = math.sqrt(66.0 * math.log(2.0))
+    assert 1.4e-7 + 2.0 ** -24 <= 3.3e-7 and rmax * 1.3e-7 <= 9e-7
+
+
 def test_gpu_normals_large_sample(oracle):
     """2048 paths x 1000 steps (6.1 M normals, 3 M Box-Muller pairs) against the oracle
     within the transform's bound; moments and a Kolmogorov-Smirnov test of the GPU's own
