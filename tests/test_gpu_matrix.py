@@ -3,7 +3,7 @@
 Per path against the FP64 oracle on the generator's own normals (hjm_generate_normals,
 which the path kernels reproduce bit for bit): forward
 rates num_x within 2e-6 absolute and discount factors num_y within 1e-6 relative up to
-10y, and no further from FP64 than the VALU FP32 kernel at 25y; EE and
+10y, and (like the VALU FP32 kernel) within 5e-5 and 5e-4 at 25y; EE and
 CVA of the precision configuration (131072 paths x 10y) within 1e-4 relative of the FP64
 kernel; shard additivity; the kernel choice rules (AUTO, VECTOR, MATRIX).
 """
