@@ -11,7 +11,7 @@
 // shocks of the block are generated on the VALU, one step per lane group per quad, and
 // each 32-entry K chunk enters the matrix cores as soon as it is complete. One
 // workgroup of 8 waves shares W's fragments in LDS.
-//   ref_paths_mfma_kernel: reference semantics (7 M-tiles): the block integrals dA are
+//   ref_paths_mfma_kernel: reference semantics (up to 7 M-tiles): the block integrals dA are
 //     Kahan-added into (A, C); numeraire slot b+1 is recorded after block b.
 //   pf_paths_mfma_kernel: portfolio semantics (4 M-tiles): the curve is checkpointed at
 //     every date and the short rate's left-point sum (row 51) extends the log bank account.
@@ -120,7 +120,7 @@ __device__ __forceinline__ void for_each_index(F& f, std::integer_sequence<int, 
 
 // One 50-step block: acc = W x [fin; shocks of steps b*50 .. b*50+49; 1]. kStore: also
 // write the block's normals to zs[step][factor][path] (FP32) for a later reverse sweep.
-template <int MT, int NT, bool kStore>
+template <int MT, int NT, bool kStore, bool kInterleave>
 __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const float (&fin)[NT][16],
                                            f4_t (&acc)[MT][NT], const MfmaGrid& g,
                                            const int64_t (&path)[NT], const bool (&live)[NT],
@@ -210,7 +210,7 @@ __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const
         constexpr IssueSlot is = issue_slot<MT>(q, 0);
         if constexpr (is.chunk >= 0) {
             tiles(is.chunk, is.m0, is.m1, xs[is.chunk]);
-            if constexpr (MT == kMfmaMTiles) {
+            if constexpr (kInterleave) {
                 // within the quad: one MFMA, then a share of the RNG's VALU instructions
 #pragma unroll
                 for (int i = 0; i < (is.m1 - is.m0) * NT * 6; ++i) {
@@ -220,21 +220,26 @@ __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const
             }
         }
         // each quad is its own scheduling region (keeps the group solver's problems small)
-        if constexpr (MT == kMfmaMTiles) __builtin_amdgcn_sched_barrier(0);
+        if constexpr (kInterleave) __builtin_amdgcn_sched_barrier(0);
     };
     for_each_index(quad, std::make_integer_sequence<int, kMfmaQuads>{});
     tiles(kMfmaChunks - 1, 0, MT, xs[kMfmaChunks - 1]);
 }
 
-template <bool kStore>
+// Reference semantics. MT = the live M-tiles of the block map: the curve (tiles 0..3,
+// rows 0..50) and the integral rows 51 + j up to the last recorded slot j = ndates (a
+// tile past it holds tenors whose numeraire is never recorded: ref_mfma_tiles).
+template <bool kStore, int MT>
 __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGrid g,
                                                                 const u4_t* __restrict__ wfrag,
                                                                 double* __restrict__ num_x,
                                                                 double* __restrict__ num_y,
                                                                 float* __restrict__ zs)
 {
+    constexpr int kInt = MT - 3;  // M-tiles holding integral rows (tile 3 holds both kinds)
     extern __shared__ u4_t lds_w[];
-    for (int t = threadIdx.x; t < kMfmaFragElems / 8; t += kMfmaBlock) lds_w[t] = wfrag[t];
+    for (int t = threadIdx.x; t < MT * kMfmaChunks * kMfmaParts * kWave; t += kMfmaBlock)
+        lds_w[t] = wfrag[t];
     __syncthreads();
 
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
@@ -251,8 +256,8 @@ __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGr
 
     // curve rows 16 m + 4 grp + r of the N-tiles (B operand of the F chunks)
     float fin[kNT][16];
-    // Kahan (A, C) of the rows of tiles 3..6 (dA_j at row 51 + j)
-    float Ak[kNT][16], Ck[kNT][16];
+    // Kahan (A, C) of the rows of tiles 3 .. MT-1 (dA_j at row 51 + j)
+    float Ak[kNT][4 * kInt], Ck[kNT][4 * kInt];
 #pragma unroll
     for (int n = 0; n < kNT; ++n) {
 #pragma unroll
@@ -263,7 +268,7 @@ __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGr
                 fin[n][4 * m + r] = (R < kTenors) ? g.spot[R < kTenors ? R : 0] : 0.f;
             }
 #pragma unroll
-        for (int m = 3; m < 7; ++m)
+        for (int m = 3; m < MT; ++m)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int j = 16 * m + 4 * grp + r - kTenors;
@@ -283,13 +288,13 @@ __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGr
     }
 
     for (int b = 1; b < g.ndates; ++b) {
-        f4_t acc[kMfmaMTiles][kNT];
-        mfma_block<kMfmaMTiles, kNT, kStore>(lds_w, fin, acc, g, path, live, b, grp, lane, zs);
-        // integrals: Kahan-add the block's dA (tiles 3..6); curve rows become next input
+        f4_t acc[MT][kNT];
+        mfma_block<MT, kNT, kStore, true>(lds_w, fin, acc, g, path, live, b, grp, lane, zs);
+        // integrals: Kahan-add the block's dA (tiles 3 .. MT-1); curve rows become next input
 #pragma unroll
         for (int n = 0; n < kNT; ++n) {
 #pragma unroll
-            for (int m = 3; m < 7; ++m)
+            for (int m = 3; m < MT; ++m)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int i = 4 * (m - 3) + r;
@@ -303,7 +308,8 @@ __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGr
 #pragma unroll
                 for (int r = 0; r < 4; ++r) fin[n][4 * m + r] = acc[m][n][r];
         }
-        // slot b+1: x = curve row b+1, integral = row 51 + b + 1 (uniform positions)
+        // slot b+1: x = curve row b+1, integral = row 51 + b + 1 (uniform positions; the
+        // launcher's MT covers row 51 + ndates)
         const int slot = b + 1;
         const int Rx = slot, Ra = kTenors + slot;
         const int gx = (Rx % 16) / 4, ix = 4 * (Rx / 16) + Rx % 4;
@@ -312,13 +318,14 @@ __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGr
         for (int n = 0; n < kNT; ++n) {
             float xv = 0.f, av = 0.f, cv = 0.f;
 #pragma unroll
-            for (int t2 = 0; t2 < 16; ++t2) {
+            for (int t2 = 0; t2 < 16; ++t2)
                 if (t2 == ix) xv = fin[n][t2];
+#pragma unroll
+            for (int t2 = 0; t2 < 4 * kInt; ++t2)
                 if (t2 == ia) {
                     av = Ak[n][t2];
                     cv = Ck[n][t2];
                 }
-            }
             if (live[n]) {
                 if (grp == gx) num_x[(int64_t)slot * P + path[n]] = (double)xv;
                 if (grp == ga)
@@ -326,6 +333,14 @@ __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGr
             }
         }
     }
+}
+
+// Live M-tiles of the reference map for a run of ndates dates: the curve's four and the
+// integral rows 51 + j up to j = ndates (the last slot recorded).
+static int ref_mfma_tiles(int ndates)
+{
+    const int last = kTenors + (ndates > 1 ? ndates : 1);
+    return last / 16 + 1 < 4 ? 4 : last / 16 + 1;
 }
 
 // Portfolio semantics: dates e = 0 .. ndates-1 every 50 steps from t = 0. At each date the
@@ -394,7 +409,7 @@ __global__ __launch_bounds__(WAVES * kWave) void pf_paths_mfma_kernel(const Mfma
             }
             if (e + 1 >= g.ndates) break;
             f4_t acc[kMfmaPfMTiles][NT];
-            mfma_block<kMfmaPfMTiles, NT, kStore>(lds_w, fin, acc, g, path, live, e, grp, lane, zs);
+            mfma_block<kMfmaPfMTiles, NT, kStore, false>(lds_w, fin, acc, g, path, live, e, grp, lane, zs);
 #pragma unroll
             for (int n = 0; n < NT; ++n) {
 #pragma unroll
@@ -419,10 +434,14 @@ struct PfPathsKernels {
 
 hipError_t mfma_kernels_prepare()
 {
-    const void* fns[2 + 4 * kPathShapes] = {(const void*)ref_paths_mfma_kernel<false>,
-                                            (const void*)ref_paths_mfma_kernel<true>};
-    int lds[2 + 4 * kPathShapes] = {2 * kMfmaFragElems, 2 * kMfmaFragElems};
-    int k = 2;
+    const void* fns[8 + 4 * kPathShapes] = {
+        (const void*)ref_paths_mfma_kernel<false, 4>, (const void*)ref_paths_mfma_kernel<false, 5>,
+        (const void*)ref_paths_mfma_kernel<false, 6>, (const void*)ref_paths_mfma_kernel<false, 7>,
+        (const void*)ref_paths_mfma_kernel<true, 4>,  (const void*)ref_paths_mfma_kernel<true, 5>,
+        (const void*)ref_paths_mfma_kernel<true, 6>,  (const void*)ref_paths_mfma_kernel<true, 7>};
+    int lds[8 + 4 * kPathShapes];
+    for (int i = 0; i < 8; ++i) lds[i] = ref_mfma_lds_bytes(4 + i % 4);
+    int k = 8;
     auto add = [&](const void* f) {
         fns[k] = f;
         lds[k++] = 2 * kMfmaPfFragElems;
@@ -448,12 +467,21 @@ hipError_t mfma_kernels_prepare()
 hipError_t launch_ref_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, double* nx, double* ny,
                                  float* zs, hipStream_t st)
 {
+    if (g.ndates < 1 || g.ndates > kTenors - 1) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((g.paths + kMfmaPathsPerBlock - 1) / kMfmaPathsPerBlock));
     const u4_t* w = reinterpret_cast<const u4_t*>(wfrag);
-    if (zs)
-        ref_paths_mfma_kernel<true><<<grid, kMfmaBlock, 2 * kMfmaFragElems, st>>>(g, w, nx, ny, zs);
-    else
-        ref_paths_mfma_kernel<false><<<grid, kMfmaBlock, 2 * kMfmaFragElems, st>>>(g, w, nx, ny, zs);
+    const int mt = ref_mfma_tiles(g.ndates);
+    const size_t lds = (size_t)ref_mfma_lds_bytes(mt);
+#define HJM_REF_LAUNCH(MT)                                                                      \
+    (zs ? ref_paths_mfma_kernel<true, MT><<<grid, kMfmaBlock, lds, st>>>(g, w, nx, ny, zs)      \
+        : ref_paths_mfma_kernel<false, MT><<<grid, kMfmaBlock, lds, st>>>(g, w, nx, ny, zs))
+    switch (mt) {
+    case 4: HJM_REF_LAUNCH(4); break;
+    case 5: HJM_REF_LAUNCH(5); break;
+    case 6: HJM_REF_LAUNCH(6); break;
+    default: HJM_REF_LAUNCH(7); break;
+    }
+#undef HJM_REF_LAUNCH
     return hipGetLastError();
 }
 

@@ -46,6 +46,9 @@ constexpr int kMfmaParts = 3;     // bf16 parts of W
 // bf16 elements of one kernel's fragments
 constexpr int kMfmaFragElems = kMfmaMTiles * kMfmaChunks * kMfmaParts * kWave * 8;
 constexpr int kMfmaPfFragElems = kMfmaPfMTiles * kMfmaChunks * kMfmaParts * kWave * 8;
+// LDS bytes of the first mt M-tiles' fragments (the reference kernel stages only its live
+// tiles)
+constexpr int ref_mfma_lds_bytes(int mt) { return mt * kMfmaChunks * kMfmaParts * kWave * 8 * 2; }
 
 enum class MfmaMap { kReference, kPortfolio };
 

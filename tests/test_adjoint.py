@@ -157,6 +157,27 @@ def test_gpu_fp32_forward_greeks_close_to_fp64():
 
 
 @pytest.mark.gpu
+def test_gpu_fp32_greeks_match_oracle_adjoint(oracle):
+    """The default FP32 Greeks (matrix-core forward with in-kernel normals, one-pass sweep
+    reading the stored shocks) against the oracle's FP64 adjoint on the same normals
+    (hjm_generate_normals gives the in-kernel generator's output): CVA within 1e-5, every
+    Greek within 1e-3 of its family's largest."""
+    _flush("[gpu] FP32 adjoint vs oracle adjoint, 4096 paths 10y, in-kernel normals")
+    m = hjm.benchmark_market()
+    P, seed = 4096, 13
+    z = hjm.generate_normals(seed, P, 1000)
+    w = oracle.cva_weights(T, 0.5, 0.02, 0.4)
+    cva, ds, dv, dm = oracle.reference_cva_greeks(m.spot_rates, m.drifts, m.volatilities,
+                                                  m.accrual, 1e6, 0.025, 10.0, 0.5, 0.01, z, w)
+    g = hjm.cva_greeks(m, hjm.InterestRateSwap(1e6, 0.025, 10.0), P, seed=seed,
+                       precision=hjm.FP32_KAHAN)
+    assert g.stored_normals_bytes == P * 1000 * 3 * 4
+    assert abs(g.cva / cva - 1) < 1e-5
+    for got, ref in ((g.d_spot, ds), (g.d_vol, dv), (g.d_drift, dm)):
+        assert np.max(np.abs(got - ref)) <= 1e-3 * np.max(np.abs(ref))
+
+
+@pytest.mark.gpu
 def test_gpu_stored_and_regenerated_shocks_give_the_same_greeks():
     """After the matrix-core FP32 forward the sweep reads the forward's stored shocks; asked
     to, it regenerates them from the generator instead: the same Greeks."""
