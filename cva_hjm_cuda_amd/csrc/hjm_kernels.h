@@ -82,9 +82,10 @@ hipError_t launch_pf_value_mfma(const PfValueParams& v, const float* curves, con
                                 float* ex, const MfmaShape& shape, hipStream_t st, int64_t count = -1,
                                 const SelRowState* filter = nullptr, int64_t fk = 0,
                                 int64_t fcap = 0);
-// FP32 plans with at most 128 sets: valuation, EE and per-path CVA moments (partials
+// FP32 plans: valuation, EE and per-path CVA moments (partials
 // [set][pf_fused_blocks(paths)][ndates + 2], for the grouped column sums) and, with sel,
 // the PFE bracket counting and candidates of every row, in one pass without exposures
+// (groups of 128 sets)
 int64_t pf_fused_blocks(int64_t paths);
 hipError_t launch_pf_value_stats_mfma(const PfValueParams& v, const float* curves, const double* logb,
                                       const uint16_t* gfrag, const float* apad, const int* tilek,

@@ -747,73 +747,51 @@ hipError_t launch_pf_value_mfma(const PfValueParams& v, const float* curves, con
 }
 
 // ---- fused valuation and statistics (FP32 plans) -------------------------------------
-// One workgroup of 4 waves values its 128 paths at every date in turn for one group of up
-// to 64 netting sets (blockIdx.y; each wave 2 N-tiles of 16 paths, the date's -gamma
-// fragments of the group's 4 M-tiles staged in LDS) and reduces the exposures where they
-// are produced, so they never go to HBM:
-//   * EE numerators: per date and set the workgroup's sum (FP32 sums of a wave's 32 paths
-//     by row reductions of the accumulator layout, then the waves in order in FP64),
-//     partials[set][block][date];
+// The exposure pass's values with paths as the matrix rows: V^T = P^T (-Gamma)^T + a, the
+// bond prices of a wave's path tile as the A operand (the registers value_bonds builds,
+// whose A and B layouts coincide) and -Gamma's fragments as the B operand, the split
+// products in the exposure pass's order. Lane l then holds paths 4 (l / 16) + r, r = 0..3,
+// of set l % 16 of a set tile, so a set's sums over paths are in-lane sums and two
+// cross-row shuffles. A workgroup of 4 waves x 2 path tiles (128 paths) values a group of
+// up to 128 sets (blockIdx.y; one group up to 128 sets, so the bond prices are computed
+// once) at every date in turn and reduces where the values are produced, so the
+// exposures never go to HBM:
+//   * EE numerators: per date and set the workgroup's sum (FP32 sums of a wave's 32
+//     paths, then the waves in order in FP64), partials[set][block][date];
 //   * the per-path CVA sample sum_e w_ge E_pge in registers (FP32) across the dates, its
 //     sum and square sum per set at the end, partials[set][block][ndates + 0, 1];
 //   * with sel: the counting pass of the PFE's sampled-bracket selection for every row
 //     (date, set) -- entries below the bracket (or the zeros), entries inside it counted
 //     and staged in LDS (kFuseSlots per set and date; more flags the row overflowed, so
-//     its selection revalues the whole row), then one claim per row and workgroup and the
+//     its selection values it again, whole), then one claim per row and workgroup and the
 //     copy of the staged entries to the row's candidate buffer.
-// The per-path CVA of 64 sets x 128 paths takes 32 VGPRs per lane; a group of 128 would
-// not fit beside the valuation, so set groups re-derive their paths' bond prices; at
-// 2 waves per SIMD two workgroups share a CU, one computing while the other reduces. The
-// exposures are those of pf_value_mfma_kernel bit for bit (same helpers).
-constexpr int kFuseWaves = 4;
+// The date's fragments and the per-set values it needs are staged in LDS; the per-set
+// arrays are double-buffered by date, so the flush of date e - 1 (EE partials, counts,
+// claims, candidates; one thread per set) runs beside the staging of date e: two
+// barriers per date.
+constexpr int kFuseWaves = 8;
 constexpr int kFuseBlock = kFuseWaves * kWave;
-constexpr int kFuseNT = 2;
-constexpr int kFusePaths = kFuseWaves * kFuseNT * 16;   // 128
-constexpr int kFuseMTiles = 4;                          // M-tiles per set group
-constexpr int kFuseSets = 16 * kFuseMTiles;             // 64 sets per group
-constexpr int kFuseSlots = 32;                          // staged inside entries per set and date
+constexpr int kFusePT = 1;                              // path tiles (16 paths) per wave
+constexpr int kFusePaths = kFuseWaves * kFusePT * 16;   // 128
+constexpr int kFuseTiles = 8;                           // set tiles per group
+constexpr int kFuseSets = 16 * kFuseTiles;              // 128 sets per group
+constexpr int kFuseSlots = 16;                          // staged inside entries per set and date
 
-// v + (v of the lane moved by the DPP control), within rows of 16 lanes (FP64 and u32)
-template <int kCtrl>
-__device__ __forceinline__ double dpp_row_add_f64(double v)
+struct FuseDate {
+    float w[kFuseSets];       // CVA weight of the set at the date (0 past the last set)
+    float a[kFuseSets];       // the set's constant a
+    uint32_t lo[kFuseSets], hi[kFuseSets];   // PFE bracket (FP32 bits)
+    uint32_t nlo[kFuseSets], nin[kFuseSets]; // entries below / inside it
+    float ee[kFuseWaves][kFuseSets];         // per-wave EE sums
+    float cand[kFuseSets][kFuseSlots];       // staged inside entries
+    int tk[kFuseTiles];                      // the set tile's largest live period
+};
+
+template <typename T>
+__device__ __forceinline__ T cross_row_sum(T x)
 {
-    const uint64_t b = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, kCtrl, 0xf, 0xf, false);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), kCtrl, 0xf, 0xf, false);
-    return v + __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-template <int kCtrl>
-__device__ __forceinline__ uint32_t dpp_row_add_u32(uint32_t v)
-{
-    return v + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, 0xf, 0xf, false);
-}
-// sums over each row of 16 lanes, in lane 15 of the row
-__device__ __forceinline__ double row_sum_f64(double v)
-{
-    v = dpp_row_add_f64<0xb1>(v);   // quad_perm [1, 0, 3, 2]
-    v = dpp_row_add_f64<0x4e>(v);   // quad_perm [2, 3, 0, 1]
-    v = dpp_row_add_f64<0x114>(v);  // row_shr:4
-    return dpp_row_add_f64<0x118>(v);  // row_shr:8
-}
-template <int kCtrl>
-__device__ __forceinline__ float dpp_row_add_f32(float v)
-{
-    return v + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), kCtrl,
-                                                                      0xf, 0xf, false));
-}
-__device__ __forceinline__ float row_sum_f32(float v)
-{
-    v = dpp_row_add_f32<0xb1>(v);
-    v = dpp_row_add_f32<0x4e>(v);
-    v = dpp_row_add_f32<0x114>(v);
-    return dpp_row_add_f32<0x118>(v);
-}
-__device__ __forceinline__ uint32_t row_sum_u32(uint32_t v)
-{
-    v = dpp_row_add_u32<0xb1>(v);
-    v = dpp_row_add_u32<0x4e>(v);
-    v = dpp_row_add_u32<0x114>(v);
-    return dpp_row_add_u32<0x118>(v);
+    x += __shfl_xor(x, 16, kWave);
+    return x + __shfl_xor(x, 32, kWave);
 }
 
 __global__ __launch_bounds__(kFuseBlock) void pf_value_stats_mfma_kernel(
@@ -822,155 +800,181 @@ __global__ __launch_bounds__(kFuseBlock) void pf_value_stats_mfma_kernel(
     const int* __restrict__ tilek, const double* __restrict__ w, double* __restrict__ partials,
     SelRowState* __restrict__ sel, float* __restrict__ cand, int64_t cap, int slots)
 {
-    // the date's fragments of the group's M-tiles; at the end, the per-wave CVA moments
-    __shared__ u4_t sfrag[kFuseMTiles * kValueFragsPerTile * kWave];
-    __shared__ double s_ee[kFuseWaves][kFuseSets];
-    __shared__ float s_w[kFuseSets];
-    __shared__ uint32_t s_lo[kFuseSets], s_hi[kFuseSets];
-    __shared__ uint32_t s_cnt[kFuseSets], s_nin[kFuseSets], s_base[kFuseSets];
-    __shared__ float s_cand[kFuseSets][kFuseSlots];
+    __shared__ u4_t sfrag[kFuseTiles * kValueFragsPerTile * kWave];
+    __shared__ FuseDate sd[2];
     const int64_t P = v.paths;
     const int G = v.nsets, nd = v.ndates, ncol = nd + 2;
-    const int m0 = blockIdx.y * kFuseMTiles, g0 = 16 * m0;   // the group's first M-tile, set
-    const int mt = mtiles - m0 < kFuseMTiles ? mtiles - m0 : kFuseMTiles;
-    const int ng = G - g0 < kFuseSets ? G - g0 : kFuseSets;   // sets of the group
+    const int m0 = blockIdx.y * kFuseTiles, g0 = 16 * m0;   // the group's first set tile, set
+    const int nt = mtiles - m0 < kFuseTiles ? mtiles - m0 : kFuseTiles;
+    const int ng = G - g0 < kFuseSets ? G - g0 : kFuseSets;
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
     const int col = lane % 16, grp = lane / 16;
-    const int tiles = (int)gridDim.x;
-    const int64_t pbase = (int64_t)blockIdx.x * kFusePaths + wave * (kFuseNT * 16);
-    bool live[kFuseNT];
+    const int blocks = (int)gridDim.x;
+    const int64_t pbase = (int64_t)blockIdx.x * kFusePaths + wave * (kFusePT * 16);
+    // the lane's accumulator rows: paths pbase + 16 t + 4 grp + r
+    bool live[kFusePT][4];
 #pragma unroll
-    for (int n = 0; n < kFuseNT; ++n) live[n] = pbase + 16 * n + col < P;
-    float cvap[kFuseMTiles][kFuseNT][4];
+    for (int t = 0; t < kFusePT; ++t)
 #pragma unroll
-    for (int m = 0; m < kFuseMTiles; ++m)
+        for (int r = 0; r < 4; ++r) live[t][r] = pbase + 16 * t + 4 * grp + r < P;
+    float cvap[kFuseTiles][kFusePT][4];
 #pragma unroll
-        for (int n = 0; n < kFuseNT; ++n)
+    for (int n = 0; n < kFuseTiles; ++n)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) cvap[m][n][r] = 0.f;
+        for (int t = 0; t < kFusePT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) cvap[n][t][r] = 0.f;
 
-    for (int e = 0; e < nd; ++e) {
-        __syncthreads();  // the previous date's LDS is no longer read
+    auto stage = [&](int e, FuseDate& d) {
         const u4_t* src = gfrag + ((int64_t)e * mtiles + m0) * kValueFragsPerTile * kWave;
-        for (int t = threadIdx.x; t < mt * kValueFragsPerTile * kWave; t += kFuseBlock) sfrag[t] = src[t];
+        for (int t = threadIdx.x; t < nt * kValueFragsPerTile * kWave; t += kFuseBlock) sfrag[t] = src[t];
         for (int gl = threadIdx.x; gl < kFuseSets; gl += kFuseBlock) {
             const bool in = gl < ng;
             const int64_t rid = (int64_t)e * G + g0 + gl;
-            s_w[gl] = in ? (float)w[(int64_t)(g0 + gl) * nd + e] : 0.f;
-            s_lo[gl] = (in && sel) ? (uint32_t)sel[rid].lo : 0u;
-            s_hi[gl] = (in && sel) ? (uint32_t)sel[rid].hi : 0u;
-            s_cnt[gl] = 0u;
-            s_nin[gl] = 0u;
+            d.w[gl] = in ? (float)w[(int64_t)(g0 + gl) * nd + e] : 0.f;
+            d.a[gl] = in ? apad[((int64_t)e * mtiles + m0 + gl / 16) * 16 + gl % 16] : 0.f;
+            d.lo[gl] = (in && sel) ? (uint32_t)sel[rid].lo : 0u;
+            d.hi[gl] = (in && sel) ? (uint32_t)sel[rid].hi : 0u;
+            d.nlo[gl] = 0u;
+            d.nin[gl] = 0u;
         }
+        if (threadIdx.x < kFuseTiles)
+            d.tk[threadIdx.x] = (int)threadIdx.x < nt ? tilek[(int64_t)e * mtiles + m0 + threadIdx.x] : 0;
+    };
+    // one thread per set: EE partial, counts, claim, candidates of date e
+    auto flush = [&](int e, const FuseDate& d) {
+        const int gl = threadIdx.x;
+        if (gl >= ng) return;
+        double t = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < kFuseWaves; ++ww) t += (double)d.ee[ww][gl];
+        partials[((int64_t)(g0 + gl) * blocks + blockIdx.x) * ncol + e] = t;
+        if (!sel) return;
+        const int64_t rid = (int64_t)e * G + g0 + gl;
+        const uint32_t lo = d.lo[gl], hi = d.hi[gl], nlo = d.nlo[gl], nin = d.nin[gl];
+        const uint32_t lo1 = lo ? lo : 1u;
+        const bool collect = hi >= lo1 && lo != hi;
+        if (nlo) atomicAdd(lo ? &sel[rid].below : &sel[rid].zin, nlo);
+        if (nin) atomicAdd(&sel[rid].nin, nin);
+        if (collect && nin > (uint32_t)slots) {
+            atomicOr(&sel[rid].claim, kSelOverflow);
+        } else if (collect && nin) {
+            const uint32_t base = atomicAdd(&sel[rid].claim, nin);
+            for (uint32_t i = 0; i < nin; ++i) {
+                const int64_t slot = (int64_t)base + i;
+                if (slot < cap) cand[rid * cap + slot] = d.cand[gl][i];
+            }
+        }
+    };
+
+    for (int e = 0; e < nd; ++e) {
+        FuseDate& d = sd[e & 1];
+        stage(e, d);
+        if (e > 0) flush(e - 1, sd[(e - 1) & 1]);
         __syncthreads();
-        float disc[kFuseNT];
-        {
-            // lane groups 0 and 1 load the discount factors of N-tiles 0 and 1
-            const int64_t p = pbase + 16 * (grp & 1) + col;
-            const float d = (float)exp(-logb[(int64_t)e * P + (p < P ? p : 0)]);
+        // discount factors of the lane's rows: lane col loads path 16 t + col's, the lane
+        // of rows 4 grp + r takes it from lane 4 grp + r
+        float d4[kFusePT][4];
 #pragma unroll
-            for (int n = 0; n < kFuseNT; ++n) disc[n] = __shfl(d, col + 16 * n, kWave);
+        for (int t = 0; t < kFusePT; ++t) {
+            const int64_t p = pbase + 16 * t + col;
+            const float df = (float)exp(-logb[(int64_t)e * P + (p < P ? p : 0)]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) d4[t][r] = __shfl(df, 4 * grp + r, kWave);
         }
-        Split8 bs[kFuseNT][2];
+        Split8 bs[kFusePT][2];
 #pragma unroll
-        for (int n = 0; n < kFuseNT; ++n) {
-            const int64_t p = pbase + 16 * n + col;
-            value_bonds(v, e, p < P ? p : 0, curves, grp, col, bs[n]);
+        for (int t = 0; t < kFusePT; ++t) {
+            const int64_t p = pbase + 16 * t + col;
+            value_bonds(v, e, p < P ? p : 0, curves, grp, col, bs[t]);
         }
 #pragma unroll
-        for (int m = 0; m < kFuseMTiles; ++m) {
-            if (m >= mt) break;
-            const int tk = tilek[(int64_t)e * mtiles + m0 + m];
-            const f4_t a4 = *reinterpret_cast<const f4_t*>(apad + ((int64_t)e * mtiles + m0 + m) * 16 + 4 * grp);
-            f4_t acc[kFuseNT];
-            value_tile<kFuseNT>(sfrag + (m * kValueFragsPerTile) * kWave, lane, tk, a4, bs, acc);
+        for (int n = 0; n < kFuseTiles; ++n) {
+            if (n >= nt) break;
+            const int gl = 16 * n + col;   // the lane's set g0 + gl (weight 0 past the last)
+            const float a = d.a[gl];
+            f4_t acc[kFusePT];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gl = 16 * m + 4 * grp + r;   // set g0 + gl; weight 0 past the last
-                const float wg = s_w[gl];
-                const uint32_t lo = s_lo[gl], hi = s_hi[gl];
-                const uint32_t lo1 = lo ? lo : 1u;     // inside: non-zero and >= lo
-                const bool any_in = sel && gl < ng && hi >= lo1;
-                const bool collect = any_in && lo != hi;
-                float es = 0.f;
-                uint32_t nlo = 0;
+            for (int t = 0; t < kFusePT; ++t) acc[t] = f4_t{a, a, a, a};
+            const int tk = d.tk[n];
+            // opaque per tile: the tiles' fragment reads stay in their tile instead of being
+            // hoisted together (12 registers per chunk and tile)
+            uint32_t off = (uint32_t)lane;
+            asm volatile("" : "+v"(off));
 #pragma unroll
-                for (int n = 0; n < kFuseNT; ++n) {
-                    const float E = live[n] ? exposure_of(acc[n][r], disc[n]) : 0.f;
-                    cvap[m][n][r] = __builtin_fmaf(wg, E, cvap[m][n][r]);
-                    es += E;
-                    const uint32_t b = __builtin_bit_cast(uint32_t, E);
-                    nlo += (live[n] && b < lo1) ? 1u : 0u;
-                    if (any_in && live[n] && b - lo1 <= hi - lo1) {
-                        const uint32_t pos = atomicAdd(&s_nin[gl], 1u);
-                        if (collect && pos < (uint32_t)slots) s_cand[gl][pos] = E;
+            for (int c = 0; c < 2; ++c) {
+                if (tk > 32 * c) {
+                    const u4_t* fr = sfrag + ((n * 2 + c) * kMfmaParts) * kWave + off;
+                    const u4_t gh = fr[0], gm = fr[kWave], glo = fr[2 * kWave];
+#pragma unroll
+                    for (int t = 0; t < kFusePT; ++t) {
+                        f4_t x = acc[t];
+                        x = mfma_bf16(bs[t][c].h, gh, x);
+                        x = mfma_bf16(bs[t][c].m, gh, x);
+                        x = mfma_bf16(bs[t][c].h, gm, x);
+                        x = mfma_bf16(bs[t][c].l, gh, x);
+                        x = mfma_bf16(bs[t][c].h, glo, x);
+                        x = mfma_bf16(bs[t][c].m, gm, x);
+                        acc[t] = x;
                     }
                 }
-                es = row_sum_f32(es);
-                if (col == 15) s_ee[wave][gl] = (double)es;
-                if (sel) {
-                    nlo = row_sum_u32(nlo);
-                    if (col == 15 && nlo) atomicAdd(&s_cnt[gl], nlo);
-                }
             }
-        }
-        __syncthreads();
-        for (int gl = threadIdx.x; gl < ng; gl += kFuseBlock) {
-            const int g = g0 + gl;
-            double t = 0.0;
+            const float wg = d.w[gl];
+            const uint32_t lo = d.lo[gl], hi = d.hi[gl];
+            const uint32_t lo1 = lo ? lo : 1u;     // inside: non-zero and >= lo
+            const bool any_in = sel && gl < ng && hi >= lo1;
+            const bool collect = any_in && lo != hi;
+            float es = 0.f;
+            uint32_t nlo = 0;
 #pragma unroll
-            for (int ww = 0; ww < kFuseWaves; ++ww) t += s_ee[ww][gl];
-            partials[((int64_t)g * tiles + blockIdx.x) * ncol + e] = t;
+            for (int t = 0; t < kFusePT; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float E = live[t][r] ? exposure_of(acc[t][r], d4[t][r]) : 0.f;
+                    cvap[n][t][r] = __builtin_fmaf(wg, E, cvap[n][t][r]);
+                    es += E;
+                    const uint32_t b = __builtin_bit_cast(uint32_t, E);
+                    nlo += (live[t][r] && b < lo1) ? 1u : 0u;
+                    if (any_in && live[t][r] && b - lo1 <= hi - lo1) {
+                        const uint32_t pos = atomicAdd(&d.nin[gl], 1u);
+                        if (collect && pos < (uint32_t)slots) d.cand[gl][pos] = E;
+                    }
+                }
+            es = cross_row_sum(es);
+            if (grp == 0 && gl < ng) d.ee[wave][gl] = es;
             if (sel) {
-                const int64_t rid = (int64_t)e * G + g;
-                const uint32_t lo = s_lo[gl], hi = s_hi[gl], nlo = s_cnt[gl], nin = s_nin[gl];
-                const uint32_t lo1 = lo ? lo : 1u;
-                const bool collect = hi >= lo1 && lo != hi;
-                if (nlo) atomicAdd(lo ? &sel[rid].below : &sel[rid].zin, nlo);
-                if (nin) atomicAdd(&sel[rid].nin, nin);
-                uint32_t base = 0;
-                if (collect && nin > (uint32_t)slots) atomicOr(&sel[rid].claim, kSelOverflow);
-                else if (collect && nin) base = atomicAdd(&sel[rid].claim, nin);
-                s_base[gl] = base;
+                nlo = cross_row_sum(nlo);
+                if (grp == 0 && nlo) atomicAdd(&d.nlo[gl], nlo);
             }
+            // each set tile is its own scheduling region: interleaving the tiles would keep
+            // their operands and accumulators live together
+            __builtin_amdgcn_sched_barrier(0);
         }
         __syncthreads();
-        if (sel) {
-            for (int t = threadIdx.x; t < ng * kFuseSlots; t += kFuseBlock) {
-                const int gl = t / kFuseSlots, i = t % kFuseSlots;
-                const uint32_t lo = s_lo[gl], hi = s_hi[gl], nin = s_nin[gl];
-                const uint32_t lo1 = lo ? lo : 1u;
-                const bool collect = hi >= lo1 && lo != hi;
-                if (collect && nin <= (uint32_t)slots && (uint32_t)i < nin) {
-                    const int64_t slot = (int64_t)s_base[gl] + i;
-                    if (slot < cap) cand[((int64_t)e * G + g0 + gl) * cap + slot] = s_cand[gl][i];
-                }
-            }
-        }
     }
-    // per-path CVA moments per set: row sums, then the waves in order
-    __syncthreads();
+    flush(nd - 1, sd[(nd - 1) & 1]);
+    // per-path CVA moments per set: in-lane and cross-row sums, then the waves in order
     double* s_cva = reinterpret_cast<double*>(sfrag);   // [wave][set][2]
 #pragma unroll
-    for (int m = 0; m < kFuseMTiles; ++m) {
-        if (m >= mt) break;
+    for (int n = 0; n < kFuseTiles; ++n) {
+        if (n >= nt) break;
+        double c1 = 0.0, c2 = 0.0;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            double c1 = 0.0, c2 = 0.0;
+        for (int t = 0; t < kFusePT; ++t)
 #pragma unroll
-            for (int n = 0; n < kFuseNT; ++n) {
-                const double c = (double)cvap[m][n][r];
+            for (int r = 0; r < 4; ++r) {
+                const double c = (double)cvap[n][t][r];
                 c1 += c;
                 c2 += c * c;
             }
-            c1 = row_sum_f64(c1);
-            c2 = row_sum_f64(c2);
-            const int gl = 16 * m + 4 * grp + r;
-            if (col == 15) {
-                s_cva[(wave * kFuseSets + gl) * 2] = c1;
-                s_cva[(wave * kFuseSets + gl) * 2 + 1] = c2;
-            }
+        c1 = cross_row_sum(c1);
+        c2 = cross_row_sum(c2);
+        const int gl = 16 * n + col;
+        if (grp == 0) {
+            s_cva[(wave * kFuseSets + gl) * 2] = c1;
+            s_cva[(wave * kFuseSets + gl) * 2 + 1] = c2;
         }
+        __builtin_amdgcn_sched_barrier(0);   // one tile's FP64 values live at a time
     }
     __syncthreads();
     for (int t = threadIdx.x; t < 2 * ng; t += kFuseBlock) {
@@ -978,7 +982,7 @@ __global__ __launch_bounds__(kFuseBlock) void pf_value_stats_mfma_kernel(
         double s = 0.0;
 #pragma unroll
         for (int ww = 0; ww < kFuseWaves; ++ww) s += s_cva[(ww * kFuseSets + gl) * 2 + q];
-        partials[((int64_t)(g0 + gl) * tiles + blockIdx.x) * ncol + nd + q] = s;
+        partials[((int64_t)(g0 + gl) * blocks + blockIdx.x) * ncol + nd + q] = s;
     }
 }
 
@@ -992,7 +996,7 @@ hipError_t launch_pf_value_stats_mfma(const PfValueParams& v, const float* curve
     const int slots = stage_slots > 0 && stage_slots < kFuseSlots ? stage_slots : kFuseSlots;
     const int mtiles = value_mtiles(v.nsets);
     if (v.paths <= 0 || v.nsets <= 0 || mtiles > kMaxPfMTiles) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)pf_fused_blocks(v.paths), (unsigned)((mtiles + kFuseMTiles - 1) / kFuseMTiles));
+    const dim3 grid((unsigned)pf_fused_blocks(v.paths), (unsigned)((mtiles + kFuseTiles - 1) / kFuseTiles));
     pf_value_stats_mfma_kernel<<<grid, kFuseBlock, 0, st>>>(v, mtiles, curves, logb,
                                                             reinterpret_cast<const u4_t*>(gfrag),
                                                             apad, tilek, w, partials, sel, cand, cap,
