@@ -120,7 +120,13 @@ __device__ __forceinline__ void for_each_index(F& f, std::integer_sequence<int, 
 
 // One 50-step block: acc = W x [fin; shocks of steps b*50 .. b*50+49; 1]. kStore: also
 // write the block's normals to zs[step][factor][path] (FP32) for a later reverse sweep.
-template <int MT, int NT, bool kStore, bool kInterleave>
+// kTwoAcc: the five small split products (Wh xm, Wm xh, Wh xl, Wl xh, Wm xm) accumulate
+// apart from Wh xh and join it once at the end. The matrix cores drop the low bits of
+// products far below the accumulator (measured: C = 1 plus 32 products of 2^-27 returns 1;
+// small positive contributions are lost, small negative ones kept), which biases a
+// single accumulator holding the curve (FP32 vs FP64: EE 1.4e-6 -> 3.2e-7 on the 10y
+// precision workload with two). It costs 4 registers per tile and N-tile.
+template <int MT, int NT, bool kStore, bool kInterleave, bool kTwoAcc>
 __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const float (&fin)[NT][16],
                                            f4_t (&acc)[MT][NT], const MfmaGrid& g,
                                            const int64_t (&path)[NT], const bool (&live)[NT],
@@ -134,6 +140,12 @@ __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const
     // inside the block instead of hoisted (hundreds of registers) out of the block loop
     uint32_t off = (uint32_t)lane;
     asm volatile("" : "+v"(off));
+    constexpr int MT2 = kTwoAcc ? MT : 1;
+    f4_t acc2[MT2][NT];
+#pragma unroll
+    for (int m = 0; m < MT2; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc2[m][n] = f4_t{0.f, 0.f, 0.f, 0.f};
     auto tiles = [&](int c, int m0, int m1, const Split8 (&xs)[NT]) {
 #pragma unroll
         for (int m = m0; m < m1; ++m) {
@@ -141,14 +153,20 @@ __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const
             const u4_t ah = a[0], am = a[kWave], al = a[2 * kWave];
 #pragma unroll
             for (int n = 0; n < NT; ++n) {
-                f4_t v = acc[m][n];
-                v = mfma_bf16(ah, xs[n].h, v);
+                f4_t v;
+                if constexpr (kTwoAcc) {
+                    acc[m][n] = mfma_bf16(ah, xs[n].h, acc[m][n]);
+                    v = acc2[m][n];
+                } else {
+                    v = mfma_bf16(ah, xs[n].h, acc[m][n]);
+                }
                 v = mfma_bf16(ah, xs[n].m, v);
                 v = mfma_bf16(am, xs[n].h, v);
                 v = mfma_bf16(ah, xs[n].l, v);
                 v = mfma_bf16(al, xs[n].h, v);
                 v = mfma_bf16(am, xs[n].m, v);
-                acc[m][n] = v;
+                if constexpr (kTwoAcc) acc2[m][n] = v;
+                else acc[m][n] = v;
             }
         }
     };
@@ -224,6 +242,12 @@ __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const
     };
     for_each_index(quad, std::make_integer_sequence<int, kMfmaQuads>{});
     tiles(kMfmaChunks - 1, 0, MT, xs[kMfmaChunks - 1]);
+    if constexpr (kTwoAcc) {
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int n = 0; n < NT; ++n) acc[m][n] += acc2[m][n];
+    }
 }
 
 // Reference semantics. MT = the live M-tiles of the block map: the curve (tiles 0..3,
@@ -289,7 +313,8 @@ __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGr
 
     for (int b = 1; b < g.ndates; ++b) {
         f4_t acc[MT][kNT];
-        mfma_block<MT, kNT, kStore, true>(lds_w, fin, acc, g, path, live, b, grp, lane, zs);
+        // two accumulators where registers allow (up to 5 live tiles: 239 VGPRs)
+        mfma_block<MT, kNT, kStore, true, (MT <= 5)>(lds_w, fin, acc, g, path, live, b, grp, lane, zs);
         // integrals: Kahan-add the block's dA (tiles 3 .. MT-1); curve rows become next input
 #pragma unroll
         for (int n = 0; n < kNT; ++n) {
@@ -409,7 +434,8 @@ __global__ __launch_bounds__(WAVES * kWave) void pf_paths_mfma_kernel(const Mfma
             }
             if (e + 1 >= g.ndates) break;
             f4_t acc[kMfmaPfMTiles][NT];
-            mfma_block<kMfmaPfMTiles, NT, kStore, false>(lds_w, fin, acc, g, path, live, e, grp, lane, zs);
+            mfma_block<kMfmaPfMTiles, NT, kStore, false, true>(lds_w, fin, acc, g, path, live, e, grp, lane,
+                                                               zs);
 #pragma unroll
             for (int n = 0; n < NT; ++n) {
 #pragma unroll
