@@ -578,6 +578,7 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
         hjm_portfolio_plan p;
         float *gbar = nullptr, *beta = nullptr, *zs = nullptr;
         double *part = nullptr, *sums = nullptr, *bp = nullptr;
+        uint16_t* tfrag = nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         ~Cleanup()
         {
@@ -587,6 +588,7 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
             dev_free(part);
             dev_free(sums);
             dev_free(bp);
+            dev_free(tfrag);
             if (e0) (void)hipEventDestroy(e0);
             if (e1) (void)hipEventDestroy(e1);
             hjm_portfolio_plan_destroy(p);
@@ -601,8 +603,22 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
     HJM_TRY(dev_alloc(&c.beta, (size_t)nd * P));
     HJM_TRY(dev_alloc(&c.part, (size_t)nb * kOut));
     HJM_TRY(dev_alloc(&c.sums, (size_t)kOut + kMaxDates));  // then the per-date sums of beta
-    const int64_t tiles = (P + kBlock - 1) / kBlock;
+    const bool f64 = p->precision == HJM_FP64;
+    // rows of the seeds' beta partials: blocks of 256 paths (FP64) or the matrix-core
+    // kernel's workgroups (FP32)
+    const int64_t tiles = f64 ? (P + kBlock - 1) / kBlock : pf_adjoint_seed_mfma_blocks(P);
     HJM_TRY(dev_alloc(&c.bp, (size_t)tiles * nd));
+    if (!f64) {
+        // -Gamma^T in the order the seed's second product reads it
+        std::vector<double> coef, a;
+        std::vector<int> kmax;
+        build_tables(portfolio, nd, coef, a, kmax);
+        std::vector<uint16_t> tf;
+        build_adjoint_fragments(coef, nd, p->nsets, kMaxPeriods, tf);
+        HJM_TRY(dev_alloc(&c.tfrag, tf.size()));
+        HJM_HIP(hipMemcpy(c.tfrag, tf.data(), tf.size() * sizeof(uint16_t), hipMemcpyHostToDevice),
+                HJM_ERR_DEVICE);
+    }
     HJM_HIP(hipEventCreate(&c.e0), HJM_ERR_DEVICE);
     HJM_HIP(hipEventCreate(&c.e1), HJM_ERR_DEVICE);
     // the matrix-core forward keeps its normals for the reverse sweep; without the memory
@@ -613,13 +629,12 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
     hipStream_t st = p->stream;
     HJM_TRY(pplan_enqueue(p, true, c.zs));
     HJM_HIP(hipEventRecord(c.e0, st), HJM_ERR_DEVICE);
-    const bool f64 = p->precision == HJM_FP64;
     hipError_t err = f64 ? launch_pf_adjoint_seed_f64(p->vp, (const double*)p->d_curves, p->d_logb,
                                                       (const double*)p->d_coef, (const double*)p->d_a,
                                                       p->d_kmax, p->d_w, c.gbar, c.beta, c.bp, st)
-                         : launch_pf_adjoint_seed_f32(p->vp, (const float*)p->d_curves, p->d_logb,
-                                                      (const float*)p->d_coef, (const float*)p->d_a,
-                                                      p->d_kmax, p->d_w, c.gbar, c.beta, c.bp, st);
+                         : launch_pf_adjoint_seed_mfma(p->vp, (const float*)p->d_curves, p->d_logb,
+                                                       p->d_vfrag, c.tfrag, p->d_apad, p->d_tilek,
+                                                       p->d_w, c.gbar, c.beta, c.bp, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
     const double cc = p->grid.dt / portfolio->dtau;
     // beta summed over the paths at every date (the drift gradient's bank-account terms)

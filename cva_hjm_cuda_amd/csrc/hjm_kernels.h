@@ -118,15 +118,18 @@ hipError_t launch_bracket_finish(const double* x, int rows, int64_t n, int64_t k
 hipError_t launch_bracket_finish_f32(const float* x, int rows, int64_t n, int64_t k, void* ws,
                                      double* out, hipStream_t st);
 
-// beta_partials: [blocks of 256 paths][ndates] sums of beta over the block's paths
+// FP64 plans; beta_partials: [blocks of 256 paths][ndates] sums of beta over the block's paths
 hipError_t launch_pf_adjoint_seed_f64(const PfValueParams& v, const double* curves,
                                       const double* logb, const double* coef, const double* a,
                                       const int* kmax, const double* w, float* gbar, float* beta,
                                       double* beta_partials, hipStream_t st);
-hipError_t launch_pf_adjoint_seed_f32(const PfValueParams& v, const float* curves,
-                                      const double* logb, const float* coef, const float* a,
-                                      const int* kmax, const double* w, float* gbar, float* beta,
-                                      double* beta_partials, hipStream_t st);
+// FP32 plans: both products on the matrix cores over build_value_fragments' and
+// build_adjoint_fragments' tables; beta_partials: [pf_adjoint_seed_mfma_blocks(paths)][ndates]
+int64_t pf_adjoint_seed_mfma_blocks(int64_t paths);
+hipError_t launch_pf_adjoint_seed_mfma(const PfValueParams& v, const float* curves, const double* logb,
+                                       const uint16_t* gfrag, const uint16_t* tfrag, const float* apad,
+                                       const int* tilek, const double* w, float* gbar, float* beta,
+                                       double* beta_partials, hipStream_t st);
 // One-pass sweep, two lanes per path; partials [pf_adjoint_blocks(paths)][5 * 51] as the
 // reference sweep's.
 int64_t pf_adjoint_blocks(int64_t paths);

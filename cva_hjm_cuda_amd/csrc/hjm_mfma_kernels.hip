@@ -570,7 +570,7 @@ hipError_t launch_pf_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, float*
 // shuffled to the others by the caller).
 __device__ __forceinline__ void value_bonds(const PfValueParams& v, int e, int64_t p,
                                             const float* __restrict__ curves, int grp, int col,
-                                            Split8 (&bs)[2])
+                                            Split8 (&bs)[2], float* __restrict__ pk = nullptr)
 {
     const int64_t P = v.paths;
     const float hw = (float)(0.5 * v.dtau), dtf = (float)v.dtau;
@@ -621,9 +621,19 @@ __device__ __forceinline__ void value_bonds(const PfValueParams& v, int e, int64
     }
     bs[0] = split8(pa);
     bs[1] = split8(pb);
+    // pk (or null): the unsplit prices of the lane's periods 8 grp + i and 32 + 8 grp + i
+    if (pk) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            pk[8 * grp + i] = pa[i];
+            pk[32 + 8 * grp + i] = pb[i];
+        }
+    }
 }
 
 // One M-tile's values for NT N-tiles: acc[n] = a + (-gamma) P over the chunks the tile uses.
+// (A second accumulator for the small split products, as in the path kernels, changed the
+// per-set CVA error by 2 % and cost the valuation a wave per SIMD: not used here.)
 template <int NT>
 __device__ __forceinline__ void value_tile(const u4_t* __restrict__ fr0, int lane, int tk, f4_t a4,
                                            const Split8 (&bs)[NT][2], f4_t (&acc)[NT])
@@ -772,6 +782,212 @@ hipError_t launch_pf_value_mfma(const PfValueParams& v, const float* curves, con
     return hipGetLastError();
 }
 
+// the sum over the wave's four rows of 16 lanes, in every lane
+template <typename T>
+__device__ __forceinline__ T cross_row_sum(T x)
+{
+    x += __shfl_xor(x, 16, kWave);
+    return x + __shfl_xor(x, 32, kWave);
+}
+
+// ---- seeds of the portfolio adjoint (FP32 plans) ---------------------------------------
+// Per path and date, from the curve checkpoint (as pf_adjoint_seed_kernel):
+//   V = a - Gamma P (the exposure pass's product, sets as rows), u_g = w_g D 1[V_g > 0],
+//   beta = -sum_g w_g max(V_g, 0) D,  dP = -Gamma^T u,  iota_k = -P_k dP_k,
+//   gbar_l = dtau/2 (S_{l+1} [l <= 49] + S_l [l >= 1]) + [l = 50] dtau sum_{k>50} (k-50) iota_k,
+//   S_m = sum_{k >= m} iota_k.
+// Both products run on the matrix cores (bf16x6 split products). The second takes u
+// straight from the first's accumulators: its K chunk c holds sets 16 (2c + i/4) + 4 g +
+// i % 4 in lane group g's slot i, which are the registers i % 4 of set tiles 2c and 2c + 1
+// (build_adjoint_fragments orders -Gamma^T the same way), and its output rows are the
+// periods 16 pm + 4 g + r. A workgroup of 8 waves values 128 paths at one date
+// (blockIdx.y), then the next of its kSeedGroups groups; -Gamma and -Gamma^T of 128 sets
+// at a time are staged in LDS (once per workgroup up to 128 sets), the bond prices go
+// through LDS to the lanes that hold the matching dP, and each lane then forms the suffix
+// sums of its path and stores the tenors 13 g .. 13 g + 12 of gbar.
+constexpr int kSeedWaves = 8;
+constexpr int kSeedBlock = kSeedWaves * kWave;
+constexpr int kSeedPaths = kSeedWaves * 16;     // one N-tile per wave
+constexpr int kSeedTiles = 8;                   // set tiles staged at a time
+constexpr int kPeriodTiles = 4;                 // 64 period rows of the second product
+constexpr int kSeedGroups = 8;                  // path groups per workgroup (staging amortised)
+
+__global__ __launch_bounds__(kSeedBlock) void pf_adjoint_seed_mfma_kernel(
+    const PfValueParams v, int mtiles, const float* __restrict__ curves,
+    const double* __restrict__ logb, const u4_t* __restrict__ gfrag, const u4_t* __restrict__ tfrag,
+    const float* __restrict__ apad, const int* __restrict__ tilek, const double* __restrict__ w,
+    float* __restrict__ gbar, float* __restrict__ beta, double* __restrict__ beta_partials)
+{
+    __shared__ u4_t s1[kSeedTiles * kValueFragsPerTile * kWave];
+    __shared__ u4_t s2[kPeriodTiles * (kSeedTiles / 2) * kMfmaParts * kWave];
+    // per path: bond prices, then iota; rows of 65 floats so the 16 paths of a wave fall
+    // in distinct LDS banks
+    __shared__ float scr[kSeedWaves][16][65];
+    __shared__ float s_w[16 * kSeedTiles];
+    __shared__ int s_tk[kSeedTiles];
+    __shared__ double s_b[kSeedWaves];
+    const int e = blockIdx.y;
+    const int64_t P = v.paths;
+    const int G = v.nsets, nd = v.ndates;
+    const int nch = (mtiles + 1) / 2;
+    const bool one_group = mtiles <= kSeedTiles;   // fragments staged once per workgroup
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const int col = lane % 16, grp = lane / 16;
+    const float hw = (float)(0.5 * v.dtau), dtf = (float)v.dtau;
+    auto stage = [&](int m0, int nt) {
+        const int c0 = m0 / 2, ncg = (nt + 1) / 2;
+        const u4_t* src = gfrag + ((int64_t)e * mtiles + m0) * kValueFragsPerTile * kWave;
+        for (int t = threadIdx.x; t < nt * kValueFragsPerTile * kWave; t += kSeedBlock) s1[t] = src[t];
+        for (int t = threadIdx.x; t < kPeriodTiles * ncg * kMfmaParts * kWave; t += kSeedBlock) {
+            const int q = t / (ncg * kMfmaParts * kWave), rest = t % (ncg * kMfmaParts * kWave);
+            s2[(q * (kSeedTiles / 2)) * kMfmaParts * kWave + rest] =
+                tfrag[(((int64_t)e * kPeriodTiles + q) * nch + c0) * kMfmaParts * kWave + rest];
+        }
+        for (int gl = threadIdx.x; gl < 16 * kSeedTiles; gl += kSeedBlock) {
+            const int g = 16 * m0 + gl;
+            s_w[gl] = g < G ? (float)w[(int64_t)g * nd + e] : 0.f;
+        }
+        if (threadIdx.x < kSeedTiles)
+            s_tk[threadIdx.x] = (int)threadIdx.x < nt ? tilek[(int64_t)e * mtiles + m0 + threadIdx.x] : 0;
+    };
+    if (one_group) {
+        stage(0, mtiles);
+        __syncthreads();
+    }
+    double bsum = 0.0;   // the wave's sum of beta over its paths of all groups
+    for (int pg = 0; pg < kSeedGroups; ++pg) {
+        const int64_t p = ((int64_t)blockIdx.x * kSeedGroups + pg) * kSeedPaths + wave * 16 + col;
+        const bool live = p < P;
+        const int64_t pc = live ? p : 0;
+        const float D = (float)exp(-logb[(int64_t)e * P + pc]);
+        Split8 bs[1][2];
+        value_bonds(v, e, pc, curves, grp, col, bs[0], &scr[wave][col][0]);
+        f4_t dp[kPeriodTiles];
+#pragma unroll
+        for (int q = 0; q < kPeriodTiles; ++q) dp[q] = f4_t{0.f, 0.f, 0.f, 0.f};
+        float b = 0.f;
+        for (int m0 = 0; m0 < mtiles; m0 += kSeedTiles) {
+            const int nt = mtiles - m0 < kSeedTiles ? mtiles - m0 : kSeedTiles;
+            const int ncg = (nt + 1) / 2;
+            if (!one_group) {
+                __syncthreads();   // the previous group's fragments are no longer read
+                stage(m0, nt);
+                __syncthreads();
+            }
+            // V of the group's set tiles, u and beta where V is produced
+            float u[kSeedTiles][4];
+#pragma unroll
+            for (int m = 0; m < kSeedTiles; ++m) {
+                if (m < nt) {
+                    const f4_t a4 =
+                        *reinterpret_cast<const f4_t*>(apad + ((int64_t)e * mtiles + m0 + m) * 16 + 4 * grp);
+                    f4_t acc[1];
+                    value_tile<1>(s1 + (m * kValueFragsPerTile) * kWave, lane, s_tk[m], a4, bs, acc);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const float V = acc[0][r], wg = s_w[16 * m + 4 * grp + r];
+                        const bool pos = V > 0.f;
+                        u[m][r] = pos ? wg * D : 0.f;
+                        b -= pos ? wg * V * D : 0.f;
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) u[m][r] = 0.f;
+                }
+            }
+            // dP += (-Gamma^T) u over the group's K chunks
+#pragma unroll
+            for (int cc = 0; cc < kSeedTiles / 2; ++cc) {
+                if (cc < ncg) {
+                    float x[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) x[i] = u[2 * cc + i / 4][i % 4];
+                    const Split8 bu = split8(x);
+#pragma unroll
+                    for (int q = 0; q < kPeriodTiles; ++q) {
+                        const u4_t* fr = s2 + ((q * (kSeedTiles / 2) + cc) * kMfmaParts) * kWave + lane;
+                        const u4_t ah = fr[0], am = fr[kWave], al = fr[2 * kWave];
+                        f4_t y = dp[q];
+                        y = mfma_bf16(ah, bu.h, y);
+                        y = mfma_bf16(ah, bu.m, y);
+                        y = mfma_bf16(am, bu.h, y);
+                        y = mfma_bf16(ah, bu.l, y);
+                        y = mfma_bf16(al, bu.h, y);
+                        y = mfma_bf16(am, bu.m, y);
+                        dp[q] = y;
+                    }
+                }
+            }
+        }
+        // beta of the lane's path: its sets' terms summed over the four lane groups
+        b = cross_row_sum(b);
+        if (grp == 0 && live) beta[(int64_t)e * P + p] = b;
+        bsum += wave_sum((grp == 0 && live) ? (double)b : 0.0);
+        // iota at the lane's dP rows (periods 16 q + 4 grp + r), over the prices in LDS (the
+        // wave's own rows: its LDS accesses stay in program order)
+#pragma unroll
+        for (int q = 0; q < kPeriodTiles; ++q) {
+            float* row = &scr[wave][col][16 * q + 4 * grp];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) row[r] = -row[r] * dp[q][r];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // suffix sums S_k (k = 1 .. 60 is period index k - 1) and this lane's tenors of gbar
+        const float* io = &scr[wave][col][0];
+        float S[kTenors + 1];   // S[l] = S_l for l = 1 .. 51
+        float run = 0.f, ext = 0.f;
+#pragma unroll
+        for (int k = kMaxPeriods; k >= 1; --k) {
+            const float iota = io[k - 1];
+            run += iota;
+            if (k <= kTenors) S[k] = run;
+            if (k > kTenors - 1) ext += (float)(k - (kTenors - 1)) * iota;
+        }
+        if (live) {
+            float* gb = gbar + (int64_t)e * kTenors * P + p;
+#pragma unroll
+            for (int l = 0; l < kTenors; ++l) {
+                if (l / 13 != grp) continue;
+                float val = 0.f;
+                if (l <= kTenors - 2) val += hw * S[l + 1];
+                if (l >= 1) val += hw * S[l];
+                if (l == kTenors - 1) val += dtf * ext;
+                gb[(int64_t)l * P] = val;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();   // the next group's prices overwrite these rows
+    }
+    if (lane == 0) s_b[wave] = bsum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < kSeedWaves; ++ww) t += s_b[ww];
+        beta_partials[(int64_t)blockIdx.x * nd + e] = t;
+    }
+}
+
+int64_t pf_adjoint_seed_mfma_blocks(int64_t paths)
+{
+    return (paths + kSeedPaths * kSeedGroups - 1) / (kSeedPaths * kSeedGroups);
+}
+
+hipError_t launch_pf_adjoint_seed_mfma(const PfValueParams& v, const float* curves, const double* logb,
+                                       const uint16_t* gfrag, const uint16_t* tfrag, const float* apad,
+                                       const int* tilek, const double* w, float* gbar, float* beta,
+                                       double* beta_partials, hipStream_t st)
+{
+    const int mtiles = value_mtiles(v.nsets);
+    if (v.paths <= 0 || v.nsets <= 0 || v.ndates <= 0 || mtiles > kMaxPfMTiles) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)pf_adjoint_seed_mfma_blocks(v.paths), (unsigned)v.ndates);
+    pf_adjoint_seed_mfma_kernel<<<grid, kSeedBlock, 0, st>>>(
+        v, mtiles, curves, logb, reinterpret_cast<const u4_t*>(gfrag), reinterpret_cast<const u4_t*>(tfrag),
+        apad, tilek, w, gbar, beta, beta_partials);
+    return hipGetLastError();
+}
+
 // ---- fused valuation and statistics (FP32 plans) -------------------------------------
 // The exposure pass's values with paths as the matrix rows: V^T = P^T (-Gamma)^T + a, the
 // bond prices of a wave's path tile as the A operand (the registers value_bonds builds,
@@ -812,13 +1028,6 @@ struct FuseDate {
     float cand[kFuseSets][kFuseSlots];       // staged inside entries
     int tk[kFuseTiles];                      // the set tile's largest live period
 };
-
-template <typename T>
-__device__ __forceinline__ T cross_row_sum(T x)
-{
-    x += __shfl_xor(x, 16, kWave);
-    return x + __shfl_xor(x, 32, kWave);
-}
 
 __global__ __launch_bounds__(kFuseBlock) void pf_value_stats_mfma_kernel(
     const PfValueParams v, int mtiles, const float* __restrict__ curves,

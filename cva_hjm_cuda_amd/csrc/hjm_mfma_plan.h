@@ -251,4 +251,35 @@ inline void build_value_fragments(const std::vector<double>& coef, const std::ve
         }
 }
 
+// The portfolio adjoint's second product dP = (-Gamma)^T u on the matrix cores
+// (pf_adjoint_seed_mfma_kernel): per date, 4 period tiles x ceil(mtiles / 2) K chunks of 32
+// sets x 3 bf16 parts, [date][pm][c][part][lane][i] = part of -gamma[date][set][16 pm +
+// lane % 16] with set = 16 (2c + i/4) + 4 (lane/16) + i%4, the order in which the first
+// product's accumulators hold u (0 past the last set or period).
+inline void build_adjoint_fragments(const std::vector<double>& coef, int ndates, int nsets, int kpad,
+                                    std::vector<uint16_t>& frag)
+{
+    const int nch = (value_mtiles(nsets) + 1) / 2;
+    frag.assign((size_t)ndates * 4 * nch * kMfmaParts * kWave * 8, 0);
+    for (int e = 0; e < ndates; ++e)
+        for (int pm = 0; pm < 4; ++pm)
+            for (int c = 0; c < nch; ++c)
+                for (int l = 0; l < kWave; ++l)
+                    for (int i = 0; i < 8; ++i) {
+                        const int set = 16 * (2 * c + i / 4) + 4 * (l / 16) + i % 4;
+                        const int kk = 16 * pm + l % 16;
+                        const double w = (set < nsets && kk < kpad)
+                                             ? -coef[((size_t)e * nsets + set) * kpad + kk]
+                                             : 0.0;
+                        const uint16_t h = bf16_rne(w);
+                        const double r1 = w - bf16_value(h);
+                        const uint16_t mm = bf16_rne(r1);
+                        const uint16_t lo = bf16_rne(r1 - bf16_value(mm));
+                        const uint16_t parts[3] = {h, mm, lo};
+                        for (int q = 0; q < kMfmaParts; ++q)
+                            frag[(((((size_t)e * 4 + pm) * nch + c) * kMfmaParts + q) * kWave + l) * 8 + i] =
+                                parts[q];
+                    }
+}
+
 }  // namespace hjm

@@ -620,37 +620,15 @@ __global__ __launch_bounds__(kBlock) void pf_adjoint_seed_kernel(const PfValuePa
             }
         }
     };
-    if constexpr (sizeof(Real) == 4) {
-        // FP32: the date's coefficient rows staged in LDS, kValueSets sets at a time, read
-        // with workgroup-uniform addresses (broadcasts); scalar loads of them from every
-        // wave cost a third more time
-        __shared__ Real sg[kValueSets * KP];
-        __shared__ Real sa[kValueSets];
-        __shared__ double sw[kValueSets];
-        __shared__ int sk[kValueSets];
-        for (int g0 = 0; g0 < v.nsets; g0 += kValueSets) {
-            const int ng = v.nsets - g0 < kValueSets ? v.nsets - g0 : kValueSets;
-            __syncthreads();
-            const int64_t base = (int64_t)e * v.nsets + g0;
-            for (int t = threadIdx.x; t < ng * KP; t += blockDim.x) sg[t] = coef[base * KP + t];
-            for (int t = threadIdx.x; t < ng; t += blockDim.x) {
-                sa[t] = aconst[base + t];
-                sk[t] = kmax[base + t];
-                sw[t] = w[(int64_t)(g0 + t) * v.ndates + e];
-            }
-            __syncthreads();
-            for (int gg = 0; gg < ng; ++gg) set_adjoint(sk[gg], sg + gg * KP, sa[gg], sw[gg]);
-        }
-    } else {
-        // FP64: scalar loads (LDS staging needs more registers than two waves per SIMD allow)
-        kconst_ptr_pf<Real> cp = (kconst_ptr_pf<Real>)coef;
-        kconst_ptr_pf<Real> ap = (kconst_ptr_pf<Real>)aconst;
-        kconst_ptr_pf<int> kp = (kconst_ptr_pf<int>)kmax;
-        kconst_ptr_pf<double> wp = (kconst_ptr_pf<double>)w;
-        for (int gset = 0; gset < v.nsets; ++gset) {
-            const int64_t row = (int64_t)e * v.nsets + gset;
-            set_adjoint(kp[row], cp + row * KP, ap[row], wp[(int64_t)gset * v.ndates + e]);
-        }
+    // the coefficient rows by scalar loads (FP64 plans; staging them in LDS needs more
+    // registers than two waves per SIMD allow)
+    kconst_ptr_pf<Real> cp = (kconst_ptr_pf<Real>)coef;
+    kconst_ptr_pf<Real> ap = (kconst_ptr_pf<Real>)aconst;
+    kconst_ptr_pf<int> kp = (kconst_ptr_pf<int>)kmax;
+    kconst_ptr_pf<double> wp = (kconst_ptr_pf<double>)w;
+    for (int gset = 0; gset < v.nsets; ++gset) {
+        const int64_t row = (int64_t)e * v.nsets + gset;
+        set_adjoint(kp[row], cp + row * KP, ap[row], wp[(int64_t)gset * v.ndates + e]);
     }
     // suffix sums of iota_k = -P_k dP_k and the extrapolation moment
     Real S = (Real)0, ext = (Real)0;
@@ -882,15 +860,6 @@ hipError_t launch_pf_adjoint_seed_f64(const PfValueParams& v, const double* curv
 {
     return pf_adj_seed_launch<double>(v, curves, logb, coef, a, kmax, w, gbar, beta, beta_partials,
                                       st);
-}
-
-hipError_t launch_pf_adjoint_seed_f32(const PfValueParams& v, const float* curves,
-                                      const double* logb, const float* coef, const float* a,
-                                      const int* kmax, const double* w, float* gbar, float* beta,
-                                      double* beta_partials, hipStream_t st)
-{
-    return pf_adj_seed_launch<float>(v, curves, logb, coef, a, kmax, w, gbar, beta, beta_partials,
-                                     st);
 }
 
 int64_t pf_adjoint_blocks(int64_t paths) { return (paths + kAdjPaths - 1) / kAdjPaths; }
