@@ -1,11 +1,12 @@
-// hjm_mfma_kernels.hip -- FP32 path kernels on the matrix cores.
+// hjm_mfma_kernels.hip -- FP32 path, valuation and adjoint-seed kernels on the matrix cores.
 //
 // Each 50-step block is the affine map [f_out; integral rows] = W [f_in; phi; 1]
 // (hjm_mfma_plan.h), computed with v_mfma_f32_16x16x32_bf16 as six products of three-way
 // bf16 splits (W's parts precomputed, the operand split exactly in registers), which
 // keeps FP32 accuracy while running on the matrix cores concurrently with the VALU
 // (FP32 MFMA on gfx950 shares the vector pipeline and gains nothing over it). A wave owns
-// 32 paths = 2 N-tiles of 16. The accumulator layout of a 16x16 tile -- lane l holds rows
+// 32 paths = 2 N-tiles of 16 (the portfolio kernel's shape is a launch choice). The
+// accumulator layout of a 16x16 tile -- lane l holds rows
 // 4 (l / 16) + r, r = 0..3, of column l % 16 -- is used directly as the next block's B
 // operand (W's columns are ordered to match), so the curve never leaves registers. The
 // shocks of the block are generated on the VALU, one step per lane group per quad, and
@@ -17,9 +18,11 @@
 //     every date and the short rate's left-point sum (row 51) extends the log bank account.
 //   pf_value_mfma_kernel: the portfolio's netting-set values at one date, C = a - gamma P
 //     over 16-set x 16-path tiles (sets as M, paths as N, periods as K).
+//   pf_adjoint_seed_mfma_kernel: the portfolio adjoint's seeds, V = a - gamma P and
+//     dP = -gamma^T u, the second product fed from the first's accumulators.
 //   pf_value_stats_mfma_kernel: the same values at every date of a workgroup's paths,
 //     reduced to EE sums, per-path CVA moments and the PFE's bracket counting where they
-//     are produced (no exposures in HBM).
+//     are produced (no exposures in HBM; opt-in).
 #include "hjm_common.h"
 #include "hjm_kernels.h"
 #include "hjm_mfma_plan.h"
