@@ -252,6 +252,10 @@ int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfol
     HJM_TRY(dev_alloc(&p->d_w, p->w.size()));
     HJM_HIP(hipMemcpy(p->d_kmax, kmax.data(), kmax.size() * sizeof(int), hipMemcpyHostToDevice),
             HJM_ERR_DEVICE);
+    p->vp.kmax = p->d_kmax;
+    // rows of matured sets are never written (their exposures are 0) and never read by the
+    // statistics pass; they hold zeros for the PFE selection and the per-path copy
+    HJM_HIP(hipMemset(p->d_ex, 0, (size_t)pf->nsets * nd * P * rsz), HJM_ERR_DEVICE);
     HJM_HIP(hipMemcpy(p->d_w, p->w.data(), p->w.size() * sizeof(double), hipMemcpyHostToDevice),
             HJM_ERR_DEVICE);
     if (p->precision == HJM_FP64) {

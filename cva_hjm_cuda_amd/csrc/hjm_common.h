@@ -86,6 +86,9 @@ struct PfValueParams {
     int nsets;
     int ndates;
     double dtau;
+    // [date][set] the set's largest live period (0: every swap of the set has matured, its
+    // values and exposures at that date are 0), or null
+    const int* kmax;
 };
 
 // ---- wave / block reductions (fixed order, deterministic) ------------------------

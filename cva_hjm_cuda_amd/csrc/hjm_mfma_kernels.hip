@@ -667,7 +667,8 @@ __device__ __forceinline__ float exposure_of(float V, float disc) { return V > 0
 
 // Exposures of paths [0, count) at the date blockIdx.y (rows [date][set], row stride
 // v.paths). A workgroup of WAVES waves of NT N-tiles values WAVES x NT x 16 paths; the
-// date's -gamma fragments pass through LDS `stage` M-tiles at a time (dynamic LDS).
+// date's -gamma fragments pass through LDS `stage` M-tiles at a time (dynamic LDS). Rows of
+// sets whose swaps have all matured (v.kmax 0) are not written: the plan keeps them 0.
 // filter (or null): value only the M-tiles holding a set whose PFE row at this date the
 // counting pass left unresolved (bracket_row_resolved false for rank fk and fcap
 // candidate slots); a workgroup with none returns at once.
@@ -688,6 +689,7 @@ __global__ __launch_bounds__(WAVES * kWave) void pf_value_mfma_kernel(const PfVa
     constexpr int kThreads = WAVES * kWave, kPaths = WAVES * NT * 16;
     extern __shared__ u4_t sfrag[];
     __shared__ uint32_t s_need[kMaxPfMTiles];
+    __shared__ uint8_t s_live[HJM_MAX_SETS];
     __shared__ int s_any;
     const int e = blockIdx.y;
     const int64_t P = v.paths;
@@ -722,6 +724,7 @@ __global__ __launch_bounds__(WAVES * kWave) void pf_value_mfma_kernel(const PfVa
         const int64_t p = pbase + 16 * n + col;
         value_bonds(v, e, p < count ? p : 0, curves, grp, col, bs[n]);
     }
+    for (int g = threadIdx.x; g < G; g += kThreads) s_live[g] = v.kmax[(int64_t)e * G + g] > 0;
     for (int m0 = 0; m0 < mtiles; m0 += stage) {
         const int mt = mtiles - m0 < stage ? mtiles - m0 : stage;
         __syncthreads();
@@ -732,6 +735,7 @@ __global__ __launch_bounds__(WAVES * kWave) void pf_value_mfma_kernel(const PfVa
             const int m = m0 + mm;
             if (filter && !s_need[m]) continue;
             const int tk = tilek[(int64_t)e * mtiles + m];
+            if (tk == 0) continue;   // every set of the tile has matured
             const f4_t a4 = *reinterpret_cast<const f4_t*>(apad + ((int64_t)e * mtiles + m) * 16 + 4 * grp);
             f4_t acc[NT];
             value_tile<NT>(sfrag + (mm * kValueFragsPerTile) * kWave, lane, tk, a4, bs, acc);
@@ -742,7 +746,8 @@ __global__ __launch_bounds__(WAVES * kWave) void pf_value_mfma_kernel(const PfVa
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int g = 16 * m + 4 * grp + r;
-                        if (g < G) exposures[((int64_t)e * G + g) * P + p] = exposure_of(acc[n][r], disc[n]);
+                        if (g < G && s_live[g])
+                            exposures[((int64_t)e * G + g) * P + p] = exposure_of(acc[n][r], disc[n]);
                     }
                 }
             }

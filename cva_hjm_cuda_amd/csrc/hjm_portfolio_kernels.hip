@@ -227,6 +227,7 @@ __device__ __forceinline__ void pf_stats_rows(const PfValueParams v,
     __shared__ uint32_t base[kD];            // claimed candidate slot of the workgroup's rows
     __shared__ U s_lo[kD], s_hi[kD];         // the rows' brackets
     __shared__ double s_w[kMaxDates];        // the set's CVA weights
+    __shared__ uint8_t s_live[kMaxDates];    // the set has a live swap at the date
     extern __shared__ __align__(16) unsigned char stage_raw[];
     const int gset = blockIdx.y;
     const int64_t P = v.paths;
@@ -243,6 +244,7 @@ __device__ __forceinline__ void pf_stats_rows(const PfValueParams v,
     // wait for it also wait for the next date's exposures already in flight
     for (int e = threadIdx.x; e < nd; e += kBlock) {
         s_w[e] = w[(int64_t)gset * nd + e];
+        s_live[e] = v.kmax ? v.kmax[(int64_t)e * v.nsets + gset] > 0 : 1;
         if constexpr (kCount) {
             s_lo[e] = (U)sel[(int64_t)e * v.nsets + gset].lo;
             s_hi[e] = (U)sel[(int64_t)e * v.nsets + gset].hi;
@@ -252,9 +254,13 @@ __device__ __forceinline__ void pf_stats_rows(const PfValueParams v,
     // the next date's exposures are loaded while this date's are reduced and counted
     const bool full = (int64_t)(blockIdx.x + 1) * kStatPaths <= P;  // no path out of range
     Real xn[kStatPerThread];
+    // a matured set's row holds zeros: not read
     auto load = [&](int e, Real* dst) {
         const Real* row = exposures + ((int64_t)e * v.nsets + gset) * P;
-        if (full) {
+        if (!s_live[e]) {
+#pragma unroll
+            for (int k = 0; k < kStatPerThread; ++k) dst[k] = (Real)0;
+        } else if (full) {
 #pragma unroll
             for (int k = 0; k < kStatPerThread; ++k)
                 dst[k] = __builtin_nontemporal_load(row + i0 + (int64_t)k * kBlock);
