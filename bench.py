@@ -6,8 +6,9 @@ Nelson-Siegel curve, 3 parametric factors, 51 tenors, dt=0.01 -> 1000 steps,
 integrals, EE / PFE97.5 / CVA (h=2%, R=40%) in FP64. One step = one full pass (path
 kernel, exposure, reductions, PFE selection) with inputs resident on the device.
 
-Prints one JSON line. Also measured and reported: the FP64 kernel on the same
-normals (path-steps/s and the FP32-vs-FP64 EE/PFE/CVA errors), the reference's own GPU
+Prints one JSON line. Also measured and reported: the FP64 kernels (vector, and matrix
+cores: `fp64_matrix`) on the same normals (path-steps/s and the FP32-vs-FP64 EE/PFE/CVA
+errors), the reference's own GPU
 design in FP64 ("plain FP64 GPU form", one timed pass), and the CPU oracle on a bounded
 sample (cpu_baseline).
 
@@ -33,6 +34,7 @@ sys.path.insert(0, ROOT)
 FLOPS_PER_PATH_STEP = 561          # 51 tenors x (1 sub + 4 FMA + 1 add + 1 accumulate)
 PF_FLOPS_PER_PATH_STEP = 512       # 51 tenors x 5 FMA + the short-rate sum (portfolio)
 PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}   # MI355X published vector peaks
+MFMA64_FLOPS = 2 * 16 * 16 * 4     # one v_mfma_f64_16x16x4f64
 PATHS, EXPIRY, DT, SEED = 131072, 10.0, 0.01, 7
 TENORS_N = 51
 KERNEL_NAMES = {"matrix": "ref_paths_mfma_kernel", "vector": "ref_paths_kernel<float,compensated,philox>"}
@@ -361,6 +363,8 @@ def main():
         return
     log("[bench] FP64 path kernel")
     f64 = run_plan(hjm, market, swap, hjm.FP64, max(2, args.steps // 2), 1)
+    log("[bench] FP64 matrix-core path kernel")
+    f64m = run_plan(hjm, market, swap, hjm.FP64, max(2, args.steps // 2), 1, hjm.KERNEL_MATRIX)
     log("[bench] FP32 + compensated, %s path kernel" % other)
     alt = run_plan(hjm, market, swap, hjm.FP32_KAHAN, max(2, args.steps // 2), 1,
                    kernels[other])
@@ -374,6 +378,8 @@ def main():
     f64_rate = PATHS * S * f64["kernel_ms"].shape[0] / f64["wall"]
     k32 = float(np.mean(f32["kernel_ms"][:, 0]))
     k64 = float(np.mean(f64["kernel_ms"][:, 0]))
+    k64m = float(np.mean(f64m["kernel_ms"][:, 0]))
+    f64m_rate = PATHS * S * f64m["kernel_ms"].shape[0] / f64m["wall"]
     kalt = float(np.mean(alt["kernel_ms"][:, 0]))
     flops = FLOPS_PER_PATH_STEP * PATHS * evolved
     achieved = flops / (k32 * 1e-3) / 1e12
@@ -446,6 +452,19 @@ def main():
         "fp64": {"path_steps_per_s": f64_rate, "kernel_ms": k64,
                  "roofline_frac": flops / (k64 * 1e-3) / 1e12 / PEAK_TFLOPS["f64"],
                  "fp32_over_fp64": value / f64_rate},
+        "fp64_matrix": {"kernel": "ref_paths_mfma64_kernel", "kernel_ms": k64m,
+                        "path_steps_per_s": f64m_rate,
+                        "roofline_frac": flops / (k64m * 1e-3) / 1e12 / PEAK_TFLOPS["f64"],
+                        # the products the kernel issues per 16 paths and evolved block:
+                        # 52 v_mfma_f64_16x16x4f64 on each of 5 live M-tiles (20 dates)
+                        # less the 15 whose block-map entries are structurally zero
+                        "mfma_issued_frac": MFMA64_FLOPS * (52 * 5 - 15) * (PATHS // 16) * (evolved // 50)
+                        / (k64m * 1e-3) / 1e12 / PEAK_TFLOPS["f64"],
+                        "vs_fp64_vector": {
+                            "ee_max_rel": float(np.max(np.abs(f64m["ee"][big] / f64["ee"][big] - 1))),
+                            "pfe_max_rel": float(np.max(np.abs(f64m["pfe"][big] / f64["pfe"][big] - 1))),
+                            "cva_rel": abs(f64m["cva"] / f64["cva"] - 1.0)},
+                        "fp32_over_fp64_matrix": value / f64m_rate},
         "plain_fp64_gpu_form": plain,
         "fp32_vs_fp64_error": err,
         "adjoint": adj,

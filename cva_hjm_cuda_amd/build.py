@@ -16,7 +16,7 @@ LIB = os.path.join(LIBDIR, "libhjm_cva.so")
 CLI = os.path.join(LIBDIR, "hjm_cli")
 SOURCES = ["hjm_capi.hip", "hjm_ref_kernels.hip", "hjm_baseline_kernels.hip",
            "hjm_adjoint_kernels.hip", "hjm_portfolio_kernels.hip",
-           "hjm_capi_portfolio.hip", "hjm_mfma_kernels.hip", "hjm_calibration.hip",
+           "hjm_capi_portfolio.hip", "hjm_mfma_kernels.hip", "hjm_mfma64_kernels.hip", "hjm_calibration.hip",
            "hjm_select_kernels.hip"]
 ARCH = os.environ.get("HJM_OFFLOAD_ARCH", "gfx950")
 

@@ -64,7 +64,8 @@ struct hjm_plan_s {
     RefGrid grid{};
     ExposureParams ep{};
     MfmaGrid mg{};
-    uint16_t* d_wfrag = nullptr;  // bf16 parts of the block map
+    uint16_t* d_wfrag = nullptr;  // bf16 parts of the block map (FP32 plans)
+    double* d_wfrag64 = nullptr;  // the block map's FP64 fragments (FP64 plans)
     double* d_normals = nullptr;
     double* d_nx = nullptr;
     double* d_ny = nullptr;
@@ -86,6 +87,7 @@ void plan_release(hjm_plan p)
     (void)hipSetDevice(p->device);
     dev_free(p->d_normals);
     dev_free(p->d_wfrag);
+    dev_free(p->d_wfrag64);
     dev_free(p->d_nx);
     dev_free(p->d_ny);
     dev_free(p->d_ex);
@@ -160,10 +162,12 @@ int plan_init(hjm_plan p, const hjm_market* market, const hjm_swap* swap, const 
 {
     int stride, S, D;
     HJM_TRY(check_ref_config(market, swap, cfg, &stride, &S, &D));
-    const bool mfma_ok = cfg->precision == HJM_FP32_KAHAN && cfg->rng == HJM_RNG_PHILOX &&
-                         stride == kMfmaStride;
+    const bool mfma_ok = cfg->rng == HJM_RNG_PHILOX && stride == kMfmaStride;
     if (cfg->path_kernel == HJM_KERNEL_MATRIX && !mfma_ok) return HJM_ERR_UNSUPPORTED;
-    p->use_mfma = mfma_ok && cfg->path_kernel != HJM_KERNEL_VECTOR;
+    // AUTO: the matrix cores for FP32; FP64 keeps the vector kernel, which evolves step by
+    // step in the reference's operation order, unless MATRIX is asked for
+    p->use_mfma = mfma_ok && (cfg->path_kernel == HJM_KERNEL_MATRIX ||
+                              (cfg->path_kernel == HJM_KERNEL_AUTO && cfg->precision == HJM_FP32_KAHAN));
 
     p->device = cfg->device;
     p->precision = cfg->precision;
@@ -196,14 +200,26 @@ int plan_init(hjm_plan p, const hjm_market* market, const hjm_swap* swap, const 
             kTenors, bracket_plan(cfg->paths, p->pfe_rank, sizeof(double)), sizeof(double));
         if (wsb) HJM_TRY(dev_alloc(&p->d_selws, wsb));
     }
+    // host copies of the block map, kept until the stream is synchronised below
+    std::vector<uint16_t> wf;
+    std::vector<double> wf64;
     if (p->use_mfma) {
-        HJM_HIP(mfma_kernels_prepare(), HJM_ERR_DEVICE);
-        std::vector<uint16_t> wf;
-        build_mfma_block_map(market->drifts, market->volatilities, cfg->dt, swap->dtau, wf);
-        HJM_TRY(dev_alloc(&p->d_wfrag, wf.size()));
-        HJM_HIP(hipMemcpyAsync(p->d_wfrag, wf.data(), wf.size() * sizeof(uint16_t),
-                               hipMemcpyHostToDevice, p->stream),
-                HJM_ERR_DEVICE);
+        if (cfg->precision == HJM_FP64) {
+            HJM_HIP(mfma64_kernels_prepare(), HJM_ERR_DEVICE);
+            build_mfma64_block_map(market->drifts, market->volatilities, cfg->dt, swap->dtau,
+                                   ref_mfma_tiles(D), wf64);
+            HJM_TRY(dev_alloc(&p->d_wfrag64, wf64.size()));
+            HJM_HIP(hipMemcpyAsync(p->d_wfrag64, wf64.data(), wf64.size() * sizeof(double),
+                                   hipMemcpyHostToDevice, p->stream),
+                    HJM_ERR_DEVICE);
+        } else {
+            HJM_HIP(mfma_kernels_prepare(), HJM_ERR_DEVICE);
+            build_mfma_block_map(market->drifts, market->volatilities, cfg->dt, swap->dtau, wf);
+            HJM_TRY(dev_alloc(&p->d_wfrag, wf.size()));
+            HJM_HIP(hipMemcpyAsync(p->d_wfrag, wf.data(), wf.size() * sizeof(uint16_t),
+                                   hipMemcpyHostToDevice, p->stream),
+                    HJM_ERR_DEVICE);
+        }
         MfmaGrid& mg = p->mg;
         mg.ndates = D;
         mg.paths = cfg->paths;
@@ -214,6 +230,7 @@ int plan_init(hjm_plan p, const hjm_market* market, const hjm_swap* swap, const 
         for (int j = 0; j < kTenors; ++j) {
             mg.a0[j] = g.a0[j];
             mg.spot[j] = (float)market->spot_rates[j];
+            mg.spot64[j] = market->spot_rates[j];
         }
     }
     HJM_HIP(hipMemsetAsync(p->d_nx, 0, slots * sizeof(double), p->stream), HJM_ERR_DEVICE);
@@ -234,10 +251,14 @@ int plan_launch(hjm_plan p, bool keep_exposures, float* zs = nullptr)
 {
     hipStream_t st = p->stream;
     HJM_HIP(hipEventRecord(p->ev[0], st), HJM_ERR_DEVICE);
-    hipError_t err = p->use_mfma ? launch_ref_paths_mfma(p->mg, p->d_wfrag, p->d_nx, p->d_ny, zs, st)
-                     : (p->precision == HJM_FP64)
-                         ? launch_ref_paths_f64(p->kd, p->grid, p->d_normals, p->d_nx, p->d_ny, st)
-                         : launch_ref_paths_f32k(p->kf, p->grid, p->d_normals, p->d_nx, p->d_ny, st);
+    const bool f64 = p->precision == HJM_FP64;
+    hipError_t err;
+    if (p->use_mfma)
+        err = f64 ? launch_ref_paths_mfma64(p->mg, p->d_wfrag64, p->d_nx, p->d_ny, zs, st)
+                  : launch_ref_paths_mfma(p->mg, p->d_wfrag, p->d_nx, p->d_ny, zs, st);
+    else
+        err = f64 ? launch_ref_paths_f64(p->kd, p->grid, p->d_normals, p->d_nx, p->d_ny, st)
+                  : launch_ref_paths_f32k(p->kf, p->grid, p->d_normals, p->d_nx, p->d_ny, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
     HJM_HIP(hipEventRecord(p->ev[1], st), HJM_ERR_DEVICE);
     double* ex = (p->do_pfe || keep_exposures) ? p->d_ex : nullptr;

@@ -64,6 +64,7 @@ struct MfmaGrid {
     double dt;
     double a0[kTenors];   // frozen block-0 integrals (FP64)
     float spot[kTenors];
+    double spot64[kTenors];  // the FP64 kernel's initial curve
 };
 
 // Portfolio semantics: dates every dtau up to kMaxPeriods periods (30y at dtau = 0.5).

@@ -38,6 +38,11 @@ hipError_t mfma_kernels_prepare();
 // zs (optional): the forward's FP32 normals [step][factor][path] for a later reverse sweep
 hipError_t launch_ref_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, double* nx, double* ny,
                                  float* zs, hipStream_t st);
+// The FP64 matrix-core path kernel (reference semantics); wfrag: build_mfma64_block_map's
+// fragments of ref_mfma_tiles(g.ndates) M-tiles; prepare once per plan.
+hipError_t mfma64_kernels_prepare();
+hipError_t launch_ref_paths_mfma64(const MfmaGrid& g, const double* wfrag, double* nx, double* ny,
+                                   float* zs, hipStream_t st);
 // Launch shape of a matrix-core portfolio kernel (0 in a field: the kernel's default).
 //   tiles: N-tiles of 16 paths per wave; waves: per workgroup;
 //   stage (valuation): M-tiles of -gamma staged in LDS at a time;

@@ -149,9 +149,13 @@ __device__ __forceinline__ void mfma_block(const u4_t* __restrict__ lds_w, const
     for (int m = 0; m < MT2; ++m)
 #pragma unroll
         for (int n = 0; n < NT; ++n) acc2[m][n] = f4_t{0.f, 0.f, 0.f, 0.f};
+    // M-tiles [m0, m1) of chunk c, less the structurally zero curve chunks (the reference
+    // and portfolio maps share tiles 0..3, whose zeros agree; c and m are constants once
+    // unrolled)
     auto tiles = [&](int c, int m0, int m1, const Split8 (&xs)[NT]) {
 #pragma unroll
         for (int m = m0; m < m1; ++m) {
+            if (mfma_chunk_zero(m, c, false)) continue;
             const u4_t* a = lds_w + ((m * kMfmaChunks + c) * kMfmaParts) * kWave + off;
             const u4_t ah = a[0], am = a[kWave], al = a[2 * kWave];
 #pragma unroll
@@ -361,14 +365,6 @@ __global__ __launch_bounds__(kMfmaBlock) void ref_paths_mfma_kernel(const MfmaGr
             }
         }
     }
-}
-
-// Live M-tiles of the reference map for a run of ndates dates: the curve's four and the
-// integral rows 51 + j up to j = ndates (the last slot recorded).
-static int ref_mfma_tiles(int ndates)
-{
-    const int last = kTenors + (ndates > 1 ? ndates : 1);
-    return last / 16 + 1 < 4 ? 4 : last / 16 + 1;
 }
 
 // Portfolio semantics: dates e = 0 .. ndates-1 every 50 steps from t = 0. At each date the

@@ -1,5 +1,5 @@
 // hjm_mfma_plan.h -- host construction of the 50-step block maps for the matrix-core
-// FP32 path kernels.
+// path kernels (FP32 as bf16 parts, FP64 as they are) and of the valuation's fragments.
 //
 // One block of S = 50 Euler steps is affine in the curve f, the scaled shocks
 // s_{t,k} = sqrt(dt) phi_k(t) and 1:
@@ -51,6 +51,37 @@ constexpr int kMfmaPfFragElems = kMfmaPfMTiles * kMfmaChunks * kMfmaParts * kWav
 constexpr int ref_mfma_lds_bytes(int mt) { return mt * kMfmaChunks * kMfmaParts * kWave * 8 * 2; }
 
 enum class MfmaMap { kReference, kPortfolio };
+
+// Live M-tiles of the reference map for a run of ndates dates: the curve's four and the
+// integral rows 51 + j up to j = ndates (the last slot recorded).
+inline int ref_mfma_tiles(int ndates)
+{
+    const int last = kTenors + (ndates > 1 ? ndates : 1);
+    return last / 16 + 1 < 4 ? 4 : last / 16 + 1;
+}
+
+// Structural zeros of the maps' curve columns. T couples tenor j only to j + 1 (and 50 to
+// 49), so T^n and the integral rows' sums of powers vanish in columns R < min(j, 49) of
+// row j; integral row 51 + j follows tenor j; the portfolio's row 51 (the short rate's
+// sum) spans every tenor. The kernels skip a product whose curve inputs all lie below the
+// first input of every row of its M-tile.
+__host__ __device__ constexpr int map_row_first_input(int row, bool pf)
+{
+    return row < kTenors ? (row < 49 ? row : 49) : pf ? 0 : (row - kTenors < 49 ? row - kTenors : 49);
+}
+__host__ __device__ constexpr int tile_first_input(int m, bool pf)
+{
+    const int rows = pf ? kTenors + 1 : 2 * kTenors;
+    int lo = kTenors;
+    for (int r = 16 * m; r < 16 * m + 16 && r < rows; ++r)
+        lo = map_row_first_input(r, pf) < lo ? map_row_first_input(r, pf) : lo;
+    return lo;
+}
+// FP32 kernels: curve chunk c (rows 32 c .. 32 c + 31) of M-tile m is all zero
+__host__ __device__ constexpr bool mfma_chunk_zero(int m, int c, bool pf)
+{
+    return c < 2 && 32 * c + 31 < tile_first_input(m, pf);
+}
 
 // What K index k of the block map multiplies: a curve row, a shock (step, factor), the
 // constant 1, or nothing.
@@ -203,6 +234,70 @@ inline void build_mfma_block_map(const double* drift, const double* vols, double
                     for (int p = 0; p < kMfmaParts; ++p)
                         frag[((((size_t)m * kMfmaChunks + c) * kMfmaParts + p) * kWave + l) * 8 + i] = parts[p];
                 }
+}
+
+// The FP64 block map for v_mfma_f64_16x16x4f64 (ref_paths_mfma64_kernel). One instruction
+// takes K = 4 entries, one per lane group g = lane / 16, from one register of every lane;
+// its accumulator holds row 4 i + g of the 16-row tile in register i of lane group g (not
+// 4 g + i as in the FP32 16x16 forms). The kernel issues kMfma64K = 52 instructions per
+// M-tile and block:
+//   j = 0..12 (curve): register i = j % 4 of the previous block's accumulator tile j / 4,
+//     i.e. curve row R = 16 (j / 4) + 4 i + g (zero column where R > 50; tile 3 holds
+//     rows 48..50 in register 0 only, hence 13 and not 16);
+//   j = 13..51 (shocks): value v = j - 13 of lane group g, factor v % 3 of step
+//     4 (v / 3) + g -- the FP32 kernel's shocks, in its order; at step 50 factor 0 is the
+//     constant 1 carrying the drift, other entries past step 49 are zero.
+// Fragment [m][j][lane] = W[16 m + lane % 16][column (j, lane / 16)], FP64.
+constexpr int kMfma64Curve = 13;
+constexpr int kMfma64K = kMfma64Curve + 3 * kMfmaQuads;  // 52
+constexpr int kMfma64TileBytes = kMfma64K * kWave * 8;   // 26624
+constexpr int kMfma64LdsTiles = 5;  // M-tiles staged in LDS (133 KB); later tiles from HBM
+// curve instruction j < 13 of M-tile m is all zero (its rows 16 (j / 4) + 4 (j % 4) + g lie
+// below the tile's first input)
+__host__ __device__ constexpr bool mfma64_column_zero(int m, int j)
+{
+    return j < kMfma64Curve && 16 * (j / 4) + 4 * (j % 4) + 3 < tile_first_input(m, false);
+}
+
+inline MfmaColumn mfma64_column(int j, int g)
+{
+    if (j < kMfma64Curve) {
+        const int R = 16 * (j / 4) + 4 * (j % 4) + g;
+        if (R < kTenors) return {MfmaColumn::kCurve, R, 0, 0};
+        return {MfmaColumn::kZero, 0, 0, 0};
+    }
+    const int v = j - kMfma64Curve, q = v / 3, f = v % 3, t = 4 * q + g;
+    if (t < kMfmaStride) return {MfmaColumn::kShock, 0, t, f};
+    if (t == kMfmaStride && f == 0) return {MfmaColumn::kConst, 0, 0, 0};
+    return {MfmaColumn::kZero, 0, 0, 0};
+}
+
+inline void build_mfma64_block_map(const double* drift, const double* vols, double dt, double dtau,
+                                   int mtiles, std::vector<double>& frag)
+{
+    const int rows = 2 * kTenors;
+    const std::vector<double> W = mfma_block_map_dense(drift, vols, dt, dtau, MfmaMap::kReference);
+    // the dense map's column of each curve row, shock (step, factor) and the constant
+    int curve_k[kTenors] = {}, shock_k[kMfmaStride][3] = {}, const_k = -1;
+    for (int k = 0; k < kMfmaK; ++k) {
+        const MfmaColumn c = mfma_column(k);
+        if (c.kind == MfmaColumn::kCurve) curve_k[c.row] = k;
+        else if (c.kind == MfmaColumn::kShock) shock_k[c.step][c.factor] = k;
+        else if (c.kind == MfmaColumn::kConst) const_k = k;
+    }
+    frag.assign((size_t)mtiles * kMfma64K * kWave, 0.0);
+    for (int m = 0; m < mtiles; ++m)
+        for (int j = 0; j < kMfma64K; ++j)
+            for (int l = 0; l < kWave; ++l) {
+                const int row = 16 * m + l % 16;
+                const MfmaColumn c = mfma64_column(j, l / 16);
+                int k = -1;
+                if (c.kind == MfmaColumn::kCurve) k = curve_k[c.row];
+                else if (c.kind == MfmaColumn::kShock) k = shock_k[c.step][c.factor];
+                else if (c.kind == MfmaColumn::kConst) k = const_k;
+                if (row < rows && k >= 0)
+                    frag[((size_t)m * kMfma64K + j) * kWave + l] = W[(size_t)row * kMfmaK + k];
+            }
 }
 
 // Matrix-core valuation of netting sets (pf_value_mfma_kernel): per date, M-tiles of 16

@@ -51,9 +51,12 @@ enum hjm_rng_mode {
 };
 
 enum hjm_path_kernel {
-    HJM_KERNEL_AUTO = 0,   /* matrix cores for FP32 + in-kernel RNG when dt = dtau / 50 */
+    HJM_KERNEL_AUTO = 0,   /* matrix cores for FP32 + in-kernel RNG when dt = dtau / 50;
+                              FP64 on the vector kernel */
     HJM_KERNEL_VECTOR = 1, /* one path per lane, every step on the vector ALUs */
-    HJM_KERNEL_MATRIX = 2  /* FP32 only: 50-step blocks as MFMA GEMMs (else HJM_ERR_UNSUPPORTED) */
+    HJM_KERNEL_MATRIX = 2  /* 50-step blocks as matrix-core products: FP32 as bf16 split
+                              products, FP64 with v_mfma_f64; in-kernel RNG and
+                              dt = dtau / 50 only (else HJM_ERR_UNSUPPORTED) */
 };
 
 /* How the adjoint's reverse sweep gets the forward pass's shocks. */

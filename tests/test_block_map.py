@@ -126,3 +126,36 @@ def test_block_map_rejects_other_grids(market):
         hjm.block_map(market, dt=0.02)
     with pytest.raises(hjm.HJMError):
         hjm.block_map(market, dt=0.02, portfolio=True)
+
+
+def _curve_column(k):
+    """Curve row fed by K index k (chunks 0, 1), or None."""
+    c, g, i = k // 32, (k % 32) // 8, k % 8
+    if c >= 2:
+        return None
+    R = 16 * (2 * c + i // 4) + 4 * g + i % 4
+    return R if R < TENORS else None
+
+
+@pytest.mark.parametrize("portfolio", [False, True])
+def test_block_maps_vanish_below_each_rows_first_tenor(market, portfolio):
+    """The kernels skip the products whose W entries are structurally zero. T couples tenor
+    j only to j + 1, and tenor 50 to 49 (the downwind extrapolation), so T^n -- and the
+    integral rows' sums of powers -- have no entry in column R < min(j, 49) of row j; the
+    integral row 51 + j follows tenor j, the portfolio's short-rate row 51 spans every
+    tenor. Exactly zero below that bound, nonzero at it (the structure is not a
+    coincidence of this market)."""
+    W = hjm.block_map(market, portfolio=portfolio).astype(np.float64)
+    for row in range(W.shape[0]):
+        if row < TENORS:
+            first = min(row, 49)
+        else:
+            first = 0 if portfolio else min(row - TENORS, 49)
+        for k in range(K):
+            R = _curve_column(k)
+            if R is None:
+                continue
+            if R < first:
+                assert W[row, k] == 0.0, (row, R)
+            elif R == first:
+                assert W[row, k] != 0.0, (row, R)

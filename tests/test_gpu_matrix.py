@@ -87,8 +87,6 @@ def test_matrix_kernel_selection_rules(oracle):
     _flush("[gpu] matrix kernel selection rules")
     m = hjm.benchmark_market()
     swap = hjm.InterestRateSwap(1e6, 0.025, 5.0)
-    with pytest.raises(hjm.HJMError):   # FP64 has no matrix-core form
-        hjm.simulate_reference(m, swap, 64, path_kernel=hjm.KERNEL_MATRIX)
     z = oracle.normals(1, 64, 500)
     with pytest.raises(hjm.HJMError):   # host normals
         hjm.simulate_reference(m, swap, 64, normals=z, precision=hjm.FP32_KAHAN,
