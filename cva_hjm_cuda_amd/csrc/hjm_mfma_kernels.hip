@@ -571,10 +571,12 @@ __device__ __forceinline__ void value_bonds(const PfValueParams& v, int e, int64
                                             const float* __restrict__ curves, int grp, int col,
                                             Split8 (&bs)[2], float* __restrict__ pk = nullptr)
 {
-    const int64_t P = v.paths;
+    int64_t P = v.paths;
+    // opaque per call: a caller's date loop neither holds the 18 rows' offsets (8 grp + i) P
+    // in VGPRs nor their addresses
+    asm volatile("" : "+s"(P));
     const float hw = (float)(0.5 * v.dtau), dtf = (float)v.dtau;
     const float* fc = curves + (int64_t)e * kTenors * P + p;
-    // opaque per call: a caller's date loop does not hold the rows' addresses in VGPRs
     asm volatile("" : "+v"(fc));
     // segment 0: periods k = 8 grp + 1 .. 8 grp + 8 (tenors 8 grp .. 8 grp + 8 <= 32);
     // segment 1: k = 33 + 8 grp .. 40 + 8 grp (tenors capped at the last, 50)
@@ -993,53 +995,67 @@ hipError_t launch_pf_adjoint_seed_mfma(const PfValueParams& v, const float* curv
 }
 
 // ---- fused valuation and statistics (FP32 plans) -------------------------------------
-// The exposure pass's values with paths as the matrix rows: V^T = P^T (-Gamma)^T + a, the
-// bond prices of a wave's path tile as the A operand (the registers value_bonds builds,
-// whose A and B layouts coincide) and -Gamma's fragments as the B operand, the split
-// products in the exposure pass's order. Lane l then holds paths 4 (l / 16) + r, r = 0..3,
-// of set l % 16 of a set tile, so a set's sums over paths are in-lane sums and two
-// cross-row shuffles. A workgroup of 4 waves x 2 path tiles (128 paths) values a group of
-// up to 128 sets (blockIdx.y; one group up to 128 sets, so the bond prices are computed
-// once) at every date in turn and reduces where the values are produced, so the
-// exposures never go to HBM:
-//   * EE numerators: per date and set the workgroup's sum (FP32 sums of a wave's 32
-//     paths, then the waves in order in FP64), partials[set][block][date];
-//   * the per-path CVA sample sum_e w_ge E_pge in registers (FP32) across the dates, its
-//     sum and square sum per set at the end, partials[set][block][ndates + 0, 1];
+// The exposure pass's values, computed as it computes them (sets as the matrix rows, a
+// wave's paths as the columns: lane l holds sets 16 n + 4 (l / 16) + r of path l % 16 of
+// set tile n), reduced where they are produced so that the exposures never go to HBM. A
+// workgroup of kFuseWaves waves x kFuseNT N-tiles (128 paths) takes a group of up to 128
+// sets (blockIdx.y) through every date in turn: per date the bond prices once, then the
+// group's set tiles, whose -gamma fragments pass through LDS kFuseStage tiles at a time.
+// Per set tile and date:
+//   * EE numerators: the 4 sets' sums over a wave's paths (in-lane over its N-tiles, then
+//     over the 16 lanes of the row by a transposing reduction), then the waves in order in
+//     FP64 at the date's flush: partials[set][block][date];
+//   * the per-path CVA sample sum_e w_ge E_pge in registers (FP32) across the dates; its
+//     sum and square sum per set at the end: partials[set][block][ndates + 0, 1];
 //   * with sel: the counting pass of the PFE's sampled-bracket selection for every row
-//     (date, set) -- entries below the bracket (or the zeros), entries inside it counted
-//     and staged in LDS (kFuseSlots per set and date; more flags the row overflowed, so
-//     its selection values it again, whole), then one claim per row and workgroup and the
-//     copy of the staged entries to the row's candidate buffer.
-// The date's fragments and the per-set values it needs are staged in LDS; the per-set
-// arrays are double-buffered by date, so the flush of date e - 1 (EE partials, counts,
-// claims, candidates; one thread per set) runs beside the staging of date e: two
-// barriers per date.
-constexpr int kFuseWaves = 8;
+//     (date, set) -- entries below the bracket (or the zeros), counted four sets to a
+//     packed word; entries inside it counted and staged in LDS (kFuseSlots per set and
+//     date; more flags the row overflowed, so its selection values it again, whole), then
+//     one claim per row and workgroup and the copy to the row's candidate buffer.
+// The per-date arrays are double-buffered, so the flush of date e - 1 (one thread per set)
+// runs beside the staging of date e. Sums are in fixed order and counts are integers:
+// results do not depend on scheduling (the staged candidates' order does, which the
+// selection does not see).
+constexpr int kFuseWaves = 4;
 constexpr int kFuseBlock = kFuseWaves * kWave;
-constexpr int kFusePT = 1;                              // path tiles (16 paths) per wave
-constexpr int kFusePaths = kFuseWaves * kFusePT * 16;   // 128
-constexpr int kFuseTiles = 8;                           // set tiles per group
-constexpr int kFuseSets = 16 * kFuseTiles;              // 128 sets per group
-constexpr int kFuseSlots = 16;                          // staged inside entries per set and date
+constexpr int kFuseNT = 2;                                // N-tiles (16 paths) per wave
+constexpr int kFusePaths = kFuseWaves * kFuseNT * 16;     // 128
+constexpr int kFuseTiles = 8;                             // set tiles per group
+constexpr int kFuseSets = 16 * kFuseTiles;                // 128 sets per group
+constexpr int kFuseStage = 4;                             // set tiles' fragments in LDS at a time
+constexpr int kFuseSlots = 16;                            // staged inside entries per set and date
 
 struct FuseDate {
-    float w[kFuseSets];       // CVA weight of the set at the date (0 past the last set)
-    float a[kFuseSets];       // the set's constant a
-    uint32_t lo[kFuseSets], hi[kFuseSets];   // PFE bracket (FP32 bits)
-    uint32_t nlo[kFuseSets], nin[kFuseSets]; // entries below / inside it
-    float ee[kFuseWaves][kFuseSets];         // per-wave EE sums
-    float cand[kFuseSets][kFuseSlots];       // staged inside entries
-    int tk[kFuseTiles];                      // the set tile's largest live period
+    float w[kFuseSets];                       // CVA weight of the set at the date (0 past the last set)
+    uint32_t lo[kFuseSets], hi[kFuseSets];    // PFE bracket (FP32 bits)
+    uint32_t nlo[kFuseSets], nin[kFuseSets];  // entries below / inside it
+    float ee[kFuseWaves][kFuseSets];          // per-wave EE sums
+    float cand[kFuseSets][kFuseSlots];        // staged inside entries
+    int tk[kFuseTiles];                       // the set tile's largest live period
 };
 
-__global__ __launch_bounds__(kFuseBlock) void pf_value_stats_mfma_kernel(
+// Sums of v[0..3] over the 16 lanes of a row; lane c of the row ends with the sum of
+// v[c / 4] (two exchanges of halves, then two plain steps: 5 shuffles instead of 16).
+template <typename T>
+__device__ __forceinline__ T row_sum4(const T (&v)[4], int col)
+{
+    const bool h8 = col & 8, h4 = col & 4;
+    T s0 = h8 ? v[2] : v[0], s1 = h8 ? v[3] : v[1];
+    s0 += __shfl_xor(h8 ? v[0] : v[2], 8, 16);
+    s1 += __shfl_xor(h8 ? v[1] : v[3], 8, 16);
+    T u = h4 ? s1 : s0;
+    u += __shfl_xor(h4 ? s0 : s1, 4, 16);
+    u += __shfl_xor(u, 2, 16);
+    return u + __shfl_xor(u, 1, 16);
+}
+
+__global__ __launch_bounds__(kFuseBlock) __attribute__((amdgpu_waves_per_eu(2))) void pf_value_stats_mfma_kernel(
     const PfValueParams v, int mtiles, const float* __restrict__ curves,
     const double* __restrict__ logb, const u4_t* __restrict__ gfrag, const float* __restrict__ apad,
     const int* __restrict__ tilek, const double* __restrict__ w, double* __restrict__ partials,
     SelRowState* __restrict__ sel, float* __restrict__ cand, int64_t cap, int slots)
 {
-    __shared__ u4_t sfrag[kFuseTiles * kValueFragsPerTile * kWave];
+    __shared__ u4_t sfrag[kFuseStage * kValueFragsPerTile * kWave];
     __shared__ FuseDate sd[2];
     const int64_t P = v.paths;
     const int G = v.nsets, nd = v.ndates, ncol = nd + 2;
@@ -1049,29 +1065,23 @@ __global__ __launch_bounds__(kFuseBlock) void pf_value_stats_mfma_kernel(
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
     const int col = lane % 16, grp = lane / 16;
     const int blocks = (int)gridDim.x;
-    const int64_t pbase = (int64_t)blockIdx.x * kFusePaths + wave * (kFusePT * 16);
-    // the lane's accumulator rows: paths pbase + 16 t + 4 grp + r
-    bool live[kFusePT][4];
+    const int64_t pbase = (int64_t)blockIdx.x * kFusePaths + wave * (kFuseNT * 16);
+    bool live[kFuseNT];
 #pragma unroll
-    for (int t = 0; t < kFusePT; ++t)
+    for (int n = 0; n < kFuseNT; ++n) live[n] = pbase + 16 * n + col < P;
+    float cvap[kFuseTiles][kFuseNT][4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) live[t][r] = pbase + 16 * t + 4 * grp + r < P;
-    float cvap[kFuseTiles][kFusePT][4];
+    for (int t = 0; t < kFuseTiles; ++t)
 #pragma unroll
-    for (int n = 0; n < kFuseTiles; ++n)
+        for (int n = 0; n < kFuseNT; ++n)
 #pragma unroll
-        for (int t = 0; t < kFusePT; ++t)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) cvap[n][t][r] = 0.f;
+            for (int r = 0; r < 4; ++r) cvap[t][n][r] = 0.f;
 
-    auto stage = [&](int e, FuseDate& d) {
-        const u4_t* src = gfrag + ((int64_t)e * mtiles + m0) * kValueFragsPerTile * kWave;
-        for (int t = threadIdx.x; t < nt * kValueFragsPerTile * kWave; t += kFuseBlock) sfrag[t] = src[t];
+    auto stage_date = [&](int e, FuseDate& d) {
         for (int gl = threadIdx.x; gl < kFuseSets; gl += kFuseBlock) {
             const bool in = gl < ng;
             const int64_t rid = (int64_t)e * G + g0 + gl;
             d.w[gl] = in ? (float)w[(int64_t)(g0 + gl) * nd + e] : 0.f;
-            d.a[gl] = in ? apad[((int64_t)e * mtiles + m0 + gl / 16) * 16 + gl % 16] : 0.f;
             d.lo[gl] = (in && sel) ? (uint32_t)sel[rid].lo : 0u;
             d.hi[gl] = (in && sel) ? (uint32_t)sel[rid].hi : 0u;
             d.nlo[gl] = 0u;
@@ -1082,136 +1092,141 @@ __global__ __launch_bounds__(kFuseBlock) void pf_value_stats_mfma_kernel(
     };
     // one thread per set: EE partial, counts, claim, candidates of date e
     auto flush = [&](int e, const FuseDate& d) {
-        const int gl = threadIdx.x;
-        if (gl >= ng) return;
-        double t = 0.0;
+        for (int gl = threadIdx.x; gl < ng; gl += kFuseBlock) {
+            double t = 0.0;
 #pragma unroll
-        for (int ww = 0; ww < kFuseWaves; ++ww) t += (double)d.ee[ww][gl];
-        partials[((int64_t)(g0 + gl) * blocks + blockIdx.x) * ncol + e] = t;
-        if (!sel) return;
-        const int64_t rid = (int64_t)e * G + g0 + gl;
-        const uint32_t lo = d.lo[gl], hi = d.hi[gl], nlo = d.nlo[gl], nin = d.nin[gl];
-        const uint32_t lo1 = lo ? lo : 1u;
-        const bool collect = hi >= lo1 && lo != hi;
-        if (nlo) atomicAdd(lo ? &sel[rid].below : &sel[rid].zin, nlo);
-        if (nin) atomicAdd(&sel[rid].nin, nin);
-        if (collect && nin > (uint32_t)slots) {
-            atomicOr(&sel[rid].claim, kSelOverflow);
-        } else if (collect && nin) {
-            const uint32_t base = atomicAdd(&sel[rid].claim, nin);
-            for (uint32_t i = 0; i < nin; ++i) {
-                const int64_t slot = (int64_t)base + i;
-                if (slot < cap) cand[rid * cap + slot] = d.cand[gl][i];
+            for (int ww = 0; ww < kFuseWaves; ++ww) t += (double)d.ee[ww][gl];
+            partials[((int64_t)(g0 + gl) * blocks + blockIdx.x) * ncol + e] = t;
+            if (!sel) continue;
+            const int64_t rid = (int64_t)e * G + g0 + gl;
+            const uint32_t lo = d.lo[gl], hi = d.hi[gl], nlo = d.nlo[gl], nin = d.nin[gl];
+            const uint32_t lo1 = lo ? lo : 1u;
+            const bool collect = hi >= lo1 && lo != hi;
+            if (nlo) atomicAdd(lo ? &sel[rid].below : &sel[rid].zin, nlo);
+            if (nin) atomicAdd(&sel[rid].nin, nin);
+            if (collect && nin > (uint32_t)slots) {
+                atomicOr(&sel[rid].claim, kSelOverflow);
+            } else if (collect && nin) {
+                const uint32_t base = atomicAdd(&sel[rid].claim, nin);
+                for (uint32_t i = 0; i < nin; ++i) {
+                    const int64_t slot = (int64_t)base + i;
+                    if (slot < cap) cand[rid * cap + slot] = d.cand[gl][i];
+                }
             }
         }
     };
 
     for (int e = 0; e < nd; ++e) {
         FuseDate& d = sd[e & 1];
-        stage(e, d);
+        stage_date(e, d);
         if (e > 0) flush(e - 1, sd[(e - 1) & 1]);
-        __syncthreads();
-        // discount factors of the lane's rows: lane col loads path 16 t + col's, the lane
-        // of rows 4 grp + r takes it from lane 4 grp + r
-        float d4[kFusePT][4];
-#pragma unroll
-        for (int t = 0; t < kFusePT; ++t) {
-            const int64_t p = pbase + 16 * t + col;
+        // discount factors (lane group n loads N-tile n's and shares it) and bond prices
+        float disc[kFuseNT];
+        {
+            const int64_t p = pbase + 16 * (grp < kFuseNT ? grp : 0) + col;
             const float df = (float)exp(-logb[(int64_t)e * P + (p < P ? p : 0)]);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) d4[t][r] = __shfl(df, 4 * grp + r, kWave);
+            for (int n = 0; n < kFuseNT; ++n) disc[n] = __shfl(df, col + 16 * n, kWave);
         }
-        Split8 bs[kFusePT][2];
+        Split8 bs[kFuseNT][2];
 #pragma unroll
-        for (int t = 0; t < kFusePT; ++t) {
-            const int64_t p = pbase + 16 * t + col;
-            value_bonds(v, e, p < P ? p : 0, curves, grp, col, bs[t]);
-        }
-#pragma unroll
-        for (int n = 0; n < kFuseTiles; ++n) {
-            if (n >= nt) break;
-            const int gl = 16 * n + col;   // the lane's set g0 + gl (weight 0 past the last)
-            const float a = d.a[gl];
-            f4_t acc[kFusePT];
-#pragma unroll
-            for (int t = 0; t < kFusePT; ++t) acc[t] = f4_t{a, a, a, a};
-            const int tk = d.tk[n];
-            // opaque per tile: the tiles' fragment reads stay in their tile instead of being
-            // hoisted together (12 registers per chunk and tile)
-            uint32_t off = (uint32_t)lane;
-            asm volatile("" : "+v"(off));
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                if (tk > 32 * c) {
-                    const u4_t* fr = sfrag + ((n * 2 + c) * kMfmaParts) * kWave + off;
-                    const u4_t gh = fr[0], gm = fr[kWave], glo = fr[2 * kWave];
-#pragma unroll
-                    for (int t = 0; t < kFusePT; ++t) {
-                        f4_t x = acc[t];
-                        x = mfma_bf16(bs[t][c].h, gh, x);
-                        x = mfma_bf16(bs[t][c].m, gh, x);
-                        x = mfma_bf16(bs[t][c].h, gm, x);
-                        x = mfma_bf16(bs[t][c].l, gh, x);
-                        x = mfma_bf16(bs[t][c].h, glo, x);
-                        x = mfma_bf16(bs[t][c].m, gm, x);
-                        acc[t] = x;
-                    }
-                }
-            }
-            const float wg = d.w[gl];
-            const uint32_t lo = d.lo[gl], hi = d.hi[gl];
-            const uint32_t lo1 = lo ? lo : 1u;     // inside: non-zero and >= lo
-            const bool any_in = sel && gl < ng && hi >= lo1;
-            const bool collect = any_in && lo != hi;
-            float es = 0.f;
-            uint32_t nlo = 0;
-#pragma unroll
-            for (int t = 0; t < kFusePT; ++t)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const float E = live[t][r] ? exposure_of(acc[t][r], d4[t][r]) : 0.f;
-                    cvap[n][t][r] = __builtin_fmaf(wg, E, cvap[n][t][r]);
-                    es += E;
-                    const uint32_t b = __builtin_bit_cast(uint32_t, E);
-                    nlo += (live[t][r] && b < lo1) ? 1u : 0u;
-                    if (any_in && live[t][r] && b - lo1 <= hi - lo1) {
-                        const uint32_t pos = atomicAdd(&d.nin[gl], 1u);
-                        if (collect && pos < (uint32_t)slots) d.cand[gl][pos] = E;
-                    }
-                }
-            es = cross_row_sum(es);
-            if (grp == 0 && gl < ng) d.ee[wave][gl] = es;
-            if (sel) {
-                nlo = cross_row_sum(nlo);
-                if (grp == 0 && nlo) atomicAdd(&d.nlo[gl], nlo);
-            }
-            // each set tile is its own scheduling region: interleaving the tiles would keep
-            // their operands and accumulators live together
+        for (int n = 0; n < kFuseNT; ++n) {
+            const int64_t p = pbase + 16 * n + col;
+            value_bonds(v, e, p < P ? p : 0, curves, grp, col, bs[n]);
+            // one N-tile's curve rows and partial sums live at a time
             __builtin_amdgcn_sched_barrier(0);
         }
+        // the group's set tiles in pieces of kFuseStage; each piece a compile-time index, so
+        // the per-path CVA registers are addressed by constants
+        auto piece = [&](auto pc) {
+            constexpr int s0 = decltype(pc)::value * kFuseStage;
+            if (s0 >= nt) return;
+            const int st = nt - s0 < kFuseStage ? nt - s0 : kFuseStage;
+            // the previous piece's fragments are consumed (and, at s0 = 0, date e's arrays
+            // are staged and date e - 1's were flushed from the other buffer)
+            __syncthreads();
+            const u4_t* src = gfrag + ((int64_t)e * mtiles + m0 + s0) * kValueFragsPerTile * kWave;
+            for (int t = threadIdx.x; t < st * kValueFragsPerTile * kWave; t += kFuseBlock) sfrag[t] = src[t];
+            __syncthreads();
+#pragma unroll
+            for (int mm = 0; mm < kFuseStage; ++mm) {
+                const int n = s0 + mm;   // set tile within the group
+                if (mm >= st) break;
+                const f4_t a4 = *reinterpret_cast<const f4_t*>(
+                    apad + ((int64_t)e * mtiles + m0 + n) * 16 + 4 * grp);
+                f4_t acc[kFuseNT];
+                // a matured tile (tk 0) makes no products; its zeros are still counted
+                value_tile<kFuseNT>(sfrag + (mm * kValueFragsPerTile) * kWave, lane, d.tk[n], a4, bs, acc);
+                float es[4];
+                uint32_t below = 0;   // four 8-bit counts (at most 32 each)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int gl = 16 * n + 4 * grp + r;
+                    const float wg = d.w[gl];
+                    const uint32_t lo = d.lo[gl], hi = d.hi[gl];
+                    const uint32_t lo1 = lo ? lo : 1u;   // inside: non-zero and >= lo
+                    const bool any_in = sel && hi >= lo1;
+                    const bool collect = any_in && lo != hi;
+                    es[r] = 0.f;
+#pragma unroll
+                    for (int k = 0; k < kFuseNT; ++k) {
+                        const float E = live[k] ? exposure_of(acc[k][r], disc[k]) : 0.f;
+                        cvap[n][k][r] = __builtin_fmaf(wg, E, cvap[n][k][r]);
+                        es[r] += E;
+                        const uint32_t b = __builtin_bit_cast(uint32_t, E);
+                        below += (live[k] && b < lo1) ? (1u << (8 * r)) : 0u;
+                        if (any_in && live[k] && b - lo1 <= hi - lo1) {
+                            const uint32_t pos = atomicAdd(&d.nin[gl], 1u);
+                            if (collect && pos < (uint32_t)slots) d.cand[gl][pos] = E;
+                        }
+                    }
+                }
+                const float et = row_sum4(es, col);
+                const int gr = 16 * n + 4 * grp + col / 4;   // the set lane col's sums are of
+                if ((col & 3) == 0) d.ee[wave][gr] = et;
+                if (sel) {
+                    uint32_t cb = below;
+                    cb += __shfl_xor(cb, 8, 16);
+                    cb += __shfl_xor(cb, 4, 16);
+                    cb += __shfl_xor(cb, 2, 16);
+                    cb += __shfl_xor(cb, 1, 16);
+                    const uint32_t nlo = (cb >> (8 * (col / 4))) & 0xffu;
+                    if ((col & 3) == 0 && nlo) atomicAdd(&d.nlo[gr], nlo);
+                }
+                // each set tile is its own scheduling region: interleaving the tiles would
+                // keep their operands and accumulators live together
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        };
+        for_each_index(piece, std::make_integer_sequence<int, kFuseTiles / kFuseStage>{});
+        // date e's arrays are complete before the next iteration flushes them
         __syncthreads();
     }
     flush(nd - 1, sd[(nd - 1) & 1]);
-    // per-path CVA moments per set: in-lane and cross-row sums, then the waves in order
+    // per-path CVA moments per set: in-lane over the N-tiles, then the row, then the
+    // waves in order
+    __syncthreads();
     double* s_cva = reinterpret_cast<double*>(sfrag);   // [wave][set][2]
 #pragma unroll
     for (int n = 0; n < kFuseTiles; ++n) {
         if (n >= nt) break;
-        double c1 = 0.0, c2 = 0.0;
+        double c1[4], c2[4];
 #pragma unroll
-        for (int t = 0; t < kFusePT; ++t)
+        for (int r = 0; r < 4; ++r) {
+            c1[r] = 0.0;
+            c2[r] = 0.0;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const double c = (double)cvap[n][t][r];
-                c1 += c;
-                c2 += c * c;
+            for (int k = 0; k < kFuseNT; ++k) {
+                const double c = (double)cvap[n][k][r];
+                c1[r] += c;
+                c2[r] += c * c;
             }
-        c1 = cross_row_sum(c1);
-        c2 = cross_row_sum(c2);
-        const int gl = 16 * n + col;
-        if (grp == 0) {
-            s_cva[(wave * kFuseSets + gl) * 2] = c1;
-            s_cva[(wave * kFuseSets + gl) * 2 + 1] = c2;
+        }
+        const double t1 = row_sum4(c1, col), t2 = row_sum4(c2, col);
+        const int gr = 16 * n + 4 * grp + col / 4;
+        if ((col & 3) == 0) {
+            s_cva[(wave * kFuseSets + gr) * 2] = t1;
+            s_cva[(wave * kFuseSets + gr) * 2 + 1] = t2;
         }
         __builtin_amdgcn_sched_barrier(0);   // one tile's FP64 values live at a time
     }

@@ -105,16 +105,16 @@ def test_portfolio_fp32_pfe_exact_on_long_rows():
 def test_fused_statistics_equal_the_exposure_pass():
     """FP32 plans can reduce the exposures inside the valuation (EE, per-path CVA moments,
     the PFE bracket counting; tuning.fuse_statistics) unless the per-path exposures are
-    asked for. On one plan, 72 sets
-    (two set groups of the fused pass) x 70000 paths x 30y: the fused run's PFE equals the
-    exposure pass's (the exact order statistic of the same exposures), EE and CVA to 1e-6,
-    the standard error to FP32 accumulation of the per-path CVA; and with
-    one candidate slot per set and date the rows overflow, are valued again whole, and the
-    PFE is still exact. At the 97.5 % and 50 % quantiles. (EE and CVA: FP32 partial sums of
-    32 paths in the fused pass, FP64 per path in the exposure pass.)"""
+    asked for. On one plan, 160 sets (two set groups of the fused pass, 128 + 32) x 40000
+    paths (a ragged last workgroup) x 30y: the fused run's PFE equals the exposure pass's
+    (the exact order statistic of the same exposures), EE and CVA to 1e-6, the standard
+    error to FP32 accumulation of the per-path CVA; and with one candidate slot per set
+    and date the rows overflow, are valued again whole, and the PFE is still exact. At the
+    97.5 % and 50 % quantiles. (EE and CVA: FP32 partial sums of 32 paths in the fused
+    pass, FP64 per path in the exposure pass.)"""
     m = hjm.benchmark_market()
-    pf = hjm.synthetic_portfolio(144, 72, seed=5)
-    P = 70000
+    pf = hjm.synthetic_portfolio(320, 160, seed=5)
+    P = 40000
     for q in (0.975, 0.5):
         _flush("[gpu] fused statistics vs exposure pass, q = %g" % q)
         plan = hjm.PortfolioPlan(m, pf, P, horizon=30.0, seed=9, pfe_quantile=q,
