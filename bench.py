@@ -295,6 +295,9 @@ def main():
                     help="FP32 path kernel timed as the headline: matrix cores or VALU")
     ap.add_argument("--headline-only", action="store_true",
                     help="time only the headline kernel sequence (for counter runs)")
+    ap.add_argument("--headline-precision", default="fp32", choices=["fp32", "fp64"],
+                    help="with --headline-only: the precision of that sequence (fp64 "
+                         "profiles the FP64 path kernels, --path-kernel choosing which)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (calibrates HBM counters)")
     args = ap.parse_args()
@@ -355,9 +358,12 @@ def main():
     other = "vector" if args.path_kernel == "matrix" else "matrix"
     kernels = {"matrix": hjm.KERNEL_MATRIX, "vector": hjm.KERNEL_VECTOR}
     if args.headline_only:
-        f32 = run_plan(hjm, market, swap, hjm.FP32_KAHAN, args.steps, args.warmup,
-                       kernels[args.path_kernel])
-        print(json.dumps({"headline_only": True, "kernel": KERNEL_NAMES[args.path_kernel],
+        fp64 = args.headline_precision == "fp64"
+        f32 = run_plan(hjm, market, swap, hjm.FP64 if fp64 else hjm.FP32_KAHAN, args.steps,
+                       args.warmup, kernels[args.path_kernel])
+        name = ({"matrix": "ref_paths_mfma64_kernel", "vector": "ref_paths_kernel<double>"}
+                if fp64 else KERNEL_NAMES)[args.path_kernel]
+        print(json.dumps({"headline_only": True, "kernel": name, "precision": args.headline_precision,
                           "kernel_ms": float(np.mean(f32["kernel_ms"][:, 0])),
                           "path_steps_per_s": PATHS * S * args.steps / f32["wall"]}), flush=True)
         return

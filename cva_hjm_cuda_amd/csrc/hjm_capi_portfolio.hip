@@ -29,9 +29,11 @@ struct hjm_pplan_s {
     int nsets = 0, ndates = 0;
     int64_t pfe_rank = 1;
     int grid_blocks = 0;
-    bool use_mfma = false;
+    bool use_mfma = false;    // FP32 on the matrix cores (path kernel and valuation)
+    bool use_mfma64 = false;  // FP64 path kernel on the matrix cores (HJM_KERNEL_MATRIX)
     MfmaGrid mg{};
     uint16_t* d_wfrag = nullptr;  // bf16 parts of the block map
+    double* d_wfrag64 = nullptr;  // the block map's FP64 fragments
     RefStepConsts<double> kd{};
     RefStepConsts<float> kf{};
     PfGrid grid{};
@@ -80,6 +82,7 @@ void pplan_release(hjm_portfolio_plan p)
     dev_free(p->d_kmax);
     dev_free(p->d_w);
     dev_free(p->d_wfrag);
+    dev_free(p->d_wfrag64);
     for (auto& e : p->ev)
         if (e) (void)hipEventDestroy(e);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -188,10 +191,12 @@ int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfol
     if (cfg->path_kernel < HJM_KERNEL_AUTO || cfg->path_kernel > HJM_KERNEL_MATRIX) return HJM_ERR_ARG;
     if (cfg->adjoint_normals != HJM_ADJ_NORMALS_AUTO && cfg->adjoint_normals != HJM_ADJ_NORMALS_REGENERATE)
         return HJM_ERR_ARG;
-    const bool mfma_ok = cfg->precision == HJM_FP32_KAHAN && cfg->rng == HJM_RNG_PHILOX &&
-                         stride == kMfmaStride;
+    const bool mfma_ok = cfg->rng == HJM_RNG_PHILOX && stride == kMfmaStride;
     if (cfg->path_kernel == HJM_KERNEL_MATRIX && !mfma_ok) return HJM_ERR_UNSUPPORTED;
-    p->use_mfma = mfma_ok && cfg->path_kernel != HJM_KERNEL_VECTOR;
+    // AUTO: the matrix cores for FP32; FP64 keeps the vector path kernel unless MATRIX is
+    // asked for (its valuation and statistics stay on the vector ALUs either way)
+    p->use_mfma = mfma_ok && cfg->precision == HJM_FP32_KAHAN && cfg->path_kernel != HJM_KERNEL_VECTOR;
+    p->use_mfma64 = mfma_ok && cfg->precision == HJM_FP64 && cfg->path_kernel == HJM_KERNEL_MATRIX;
 
     p->device = cfg->device;
     p->precision = cfg->precision;
@@ -307,6 +312,17 @@ int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfol
         HJM_TRY(dev_alloc(&p->d_wfrag, wf.size()));
         HJM_HIP(hipMemcpy(p->d_wfrag, wf.data(), wf.size() * sizeof(uint16_t), hipMemcpyHostToDevice),
                 HJM_ERR_DEVICE);
+    }
+    if (p->use_mfma64) {
+        HJM_HIP(mfma64_kernels_prepare(), HJM_ERR_DEVICE);
+        std::vector<double> wf;
+        build_mfma64_block_map(market->drifts, market->volatilities, cfg->dt, pf->dtau,
+                               kMfmaPfMTiles, wf, MfmaMap::kPortfolio);
+        HJM_TRY(dev_alloc(&p->d_wfrag64, wf.size()));
+        HJM_HIP(hipMemcpy(p->d_wfrag64, wf.data(), wf.size() * sizeof(double), hipMemcpyHostToDevice),
+                HJM_ERR_DEVICE);
+    }
+    if (p->use_mfma || p->use_mfma64) {
         MfmaGrid& mg = p->mg;
         mg.ndates = nd;
         mg.paths = cfg->paths;
@@ -314,7 +330,10 @@ int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfol
         mg.seed_lo = g.seed_lo;
         mg.seed_hi = g.seed_hi;
         mg.dt = cfg->dt;
-        for (int j = 0; j < kTenors; ++j) mg.spot[j] = (float)market->spot_rates[j];
+        for (int j = 0; j < kTenors; ++j) {
+            mg.spot[j] = (float)market->spot_rates[j];
+            mg.spot64[j] = market->spot_rates[j];
+        }
     }
     return HJM_OK;
 }
@@ -343,13 +362,18 @@ int pplan_enqueue(hjm_portfolio_plan p, bool events = true, float* zs = nullptr,
     const int rows = p->nsets * p->ndates;
     auto mark = [&](int i) { return events ? hipEventRecord(p->ev[i], st) : hipSuccess; };
     HJM_HIP(mark(0), HJM_ERR_DEVICE);
-    hipError_t err =
-        p->use_mfma ? launch_pf_paths_mfma(p->mg, p->d_wfrag, (float*)p->d_curves, p->d_logb, zs,
-                                           p->path_shape, st)
-        : f64 ? launch_pf_paths_f64(p->kd, p->grid, p->d_normals, (double*)p->d_curves, p->d_logb,
-                                    p->tuning.block_size, p->grid_blocks, st)
-              : launch_pf_paths_f32(p->kf, p->grid, p->d_normals, (float*)p->d_curves, p->d_logb,
-                                    p->tuning.block_size, p->grid_blocks, st);
+    hipError_t err;
+    if (p->use_mfma)
+        err = launch_pf_paths_mfma(p->mg, p->d_wfrag, (float*)p->d_curves, p->d_logb, zs,
+                                   p->path_shape, st);
+    else if (p->use_mfma64)
+        err = launch_pf_paths_mfma64(p->mg, p->d_wfrag64, (double*)p->d_curves, p->d_logb, zs, st);
+    else if (f64)
+        err = launch_pf_paths_f64(p->kd, p->grid, p->d_normals, (double*)p->d_curves, p->d_logb,
+                                  p->tuning.block_size, p->grid_blocks, st);
+    else
+        err = launch_pf_paths_f32(p->kf, p->grid, p->d_normals, (float*)p->d_curves, p->d_logb,
+                                  p->tuning.block_size, p->grid_blocks, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
     HJM_HIP(mark(1), HJM_ERR_DEVICE);
     if (pplan_fused(p, keep_exposures)) {
@@ -627,7 +651,7 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
     HJM_HIP(hipEventCreate(&c.e1), HJM_ERR_DEVICE);
     // the matrix-core forward keeps its normals for the reverse sweep; without the memory
     // the sweep regenerates them
-    if (p->use_mfma && config->adjoint_normals == HJM_ADJ_NORMALS_AUTO &&
+    if ((p->use_mfma || p->use_mfma64) && config->adjoint_normals == HJM_ADJ_NORMALS_AUTO &&
         dev_alloc(&c.zs, (size_t)p->grid.nsteps * 3 * (size_t)P) != HJM_OK)
         c.zs = nullptr;
     hipStream_t st = p->stream;

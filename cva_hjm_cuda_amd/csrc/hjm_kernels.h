@@ -43,6 +43,10 @@ hipError_t launch_ref_paths_mfma(const MfmaGrid& g, const uint16_t* wfrag, doubl
 hipError_t mfma64_kernels_prepare();
 hipError_t launch_ref_paths_mfma64(const MfmaGrid& g, const double* wfrag, double* nx, double* ny,
                                    float* zs, hipStream_t st);
+// The FP64 portfolio path kernel: curves [ndates][51][paths] and logb [ndates][paths];
+// wfrag: build_mfma64_block_map's portfolio fragments.
+hipError_t launch_pf_paths_mfma64(const MfmaGrid& g, const double* wfrag, double* curves, double* logb,
+                                  float* zs, hipStream_t st);
 // Launch shape of a matrix-core portfolio kernel (0 in a field: the kernel's default).
 //   tiles: N-tiles of 16 paths per wave; waves: per workgroup;
 //   stage (valuation): M-tiles of -gamma staged in LDS at a time;

@@ -12,6 +12,9 @@
 // beside the products. A wave owns 16 paths; a workgroup of 12 or 16 waves stages W's first
 // kMfma64LdsTiles M-tiles in LDS (133 KB) and reads the rest, present only for runs of 29
 // or more dates, from global memory (26 KB per tile, L2-resident).
+//   ref_paths_mfma64_kernel: reference semantics (up to 7 M-tiles, numeraire slots).
+//   pf_paths_mfma64_kernel: portfolio semantics (4 M-tiles, curve checkpoints and the log
+//     bank account at every date).
 #include "hjm_common.h"
 #include "hjm_kernels.h"
 #include "hjm_mfma_plan.h"
@@ -185,6 +188,60 @@ __global__ __launch_bounds__(m64_waves(MT, kStore) * kWave) void ref_paths_mfma6
     }
 }
 
+// Portfolio semantics, as pf_paths_mfma_kernel in FP64: dates e = 0 .. ndates-1 every 50
+// steps from t = 0. At each date the curve goes to curves[e][tenor][path] (FP64) and the log
+// bank account dt sum_{n < 50 e} f_n(0) to logb[e][path]; block e's row 51 (the short rate's
+// left-point sum: register 0 of lane group 3 in tile 3) extends it.
+template <bool kStore>
+__global__ __launch_bounds__(m64_waves(kMfmaPfMTiles, kStore) * kWave) void pf_paths_mfma64_kernel(
+    const MfmaGrid g, const double* __restrict__ wfrag, double* __restrict__ curves,
+    double* __restrict__ logb, float* __restrict__ zs)
+{
+    constexpr int MT = kMfmaPfMTiles;
+    constexpr int kThreads = m64_waves(MT, kStore) * kWave;
+    extern __shared__ double lds_w64[];
+    for (int t = threadIdx.x; t < MT * kMfma64K * kWave; t += kThreads) lds_w64[t] = wfrag[t];
+    __syncthreads();
+
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const int grp = lane >> 4, col = lane & 15;
+    const int64_t P = g.paths;
+    const int64_t path = (int64_t)blockIdx.x * (kThreads / 4) + (int64_t)wv * 16 + col;
+    const bool live = path < P;
+    // register r of tile m: row 16 m + 4 r + grp (curve rows below 51; row 51 the sum)
+    double fin[16];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int R = 16 * m + 4 * r + grp;
+            fin[4 * m + r] = R < kTenors ? g.spot64[R < kTenors ? R : 0] : 0.0;
+        }
+    double lnB = 0.0;
+    constexpr int kSumReg = 4 * 3 + 0;
+    const bool sum_lane = grp == kTenors % 4;
+    for (int e = 0;; ++e) {
+        if (live) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int R = 16 * m + 4 * r + grp;
+                    if (R < kTenors) curves[((int64_t)e * kTenors + R) * P + path] = fin[4 * m + r];
+                }
+            if (sum_lane) logb[(int64_t)e * P + path] = lnB;
+        }
+        if (e + 1 >= g.ndates) break;
+        d4_t acc[MT];
+        mfma64_block<MT, kStore>(lds_w64, wfrag, fin, acc, g, path, live, e, grp, lane, zs);
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) fin[4 * m + r] = acc[m][r];
+        lnB += fin[kSumReg] * g.dt;
+    }
+}
+
 static size_t mfma64_lds_bytes(int mt)
 {
     return (size_t)(mt < kMfma64LdsTiles ? mt : kMfma64LdsTiles) * kMfma64TileBytes;
@@ -192,14 +249,16 @@ static size_t mfma64_lds_bytes(int mt)
 
 hipError_t mfma64_kernels_prepare()
 {
-    const void* fns[8] = {
+    const void* fns[10] = {
         (const void*)ref_paths_mfma64_kernel<false, 4>, (const void*)ref_paths_mfma64_kernel<false, 5>,
         (const void*)ref_paths_mfma64_kernel<false, 6>, (const void*)ref_paths_mfma64_kernel<false, 7>,
         (const void*)ref_paths_mfma64_kernel<true, 4>,  (const void*)ref_paths_mfma64_kernel<true, 5>,
-        (const void*)ref_paths_mfma64_kernel<true, 6>,  (const void*)ref_paths_mfma64_kernel<true, 7>};
-    for (int i = 0; i < 8; ++i) {
+        (const void*)ref_paths_mfma64_kernel<true, 6>,  (const void*)ref_paths_mfma64_kernel<true, 7>,
+        (const void*)pf_paths_mfma64_kernel<false>,      (const void*)pf_paths_mfma64_kernel<true>};
+    for (int i = 0; i < 10; ++i) {
+        const int mt = i < 8 ? 4 + i % 4 : kMfmaPfMTiles;
         const hipError_t e = hipFuncSetAttribute(fns[i], hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                 (int)mfma64_lds_bytes(4 + i % 4));
+                                                 (int)mfma64_lds_bytes(mt));
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -224,6 +283,22 @@ hipError_t launch_ref_paths_mfma64(const MfmaGrid& g, const double* wfrag, doubl
     default: HJM_REF64_LAUNCH(7); break;
     }
 #undef HJM_REF64_LAUNCH
+    return hipGetLastError();
+}
+
+// wfrag: build_mfma64_block_map's portfolio fragments (4 M-tiles); curves [ndates][51][paths]
+// and logb [ndates][paths] in FP64.
+hipError_t launch_pf_paths_mfma64(const MfmaGrid& g, const double* wfrag, double* curves, double* logb,
+                                  float* zs, hipStream_t st)
+{
+    if (g.ndates < 1 || g.ndates > kMaxDates || g.paths <= 0) return hipErrorInvalidValue;
+    const int threads = m64_waves(kMfmaPfMTiles, zs != nullptr) * kWave, per_block = threads / 4;
+    const dim3 grid((unsigned)((g.paths + per_block - 1) / per_block));
+    const size_t lds = mfma64_lds_bytes(kMfmaPfMTiles);
+    if (zs)
+        pf_paths_mfma64_kernel<true><<<grid, threads, lds, st>>>(g, wfrag, curves, logb, zs);
+    else
+        pf_paths_mfma64_kernel<false><<<grid, threads, lds, st>>>(g, wfrag, curves, logb, zs);
     return hipGetLastError();
 }
 

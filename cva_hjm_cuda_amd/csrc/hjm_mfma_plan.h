@@ -247,13 +247,15 @@ inline void build_mfma_block_map(const double* drift, const double* vols, double
 //   j = 13..51 (shocks): value v = j - 13 of lane group g, factor v % 3 of step
 //     4 (v / 3) + g -- the FP32 kernel's shocks, in its order; at step 50 factor 0 is the
 //     constant 1 carrying the drift, other entries past step 49 are zero.
-// Fragment [m][j][lane] = W[16 m + lane % 16][column (j, lane / 16)], FP64.
+// Fragment [m][j][lane] = W[16 m + lane % 16][column (j, lane / 16)], FP64; the reference
+// map (ref_paths_mfma64_kernel, up to 7 M-tiles) or the portfolio map (pf_paths_mfma64_kernel,
+// 4 M-tiles; the short rate's sum, row 51, is register 0 of lane group 3 of tile 3).
 constexpr int kMfma64Curve = 13;
 constexpr int kMfma64K = kMfma64Curve + 3 * kMfmaQuads;  // 52
 constexpr int kMfma64TileBytes = kMfma64K * kWave * 8;   // 26624
 constexpr int kMfma64LdsTiles = 5;  // M-tiles staged in LDS (133 KB); later tiles from HBM
 // curve instruction j < 13 of M-tile m is all zero (its rows 16 (j / 4) + 4 (j % 4) + g lie
-// below the tile's first input)
+// below the tile's first input; tiles 0..3 agree for both maps)
 __host__ __device__ constexpr bool mfma64_column_zero(int m, int j)
 {
     return j < kMfma64Curve && 16 * (j / 4) + 4 * (j % 4) + 3 < tile_first_input(m, false);
@@ -273,10 +275,11 @@ inline MfmaColumn mfma64_column(int j, int g)
 }
 
 inline void build_mfma64_block_map(const double* drift, const double* vols, double dt, double dtau,
-                                   int mtiles, std::vector<double>& frag)
+                                   int mtiles, std::vector<double>& frag,
+                                   MfmaMap map = MfmaMap::kReference)
 {
-    const int rows = 2 * kTenors;
-    const std::vector<double> W = mfma_block_map_dense(drift, vols, dt, dtau, MfmaMap::kReference);
+    const int rows = map == MfmaMap::kPortfolio ? kTenors + 1 : 2 * kTenors;
+    const std::vector<double> W = mfma_block_map_dense(drift, vols, dt, dtau, map);
     // the dense map's column of each curve row, shock (step, factor) and the constant
     int curve_k[kTenors] = {}, shock_k[kMfmaStride][3] = {}, const_k = -1;
     for (int k = 0; k < kMfmaK; ++k) {

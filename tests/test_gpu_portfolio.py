@@ -68,6 +68,47 @@ def test_portfolio_fp64_matches_oracle(small, oracle):
     assert np.all(r.exposures[:, :, 0] == r.exposures[0:1, :, 0])
 
 
+def test_portfolio_fp64_matrix_path_kernel(oracle):
+    """The FP64 path kernel on the matrix cores (the portfolio block map in FP64,
+    path_kernel=MATRIX): exposures within 1e-12 of the FP64 oracle's scale on the
+    generator's own normals (200 paths x 30y); EE, PFE and CVA of 20000 paths (a ragged
+    last workgroup) within 1e-12 relative of the vector FP64 kernel; the Greeks from its
+    stored shocks against the vector plan's regenerated ones; AUTO keeps FP64 on the vector
+    kernel."""
+    _flush("[gpu] portfolio FP64 matrix-core path kernel vs oracle, 200 paths x 30y")
+    m = hjm.benchmark_market()
+    pf = hjm.synthetic_portfolio(48, 12, seed=6)
+    z = hjm.generate_normals(21, 200, 3000)
+    ex_o = oracle.portfolio_exposure(m.spot_rates, m.drifts, m.volatilities, 30.0, 0.5, 0.01,
+                                     z, pf)
+    r = hjm.simulate_portfolio(m, pf, 200, horizon=30.0, precision=hjm.FP64, seed=21,
+                               keep_paths=True, path_kernel=hjm.KERNEL_MATRIX)
+    err = _rel(r.exposures, ex_o)
+    _flush("  exposures: max error %.2e of scale" % err)
+    assert err < 1e-12
+    _flush("[gpu] portfolio FP64 matrix-core vs vector path kernel, 20000 paths")
+    a = hjm.simulate_portfolio(m, pf, 20000, horizon=30.0, precision=hjm.FP64, seed=5,
+                               path_kernel=hjm.KERNEL_VECTOR)
+    b = hjm.simulate_portfolio(m, pf, 20000, horizon=30.0, precision=hjm.FP64, seed=5,
+                               path_kernel=hjm.KERNEL_MATRIX)
+    ee, pk = a.expected_exposure > 0, a.pfe > 0
+    d = (np.max(np.abs(b.expected_exposure[ee] / a.expected_exposure[ee] - 1)),
+         np.max(np.abs(b.pfe[pk] / a.pfe[pk] - 1)), np.max(np.abs(b.cva / a.cva - 1)))
+    _flush("  EE %.2e PFE %.2e CVA %.2e" % d)
+    assert max(d) < 1e-12
+    c = hjm.simulate_portfolio(m, pf, 20000, horizon=30.0, precision=hjm.FP64, seed=5)
+    assert np.array_equal(c.expected_exposure, a.expected_exposure)
+    _flush("[gpu] portfolio FP64 Greeks, matrix-core forward with stored shocks")
+    ga = hjm.portfolio_cva_greeks(m, pf, 4096, horizon=30.0, precision=hjm.FP64, seed=8,
+                                  path_kernel=hjm.KERNEL_VECTOR)
+    gb = hjm.portfolio_cva_greeks(m, pf, 4096, horizon=30.0, precision=hjm.FP64, seed=8,
+                                  path_kernel=hjm.KERNEL_MATRIX)
+    assert ga.stored_normals_bytes == 0 and gb.stored_normals_bytes == 4096 * 3000 * 3 * 4
+    assert abs(gb.cva / ga.cva - 1) < 1e-11
+    for x, y in ((gb.d_spot, ga.d_spot), (gb.d_vol, ga.d_vol), (gb.d_drift, ga.d_drift)):
+        assert np.max(np.abs(x - y)) <= 1e-6 * np.max(np.abs(y))
+
+
 def test_portfolio_fp32_pfe_exact_on_long_rows():
     """PFE over 132072 FP32 / 66000 FP64 paths (the sampled-bracket selection, counted in
     the statistics pass): the exact order statistic of the plan's own exposures, for every
