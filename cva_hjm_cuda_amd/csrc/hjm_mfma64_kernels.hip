@@ -1,17 +1,19 @@
-// hjm_mfma64_kernels.hip -- the reference-semantics path kernel in FP64 on the matrix cores
+// hjm_mfma64_kernels.hip -- the FP64 path kernels on the matrix cores
 // (v_mfma_f64_16x16x4f64).
 //
-// Each 50-step block is the affine map [f_out; dA] = W [f_in; phi; 1] of the FP32 kernels,
-// held here in FP64 (build_mfma64_block_map) and applied with one FP64 product per group
-// of 4 K entries: no operand splitting. The 16-row x 16-path accumulator tile (lane l:
-// rows 4 i + l / 16 of path l % 16 in register i) is the next block's B operand: register
-// i of curve tile m feeds instruction 4 m + i, so the curve stays in registers. The shocks
-// are the FP32 kernels' (same streams, same values), converted exactly to FP64. FP64
-// products on the matrix cores run at the FP64 VALU's rate on gfx950 (78.6 TFLOP/s each);
-// what they buy is the VALU: the generator, the conversions and the integral sums issue
-// beside the products. A wave owns 16 paths; a workgroup of 12 or 16 waves stages W's first
-// kMfma64LdsTiles M-tiles in LDS (133 KB) and reads the rest, present only for runs of 29
-// or more dates, from global memory (26 KB per tile, L2-resident).
+// Each 50-step block is the affine map [f_out; integral rows] = W [f_in; phi; 1] of the
+// FP32 kernels, held here in FP64 (build_mfma64_block_map) and applied with one FP64 product
+// per group of 4 K entries: no operand splitting. The 16-row x 16-path accumulator tile
+// (lane l: rows 4 i + l / 16 of path l % 16 in register i) is the next block's B operand:
+// register i of curve tile m feeds instruction 4 m + i, so the curve stays in registers.
+// The shocks are the FP32 kernels' (same streams, same values), converted exactly to FP64.
+// On gfx950 an FP64 product holds the matrix pipe 64 cycles (78.6 TFLOP/s, the FP64 VALU's
+// rate) and does not co-execute with VALU work, so the block's products cost what the
+// step-by-step kernel's FP64 FMAs do; the gain is utilisation: a wave's M-tiles are
+// independent accumulators that keep the pipe busy, where one path per lane waits on a
+// dependent chain of steps. A wave owns 16 paths; a workgroup of 8 to 16 waves stages W's
+// first kMfma64LdsTiles M-tiles in LDS (26 KB each) and reads the rest, present only for
+// reference runs of 29 or more dates, from global memory (L2-resident).
 //   ref_paths_mfma64_kernel: reference semantics (up to 7 M-tiles, numeraire slots).
 //   pf_paths_mfma64_kernel: portfolio semantics (4 M-tiles, curve checkpoints and the log
 //     bank account at every date).

@@ -1049,6 +1049,8 @@ __device__ __forceinline__ T row_sum4(const T (&v)[4], int col)
     return u + __shfl_xor(u, 1, 16);
 }
 
+// 2 waves per SIMD: the per-path CVA holds 64 registers beside the valuation's (at 3 the
+// kernel spills 372 bytes per lane; with one N-tile per wave it runs slower)
 __global__ __launch_bounds__(kFuseBlock) __attribute__((amdgpu_waves_per_eu(2))) void pf_value_stats_mfma_kernel(
     const PfValueParams v, int mtiles, const float* __restrict__ curves,
     const double* __restrict__ logb, const u4_t* __restrict__ gfrag, const float* __restrict__ apad,
