@@ -603,7 +603,8 @@ def autotune_portfolio(market: Market, portfolio, paths: int, horizon: float = 3
     with use_graph set, eager ms of one execute with it). Matrix-core FP32 plans tune the
     path kernel (N-tiles, waves, persistent grid), the valuation (N-tiles, waves, LDS
     stage) and the fused against the exposure pass; VALU path kernels their block size
-    and persistent grid (hjm_portfolio_autotune)."""
+    and persistent grid; FP64 plans on the matrix cores keep the defaults
+    (hjm_portfolio_autotune)."""
     mk = market._c()
     pfc = _pf_c(portfolio, horizon, recovery)
     cfg = _config(paths, dt, precision, RNG_PHILOX, seed, None, 0.0, recovery, pfe_quantile,

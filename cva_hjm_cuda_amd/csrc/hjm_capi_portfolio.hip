@@ -810,7 +810,8 @@ int hjm_portfolio_autotune(const hjm_market* market, const hjm_portfolio* portfo
             if (rc == HJM_OK) passes[fuse] = ms[4] - ms[0];
         }
         bt.fuse_statistics = passes[1] < passes[0] ? 1 : 0;
-    } else {
+    } else if (!p->use_mfma64) {
+        // (the FP64 matrix-core path kernel has one launch shape: nothing to sweep)
         double bv = 1e300;
         for (int bs : {64, 128, 256})
             for (int bpc : {0, 1, 2, 4, 8}) {

@@ -210,7 +210,8 @@ typedef struct {
  * path kernel (FP64, host normals, or path_kernel = VECTOR) uses block_size and
  * blocks_per_cu; the matrix-core FP32 path kernel path_tiles, path_waves and
  * blocks_per_cu (0 or 1: its block map fills 84 KB of LDS, one workgroup per CU); the
- * matrix-core valuation value_tiles, value_waves and value_stage. Results do not depend
+ * matrix-core valuation value_tiles, value_waves and value_stage. The FP64 matrix-core
+ * path kernel (FP64 with path_kernel = MATRIX) has a fixed shape. Results do not depend
  * on the shapes. hjm_portfolio_autotune measures them. */
 typedef struct {
     int32_t block_size;     /* threads per workgroup of the VALU path kernel: 64, 128 or 256 */
@@ -263,7 +264,8 @@ int hjm_portfolio_cva_greeks(const hjm_market* market, const hjm_portfolio* port
  * FP32 plans: the path kernel over its four shapes x {one workgroup per group, persistent
  * one per CU}, the valuation over tiles x waves x LDS stage, then the exposure pass
  * against the fused valuation-statistics pass. VALU path kernels: block sizes {64, 128,
- * 256} x persistent grids {0, 1, 2, 4, 8 workgroups per CU} on a 2-date horizon. */
+ * 256} x persistent grids {0, 1, 2, 4, 8 workgroups per CU} on a 2-date horizon. FP64
+ * plans on the matrix cores (path_kernel = MATRIX) have no shape to choose: defaults. */
 int hjm_portfolio_autotune(const hjm_market* market, const hjm_portfolio* portfolio,
                            const hjm_config* config, hjm_tuning* best, double* best_ms);
 

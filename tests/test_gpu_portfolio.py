@@ -74,7 +74,7 @@ def test_portfolio_fp64_matrix_path_kernel(oracle):
     generator's own normals (200 paths x 30y); EE, PFE and CVA of 20000 paths (a ragged
     last workgroup) within 1e-12 relative of the vector FP64 kernel; the Greeks from its
     stored shocks against the vector plan's regenerated ones; AUTO keeps FP64 on the vector
-    kernel."""
+    kernel; the autotuner leaves the matrix-core plan at its defaults."""
     _flush("[gpu] portfolio FP64 matrix-core path kernel vs oracle, 200 paths x 30y")
     m = hjm.benchmark_market()
     pf = hjm.synthetic_portfolio(48, 12, seed=6)
@@ -107,6 +107,10 @@ def test_portfolio_fp64_matrix_path_kernel(oracle):
     assert abs(gb.cva / ga.cva - 1) < 1e-11
     for x, y in ((gb.d_spot, ga.d_spot), (gb.d_vol, ga.d_vol), (gb.d_drift, ga.d_drift)):
         assert np.max(np.abs(x - y)) <= 1e-6 * np.max(np.abs(y))
+    # nothing to tune on the FP64 matrix-core path kernel: the defaults, with a graph
+    t, ms = hjm.autotune_portfolio(m, pf, 4096, horizon=30.0, precision=hjm.FP64,
+                                   path_kernel=hjm.KERNEL_MATRIX)
+    assert (t.block_size, t.blocks_per_cu, t.use_graph) == (256, 0, 1) and ms > 0
 
 
 def test_portfolio_fp32_pfe_exact_on_long_rows():
