@@ -97,7 +97,9 @@ def run_portfolio(hjm, args, nswaps, nsets, paths):
     of every netting set at the launch shapes the autotuner picks for this problem
     (hjm_portfolio_autotune, run before the timed region), replayed as one hipGraph per
     step (timed on the wall clock); the per-kernel breakdown comes from eager runs of the
-    same tuning, beside the eager total at the default shapes."""
+    same tuning, beside the eager total at the default shapes. After the timed region: the
+    same book in FP64 on the same normals (`fp64`: per-pass times with the vector and the
+    matrix-core path kernel, and the FP32 run's EE / PFE / CVA errors against FP64)."""
     market = hjm.benchmark_market()
     pf = hjm.synthetic_portfolio(nswaps, nsets, seed=2024)
     S = 3000
@@ -117,8 +119,38 @@ def run_portfolio(hjm, args, nswaps, nsets, paths):
     for _ in range(args.steps):
         plan.execute()
     wall = time.perf_counter() - t0
+    r32 = plan.execute()
     plan.close()
     kp = float(np.mean(times[:, 0]))
+    fp64 = None
+    if not args.no_fp64:
+        # the same book in FP64 on the same normals: the vector and the matrix-core path
+        # kernel (eager, median of 3), and the FP32 run's errors against it
+        log("[bench] FP64 legs (vector and matrix-core path kernels)")
+        fp64 = {}
+        r64 = None
+        for name, pk in (("vector", hjm.KERNEL_VECTOR), ("matrix", hjm.KERNEL_MATRIX)):
+            p64 = hjm.PortfolioPlan(market, pf, paths, 30.0, DT, precision=hjm.FP64, seed=2024,
+                                    path_kernel=pk)
+            p64.execute()
+            runs = [p64.execute() for _ in range(3)]
+            p64.close()
+            t = np.median(np.array([r.timings_ms for r in runs]), axis=0)
+            fp64[name] = {"paths": float(t[0]), "valuation": float(t[1]),
+                          "statistics": float(t[2]), "pfe_select": float(t[3]),
+                          "device_total": float(t[4])}
+            if r64 is None:
+                r64 = runs[-1]
+        # (dates whose EE is above 1e-3 of the set's peak; PFE where it is non-zero)
+        peak = r64.expected_exposure.max(axis=1, keepdims=True)
+        big = r64.expected_exposure > 1e-3 * peak
+        pk = big & (r64.pfe > 0)
+        fp64["fp32_vs_fp64_error"] = {
+            "ee_max_rel": float(np.max(np.abs(r32.expected_exposure[big] / r64.expected_exposure[big] - 1))),
+            "ee_max_err_of_set_peak": float(np.max(np.abs(r32.expected_exposure - r64.expected_exposure) / peak)),
+            "pfe_max_rel": float(np.max(np.abs(r32.pfe[pk] / r64.pfe[pk] - 1))),
+            "cva_max_rel": float(np.max(np.abs(r32.cva / r64.cva - 1))),
+            "cva_se_rel_min": float(np.min(r64.cva_stderr / r64.cva))}
     adj = None
     if args.config == "full":
         # CVA + 204 Greeks over the whole book (forward keeping its shocks, one-pass reverse
@@ -160,6 +192,7 @@ def run_portfolio(hjm, args, nswaps, nsets, paths):
                                 "device_total_eager_default_shapes": float(np.mean(default[:, 4])),
                                 "paths_default_shapes": float(np.mean(default[:, 0])),
                                 "valuation_default_shapes": float(np.mean(default[:, 1]))},
+        "fp64": fp64,
         "adjoint": adj,
         "cpu_baseline": None,
     }
@@ -287,6 +320,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="batched / full: skip the FP64 legs of the book")
     ap.add_argument("--config", default="precision",
                     choices=["precision", "batched", "full", "calibration"])
     ap.add_argument("--no-baseline-form", action="store_true",
