@@ -8,7 +8,7 @@ from .api import (FP32_KAHAN, FP64, RNG_HOST, RNG_PHILOX, TENORS, HJMError,  # n
                   calculate_exposure_multi_gpu, cva_greeks, cva_weights, device_count, generate_normals,
                   hjm_drift, library, simulate_reference, PortfolioPlan, PortfolioResult,
                   Tuning, simulate_portfolio, autotune_portfolio, portfolio_cva_greeks)
-from .api import KERNEL_AUTO, KERNEL_MATRIX, KERNEL_VECTOR, block_map  # noqa: F401
+from .api import KERNEL_AUTO, KERNEL_MATRIX, KERNEL_VECTOR, block_map, block_map64  # noqa: F401
 from .api import Calibration, box_muller_parts, calibrate_pca, select_kth  # noqa: F401
 from .api import NOALLOC_MAX_PATHS, simulate_reference_noalloc  # noqa: F401
 from .history import synthetic_history  # noqa: F401

@@ -161,6 +161,7 @@ def _declare(lib):
     lib.hjm_calculate_exposure_multi_gpu_f32.argtypes = [fp, f, f, f, f, fp, fp, fp, fp, f, i, i,
                                                          f]
     lib.hjm_block_map.argtypes = [_dp, _dp, d, d, C.c_int32, fp]
+    lib.hjm_block_map64.argtypes = [_dp, _dp, d, d, C.c_int32, C.c_int32, _dp]
     lib.hjm_select_kth.argtypes = [_dp, C.c_int32, C.c_int64, C.c_int64, C.c_int32, _dp]
     lib.hjm_calibrate_pca.argtypes = [C.POINTER(_Calibration), C.POINTER(_CalibResults)]
     lib.hjm_generate_normals.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_int32, C.c_int32,
@@ -422,6 +423,18 @@ def block_map(market: Market, dt: float = 0.01, dtau: float = 0.5,
     _check(library().hjm_block_map(_ptr(market.drifts), _ptr(market.volatilities), dt, dtau,
                                    int(bool(portfolio)),
                                    out.ctypes.data_as(C.POINTER(C.c_float))), "hjm_block_map")
+    return out
+
+
+def block_map64(market: Market, dt: float = 0.01, dtau: float = 0.5,
+                portfolio: bool = False, mtiles: int = 0) -> np.ndarray:
+    """The FP64 matrix-core kernels' block map as they read it: A operands of
+    v_mfma_f64_16x16x4f64, [M-tile][52 instructions][64 lanes] (hjm_block_map64); mtiles 0
+    means all (7 reference, 4 portfolio)."""
+    mt = mtiles or (4 if portfolio else 7)
+    out = np.zeros((mt, 52, 64))
+    _check(library().hjm_block_map64(_ptr(market.drifts), _ptr(market.volatilities), dt, dtau,
+                                     int(bool(portfolio)), int(mt), _ptr(out)), "hjm_block_map64")
     return out
 
 

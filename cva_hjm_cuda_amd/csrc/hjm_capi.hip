@@ -677,6 +677,21 @@ int hjm_block_map(const double* drifts, const double* vols, double dt, double dt
     return HJM_OK;
 }
 
+int hjm_block_map64(const double* drifts, const double* vols, double dt, double dtau,
+                    int32_t portfolio, int32_t mtiles, double* out)
+{
+    if (!drifts || !vols || !out) return HJM_ERR_NULL;
+    if (portfolio != 0 && portfolio != 1) return HJM_ERR_ARG;
+    if (mtiles < 1 || mtiles > (portfolio ? kMfmaPfMTiles : kMfmaMTiles)) return HJM_ERR_ARG;
+    if (!(dt > 0.0) || !(dtau > 0.0) || fabs(dtau / dt - kMfmaStride) > 1e-6 * kMfmaStride)
+        return HJM_ERR_GRID;
+    std::vector<double> frag;
+    build_mfma64_block_map(drifts, vols, dt, dtau, mtiles, frag,
+                           portfolio ? MfmaMap::kPortfolio : MfmaMap::kReference);
+    memcpy(out, frag.data(), frag.size() * sizeof(double));
+    return HJM_OK;
+}
+
 int hjm_generate_normals(uint64_t seed, int64_t path_offset, int64_t paths, int32_t nsteps,
                          int32_t stride, int32_t device, double* out)
 {

@@ -293,6 +293,15 @@ int hjm_calculate_exposure_multi_gpu_f32(float* expected_exposure, float notiona
  * block, then the short rate's left-point sum over the block's 50 steps). */
 int hjm_block_map(const double* drifts, const double* volatilities, double dt, double dtau,
                   int32_t portfolio, float* out);
+/* The FP64 matrix-core kernels' block map as they read it, for inspection and tests
+ * (host only): the A operands of v_mfma_f64_16x16x4f64, out[m][j][lane] = W[16 m + lane %
+ * 16][column (j, lane / 16)] in FP64 for M-tiles m < mtiles and instructions j < 52 (13
+ * curve instructions taking row 16 (j / 4) + 4 (j % 4) + g of the previous block, 39
+ * shock values v = j - 13: factor v % 3 of step 4 (v / 3) + g, the constant 1 at step 50
+ * factor 0). portfolio = 0: reference map, mtiles 1..7; 1: portfolio map, mtiles 1..4.
+ * out holds mtiles x 52 x 64 doubles. */
+int hjm_block_map64(const double* drifts, const double* volatilities, double dt, double dtau,
+                    int32_t portfolio, int32_t mtiles, double* out);
 
 /* ---- calibration upstream of the path (no code in the reference; its roadmap,
  * doc/TODO:16, names the volatility and drift calibration of the HJM model) ----------

@@ -159,3 +159,72 @@ def test_block_maps_vanish_below_each_rows_first_tenor(market, portfolio):
                 assert W[row, k] == 0.0, (row, R)
             elif R == first:
                 assert W[row, k] != 0.0, (row, R)
+
+
+def _mfma64_block(frag, f_in, phi):
+    """One block as the FP64 matrix-core kernels compute it from their A operands: per
+    M-tile the sum over the 52 instructions of A_j (16 x 4, lane l holding A[l % 16][l // 16])
+    times B_j (4 x 16, lane l holding B[l // 16][l % 16]), lane group g supplying what the
+    kernel's registers hold: curve row 16 (j // 4) + 4 (j % 4) + g for j < 13 (0 past row
+    50), factor v % 3 of step 4 (v // 3) + g for v = j - 13 (1 at step 50 factor 0, 0
+    past it). One path: every column of B is the same."""
+    mt = frag.shape[0]
+    out = np.zeros(16 * mt)
+    for m in range(mt):
+        for j in range(52):
+            A = np.zeros((16, 4))
+            for lane in range(64):
+                A[lane % 16, lane // 16] = frag[m, j, lane]
+            B = np.zeros(4)
+            for g in range(4):
+                if j < 13:
+                    R = 16 * (j // 4) + 4 * (j % 4) + g
+                    B[g] = f_in[R] if R < TENORS else 0.0
+                else:
+                    q, fac = divmod(j - 13, 3)
+                    t = 4 * q + g
+                    B[g] = phi[t, fac] if t < STRIDE else float(t == STRIDE and fac == 0)
+            out[16 * m:16 * m + 16] += A @ B
+    return out
+
+
+@pytest.mark.parametrize("portfolio", [False, True])
+def test_fp64_block_map_operands_equal_fifty_euler_steps(market, portfolio):
+    """The FP64 kernels' A operands (hjm_block_map64), fed the B operands their registers
+    hold, give the block's 50 FP64 Euler steps: the curve, and the integral increments
+    (reference) or the short rate's left-point sum (portfolio), to FP64 rounding."""
+    frag = hjm.block_map64(market, portfolio=portfolio)
+    assert frag.shape == ((4 if portfolio else 7), 52, 64)
+    rng = np.random.default_rng(13)
+    vols = market.volatilities.reshape(3, TENORS)
+    for _ in range(2):
+        f_in = 0.03 + 0.01 * rng.standard_normal(TENORS)
+        phi = rng.standard_normal((STRIDE, 3))
+        left = [0.0]
+        f_out, area = _euler(f_in, market.drifts, vols, phi, 0.01, 0.5, left)
+        y = _mfma64_block(frag, f_in, phi)
+        np.testing.assert_allclose(y[:TENORS], f_out, rtol=1e-12, atol=1e-15)
+        if portfolio:
+            assert abs(y[TENORS] / left[0] - 1) < 1e-12
+            assert not np.any(y[TENORS + 1:])
+        else:
+            np.testing.assert_allclose(y[TENORS:2 * TENORS], area, rtol=1e-12, atol=0)
+            assert not np.any(y[2 * TENORS:])
+    # the structural zeros the kernels skip are zeros of these operands
+    for m in range(frag.shape[0]):
+        for j in range(13):
+            if 16 * (j // 4) + 4 * (j % 4) + 3 < _tile_first_input(m, portfolio):
+                assert not np.any(frag[m, j]), (m, j)
+    with pytest.raises(hjm.HJMError):
+        hjm.block_map64(market, portfolio=portfolio, mtiles=8)
+
+
+def _tile_first_input(m, portfolio):
+    rows = TENORS + 1 if portfolio else 2 * TENORS
+    first = []
+    for row in range(16 * m, min(16 * m + 16, rows)):
+        if row < TENORS:
+            first.append(min(row, 49))
+        else:
+            first.append(0 if portfolio else min(row - TENORS, 49))
+    return min(first) if first else TENORS
