@@ -209,6 +209,20 @@ __global__ __launch_bounds__(kBlock) void pf_value_kernel(const PfValueParams v,
 constexpr int kStatPerThread = 8;
 constexpr int kStatPaths = kBlock * kStatPerThread;
 
+// lane mask of an unsigned compare (LLVM predicates 36: <, 37: <=) made by the compare
+// itself: a ballot of a condition the compiler has folded into a subtraction's borrow goes
+// through a select and a second compare
+template <int kPred>
+__device__ __forceinline__ uint64_t lanes_where(uint32_t a, uint32_t b)
+{
+    return __builtin_amdgcn_uicmp(a, b, kPred);
+}
+template <int kPred>
+__device__ __forceinline__ uint64_t lanes_where(uint64_t a, uint64_t b)
+{
+    return __builtin_amdgcn_uicmpl(a, b, kPred);
+}
+
 template <typename Real, bool kCount>
 __device__ __forceinline__ void pf_stats_rows(const PfValueParams v,
                                               const Real* __restrict__ exposures,
@@ -233,7 +247,10 @@ __device__ __forceinline__ void pf_stats_rows(const PfValueParams v,
     const int64_t P = v.paths;
     const int64_t i0 = (int64_t)blockIdx.x * kStatPaths + threadIdx.x;
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-    const uint64_t below_me = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t npad = 0;   // wave-uniform: the wave's (k, lane) slots past the last path
+#pragma unroll
+    for (int k = 0; k < kStatPerThread; ++k)
+        npad += (uint32_t)__popcll(__ballot(i0 + (int64_t)k * kBlock >= P));
     U* slice = reinterpret_cast<U*>(stage_raw) + (size_t)wave * stage;
     uint32_t used = 0;  // wave-uniform
     const int nd = v.ndates;
@@ -325,16 +342,19 @@ __device__ __forceinline__ void pf_stats_rows(const PfValueParams v,
             const bool any_in = hi >= lo1;
             const U span = any_in ? (U)(hi - lo1) : (U)0;
             const bool collect = any_in && lo != hi;
+            // keys past the last path are 0: below every bracket (lo1 >= 1; npad is taken
+            // off again) and inside none (0 - lo1 wraps above every span of finite keys)
             uint32_t nlo = 0, nin = 0;
             uint64_t masks[kStatPerThread];
 #pragma unroll
             for (int k = 0; k < kStatPerThread; ++k) {
-                const bool valid = i0 + (int64_t)k * kBlock < P;
                 const U b = __builtin_bit_cast(U, xs[k]);
-                nlo += (uint32_t)__popcll(__ballot(valid && b < lo1));
-                masks[k] = __ballot(valid && any_in && (U)(b - lo1) <= span);
+                nlo += (uint32_t)__popcll(lanes_where<36>(b, lo1));
+                const uint64_t mk = lanes_where<37>((U)(b - lo1), span);
+                masks[k] = any_in ? mk : 0ull;
                 nin += (uint32_t)__popcll(masks[k]);
             }
+            nlo -= npad;
             uint32_t staged = 0;
             if (collect && nin) {
                 if (used + nin <= (uint32_t)stage) {
@@ -342,9 +362,13 @@ __device__ __forceinline__ void pf_stats_rows(const PfValueParams v,
 #pragma unroll
                     for (int k = 0; k < kStatPerThread; ++k) {
                         const uint64_t mk = masks[k];
-                        if ((mk >> lane) & 1ull)
-                            slice[off + (uint32_t)__popcll(mk & below_me)] = __builtin_bit_cast(U, xs[k]);
-                        off += (uint32_t)__popcll(mk);
+                        if (mk) {   // wave-uniform: skips the 64-path groups holding none
+                            const U b = __builtin_bit_cast(U, xs[k]);
+                            const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                                (uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u));
+                            if ((U)(b - lo1) <= span) slice[off + below] = b;
+                            off += (uint32_t)__popcll(mk);
+                        }
                     }
                     staged = nin;
                 } else if (lane == 0) {
@@ -398,10 +422,13 @@ __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
     pf_stats_rows<Real, kCount>(v, exposures, w, partials, sel, cand, cap, stage, flush);
 }
 
-// FP32 rows with the PFE counting: held to 80 VGPRs for 6 waves per SIMD (20 bytes per
-// lane spill), whose extra loads in flight outweigh the spills (7.87 -> 7.63 ms at 1048576
-// paths x 61 dates x 128 sets; at 72 VGPRs, 7 waves, the 60 bytes of spills cost 11.9 ms).
-// The FP64 rows keep their 102 VGPRs (4 waves): at 80 they would spill 80 bytes per lane.
+// FP32 rows with the PFE counting: held to 80 VGPRs for 6 waves per SIMD, which it fits
+// without spills (it first spilled 20 bytes per lane there and was still faster: 7.87 ->
+// 7.63 ms at 1048576 paths x 61 dates x 128 sets). More waves have not paid: with FP32
+// per-path CVA sums and a staging slice of 768 entries (two flushes instead of one, LDS
+// for 8 workgroups) it took 6.9-7.0 ms at 68 VGPRs (7 waves) and 6.7 ms at 63 (8 waves,
+// with spills), against 6.5-6.6 ms here.
+// The FP64 rows keep their 96 VGPRs (5 waves).
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void pf_stats_count_f32_kernel(
     const PfValueParams v, const float* __restrict__ exposures, const double* __restrict__ w,
     double* __restrict__ partials, SelRowState* __restrict__ sel, float* __restrict__ cand,
