@@ -154,3 +154,88 @@ def test_host_normals_shape_rejected_before_the_library(shape):
         hjm.PortfolioPlan(m, pf, 16, horizon=5.0, normals=z)
     with pytest.raises(ValueError):
         hjm.portfolio_cva_greeks(m, pf, 16, horizon=5.0, normals=z)
+
+
+_PF_CASES = [
+    ("null_portfolio", 1), ("no_swaps", 2), ("no_sets", 2), ("period_zero", 2),
+    ("period_past_horizon", 2), ("set_out_of_range", 2), ("nan_notional", 2),
+    ("negative_hazard", 2), ("recovery_above_one", 2), ("horizon_not_whole", 3),
+    ("horizon_past_30y", 3), ("dt_not_dividing", 3), ("paths0", 2), ("quantile_above_one", 2),
+    ("bad_precision", 2), ("bad_path_kernel", 2), ("bad_adjoint_normals", 2),
+    ("host_rng_no_normals", 1), ("matrix_with_host_normals", 7), ("bad_block_size", 2),
+    ("bad_path_shape", 2), ("bad_value_shape", 2), ("bad_value_stage", 2),
+    ("bad_stage_slots", 2)]
+
+
+@pytest.mark.parametrize("case,code", _PF_CASES)
+def test_portfolio_argument_errors_before_device(case, code):
+    """hjm_portfolio_plan_create checks the book (periods within the horizon, sets in range,
+    finite notionals and rates, non-negative hazards), the grid (a whole number of periods,
+    at most 60, dt dividing dtau), the configuration and the tuning before any device work,
+    and leaves the plan handle NULL."""
+    lib = hjm.library()
+    m = hjm.benchmark_market()
+    mk = m._c()
+    pf = hjm.synthetic_portfolio(8, 3, seed=1)
+    periods, notional, sets, hazard = (pf.periods.copy(), pf.sign_notional.copy(),
+                                       pf.set_of.copy(), pf.hazard.copy())
+    pf = hjm.Portfolio(np.minimum(periods, 20), notional, pf.fixed_rate, sets, hazard)
+    pfc = api._pf_c(pf, 10.0, 0.4)
+    z = np.zeros((16, 1000, 3))
+    cfg = api._config(16, 0.01, hjm.FP32_KAHAN, api.RNG_PHILOX, 1, None, 0.0, 0.4, 0.975, 0, 0)
+    tun = hjm.Tuning()
+    pfp = C.byref(pfc)
+    if case == "null_portfolio":
+        pfp = None
+    elif case == "no_swaps":
+        pfc.nswaps = 0
+    elif case == "no_sets":
+        pfc.nsets = 0
+    elif case == "period_zero":
+        pf.periods[0] = 0
+    elif case == "period_past_horizon":
+        pf.periods[0] = 21          # 10y of 0.5y periods is 20
+    elif case == "set_out_of_range":
+        pf.set_of[0] = pf.nsets
+    elif case == "nan_notional":
+        pf.sign_notional[0] = np.nan
+    elif case == "negative_hazard":
+        pf.hazard[0] = -0.01
+    elif case == "recovery_above_one":
+        pfc.recovery = 1.5
+    elif case == "horizon_not_whole":
+        pfc.horizon = 10.25
+    elif case == "horizon_past_30y":
+        pfc.horizon = 30.5          # 61 periods
+    elif case == "dt_not_dividing":
+        cfg.dt = 0.03
+    elif case == "paths0":
+        cfg.paths = 0
+    elif case == "quantile_above_one":
+        cfg.pfe_quantile = 1.5
+    elif case == "bad_precision":
+        cfg.precision = 9
+    elif case == "bad_path_kernel":
+        cfg.path_kernel = 9
+    elif case == "bad_adjoint_normals":
+        cfg.adjoint_normals = 5
+    elif case == "host_rng_no_normals":
+        cfg.rng = api.RNG_HOST
+    elif case == "matrix_with_host_normals":
+        cfg = api._config(16, 0.01, hjm.FP32_KAHAN, api.RNG_HOST, 1, z, 0.0, 0.4, 0.975, 0, 0,
+                          path_kernel=hjm.KERNEL_MATRIX)
+    elif case == "bad_block_size":
+        tun.block_size = 100
+    elif case == "bad_path_shape":
+        tun.path_tiles = 3
+    elif case == "bad_value_shape":
+        tun.value_waves = 5
+    elif case == "bad_value_stage":
+        tun.value_stage = 9
+    elif case == "bad_stage_slots":
+        tun.pfe_stage_slots = 33
+    tc = tun._c()
+    h = C.c_void_p(1)
+    assert lib.hjm_portfolio_plan_create(C.byref(h), C.byref(mk), pfp, C.byref(cfg),
+                                         C.byref(tc)) == code
+    assert not h.value
