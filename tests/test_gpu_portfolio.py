@@ -5,7 +5,8 @@ netting-set exposures, EE, CVA and SE within 1e-10 relative of the oracle; PFE t
 order statistic; in-kernel Philox bit-identical to the same normals supplied from the host; hipGraph replays bit-identical to eager runs (several replays with eager
 work in between); tuned persistent grids and every launch shape of the matrix-core
 kernels bit-identical to the defaults, the autotuner's choice replayed bit for bit; FP32 versus
-FP64 at the batched configuration's size; shard additivity.
+FP64 at the batched configuration's size and for the largest book the ABI takes (4096 swaps
+in 512 sets), forward and Greeks; shard additivity.
 """
 import dataclasses
 import math
@@ -301,6 +302,40 @@ def test_batched_config_fp32_vs_fp64_and_shards():
                                 path_offset=131072)
     ee = 0.5 * (h1.expected_exposure + h2.expected_exposure)
     assert _rel(ee, a.expected_exposure) < 1e-12
+
+
+def test_largest_book_fp32_against_fp64_forward_and_greeks():
+    """The largest book the ABI takes, 4096 swaps in 512 netting sets (32 set tiles: the
+    valuation stages them 8 at a time, the matrix-core adjoint seeds take them in four
+    groups), and 600 swaps in 200 sets (13 tiles: a partial second seed group); 16384
+    paths x 30y. FP32 against FP64 on the same in-kernel normals: total CVA within 1e-5, every
+    set's CVA within 1e-4, EE within 2e-6 of the set's gross notional (FP32 rounds a set's
+    value at the scale of its swaps' notionals, which netting can leave far above its
+    exposure: measured 2.6e-7 of the gross, up to 1.1e-5 of a set's peak EE; CVA 6.9e-6);
+    CVA + 204 Greeks of the book: the CVA equal to the forward's, the Greeks within 1e-4 x
+    scale (measured 7.8e-6)."""
+    m = hjm.benchmark_market()
+    P = 16384
+    for nsw, nsets in ((4096, 512), (600, 200)):
+        _flush("[gpu] %d swaps in %d sets, %d paths x 30y: FP32 vs FP64, forward and Greeks"
+               % (nsw, nsets, P))
+        pf = hjm.synthetic_portfolio(nsw, nsets, seed=77)
+        a = hjm.simulate_portfolio(m, pf, P, horizon=30.0, precision=hjm.FP64, seed=5)
+        b = hjm.simulate_portfolio(m, pf, P, horizon=30.0, precision=hjm.FP32_KAHAN, seed=5)
+        assert np.all(np.isfinite(b.cva)) and np.all(np.isfinite(b.expected_exposure))
+        assert np.all(a.cva >= 0) and a.cva.max() > 0
+        assert abs(b.cva.sum() / a.cva.sum() - 1) < 1e-5
+        assert np.all(np.abs(b.cva - a.cva) <= 1e-4 * a.cva)
+        gross = np.bincount(pf.set_of, weights=np.abs(pf.sign_notional), minlength=nsets)
+        assert np.all(np.abs(b.expected_exposure - a.expected_exposure) <= 2e-6 * gross[:, None])
+        g32 = hjm.portfolio_cva_greeks(m, pf, P, horizon=30.0, seed=5)
+        g64 = hjm.portfolio_cva_greeks(m, pf, P, horizon=30.0, precision=hjm.FP64, seed=5)
+        assert g32.cva == pytest.approx(b.cva.sum(), rel=1e-12)
+        assert g64.cva == pytest.approx(a.cva.sum(), rel=1e-12)
+        for x, y in ((g32.d_spot, g64.d_spot), (g32.d_vol, g64.d_vol),
+                     (g32.d_drift, g64.d_drift), (g32.d_vol_total, g64.d_vol_total)):
+            assert np.all(np.isfinite(x))
+            assert np.max(np.abs(x - y)) <= 1e-4 * np.max(np.abs(y))
 
 
 def test_portfolio_greeks_match_oracle_adjoint(small, oracle):
