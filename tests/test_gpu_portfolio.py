@@ -313,7 +313,8 @@ def test_largest_book_fp32_against_fp64_forward_and_greeks():
     value at the scale of its swaps' notionals, which netting can leave far above its
     exposure: measured 2.6e-7 of the gross, up to 1.1e-5 of a set's peak EE; CVA 6.9e-6);
     CVA + 204 Greeks of the book: the CVA equal to the forward's, the Greeks within 1e-4 x
-    scale (measured 7.8e-6)."""
+    scale (measured 7.8e-6); at 512 sets the fused valuation-statistics pass (four set groups)
+    equal to the exposure pass: PFE exactly, EE and CVA to 1e-6."""
     m = hjm.benchmark_market()
     P = 16384
     for nsw, nsets in ((4096, 512), (600, 200)):
@@ -328,6 +329,15 @@ def test_largest_book_fp32_against_fp64_forward_and_greeks():
         assert np.all(np.abs(b.cva - a.cva) <= 1e-4 * a.cva)
         gross = np.bincount(pf.set_of, weights=np.abs(pf.sign_notional), minlength=nsets)
         assert np.all(np.abs(b.expected_exposure - a.expected_exposure) <= 2e-6 * gross[:, None])
+        if nsets == 512:
+            # the fused valuation-statistics pass over four groups of 128 sets
+            plan = hjm.PortfolioPlan(m, pf, P, horizon=30.0, seed=5,
+                                     tuning=hjm.Tuning(fuse_statistics=True))
+            c = plan.execute()
+            plan.close()
+            assert np.array_equal(c.pfe, b.pfe)
+            assert np.allclose(c.expected_exposure, b.expected_exposure, rtol=1e-6, atol=0)
+            assert np.allclose(c.cva, b.cva, rtol=1e-6, atol=0)
         g32 = hjm.portfolio_cva_greeks(m, pf, P, horizon=30.0, seed=5)
         g64 = hjm.portfolio_cva_greeks(m, pf, P, horizon=30.0, precision=hjm.FP64, seed=5)
         assert g32.cva == pytest.approx(b.cva.sum(), rel=1e-12)
