@@ -69,6 +69,39 @@ def test_portfolio_fp64_matches_oracle(small, oracle):
     assert np.all(r.exposures[:, :, 0] == r.exposures[0:1, :, 0])
 
 
+def test_portfolio_tiny_path_counts(small, oracle):
+    """1 and 3 paths: FP64 on host normals equals the oracle per path and in EE, CVA and its
+    standard error (0 for one path); PFE is the exact order statistic; the FP32 matrix-core
+    passes give those paths the exposures of the first paths of a 64-path run, bit for
+    bit."""
+    m, pf, z, ex_o = small
+    scale = np.max(np.abs(ex_o))
+    for P in (1, 3):
+        _flush("[gpu] portfolio, %d paths: FP64 host normals vs oracle" % P)
+        r = hjm.simulate_portfolio(m, pf, P, horizon=30.0, precision=hjm.FP64, normals=z[:P],
+                                   keep_paths=True)
+        assert np.max(np.abs(r.exposures - ex_o[:P])) < 1e-10 * scale
+        ee = ex_o[:P].mean(axis=0)
+        assert np.max(np.abs(r.expected_exposure - ee)) < 1e-10 * scale
+        k = math.ceil(0.975 * P)
+        assert np.array_equal(r.pfe, np.sort(r.exposures, axis=0)[k - 1])
+        for g in range(pf.nsets):
+            w = oracle.cva_weights(61, 0.5, pf.hazard[g], 0.4)
+            per = ex_o[:P, g, :] @ w
+            assert abs(r.cva[g] - per.mean()) <= 1e-10 * max(abs(per.mean()), 1e-300)
+            if P == 1:
+                assert r.cva_stderr[g] == 0.0
+            else:
+                se = per.std(ddof=1) / math.sqrt(P)
+                # (sum of squares - square of sum / P): rounding at the scale of the mean
+                assert abs(r.cva_stderr[g] - se) <= 1e-8 * se + 1e-10 * abs(per.mean())
+    _flush("[gpu] portfolio, 1 and 3 paths, FP32 vs the first paths of 64")
+    big = hjm.simulate_portfolio(m, pf, 64, horizon=30.0, seed=9, keep_paths=True)
+    for P in (1, 3):
+        s = hjm.simulate_portfolio(m, pf, P, horizon=30.0, seed=9, keep_paths=True)
+        assert np.array_equal(s.exposures, big.exposures[:P])
+
+
 def test_portfolio_fp64_matrix_path_kernel(oracle):
     """The FP64 path kernel on the matrix cores (the portfolio block map in FP64,
     path_kernel=MATRIX): exposures within 1e-12 of the FP64 oracle's scale on the

@@ -273,6 +273,39 @@ def test_pfe_is_exact_order_statistic(c1):
             assert r.pfe[t] == np.partition(col, k - 1)[k - 1]
 
 
+def test_tiny_path_counts(c1, oracle):
+    """1, 2 and 33 paths (a partly live wave, N-tile and workgroup in every pass): FP64 on
+    host normals equals the oracle per path and in EE, PFE, CVA and its standard error (0
+    for one path, as the oracle defines it); the FP32 matrix-core kernel's numeraires of
+    those paths equal the first paths of a 64-path run bit for bit."""
+    m, swap = c1["m"], c1["swap"]
+    w = oracle.cva_weights(51, 0.5, C1["hazard"], C1["recovery"])
+    scale = np.max(np.abs(c1["ex"]))
+    for P in (1, 2, 33):
+        _flush("[gpu] %d paths, FP64 host normals vs oracle" % P)
+        r = hjm.simulate_reference(m, swap, P, normals=c1["z"][:P], hazard=C1["hazard"],
+                                   recovery=C1["recovery"], keep_paths=True)
+        ex = c1["ex"][:P]
+        assert np.max(np.abs(r.exposures - ex)) < 1e-10 * scale
+        ee, pfe, cva, se = oracle.stats(ex, w, 0.975)
+        assert np.all(r.expected_exposure[ee == 0] == 0)
+        assert np.max(_rel(r.expected_exposure[ee > 0], ee[ee > 0])) < 1e-10
+        assert np.max(np.abs(r.pfe - pfe)) <= 1e-10 * scale
+        assert abs(r.cva - cva) <= 1e-10 * abs(cva)
+        if P == 1:
+            assert r.cva_stderr == 0.0 and se == 0.0
+        else:
+            assert abs(r.cva_stderr - se) <= 1e-8 * se + 1e-10 * abs(cva)
+    _flush("[gpu] 1, 2, 33 paths, FP32 matrix-core kernel vs the first paths of 64")
+    big = hjm.simulate_reference(m, swap, 64, seed=9, precision=hjm.FP32_KAHAN, keep_paths=True,
+                                 path_kernel=hjm.KERNEL_MATRIX)
+    for P in (1, 2, 33):
+        s = hjm.simulate_reference(m, swap, P, seed=9, precision=hjm.FP32_KAHAN, keep_paths=True,
+                                   path_kernel=hjm.KERNEL_MATRIX)
+        assert np.array_equal(s.num_x, big.num_x[:P]) and np.array_equal(s.num_y, big.num_y[:P])
+        assert np.array_equal(s.exposures, big.exposures[:P])
+
+
 def test_gpu_argument_errors(c1):
     with pytest.raises(hjm.HJMError) as ei:
         hjm.simulate_reference(c1["m"], c1["swap"], 64, device=99)
