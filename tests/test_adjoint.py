@@ -73,17 +73,21 @@ def _flush(msg):
 
 @pytest.mark.gpu
 def test_gpu_adjoint_matches_oracle_adjoint(oracle):
-    _flush("[gpu] adjoint vs oracle adjoint, 512 paths 5y FP64 forward, host normals")
+    """512 paths, and 33 and 1 (a partly live workgroup of the seed and sweep kernels)."""
     m = hjm.benchmark_market()
     z = oracle.normals(21, 512, 500)
     w = oracle.cva_weights(T, 0.5, 0.02, 0.4)
-    cva, ds, dv, dm = oracle.reference_cva_greeks(m.spot_rates, m.drifts, m.volatilities,
-                                                  m.accrual, 1e6, 0.025, 5.0, 0.5, 0.01, z, w)
-    g = hjm.cva_greeks(m, hjm.InterestRateSwap(1e6, 0.025, 5.0), 512, normals=z,
-                       precision=hjm.FP64)
-    assert g.cva == pytest.approx(cva, rel=1e-10)
-    for got, ref in ((g.d_spot, ds), (g.d_vol, dv), (g.d_drift, dm)):
-        assert np.max(np.abs(got - ref)) <= 2e-5 * np.max(np.abs(ref))
+    for P in (512, 33, 1):
+        _flush("[gpu] adjoint vs oracle adjoint, %d paths 5y FP64 forward, host normals" % P)
+        zp = np.ascontiguousarray(z[:P])
+        cva, ds, dv, dm = oracle.reference_cva_greeks(m.spot_rates, m.drifts, m.volatilities,
+                                                      m.accrual, 1e6, 0.025, 5.0, 0.5, 0.01, zp,
+                                                      w)
+        g = hjm.cva_greeks(m, hjm.InterestRateSwap(1e6, 0.025, 5.0), P, normals=zp,
+                           precision=hjm.FP64)
+        assert g.cva == pytest.approx(cva, rel=1e-10)
+        for got, ref in ((g.d_spot, ds), (g.d_vol, dv), (g.d_drift, dm)):
+            assert np.max(np.abs(got - ref)) <= 2e-5 * np.max(np.abs(ref))
 
 
 def _cva64(m, swap, paths, seed):

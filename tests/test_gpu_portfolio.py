@@ -382,15 +382,18 @@ def test_largest_book_fp32_against_fp64_forward_and_greeks():
 
 
 def test_portfolio_greeks_match_oracle_adjoint(small, oracle):
-    _flush("[gpu] portfolio adjoint vs oracle adjoint, 128 paths x 30y, FP64 forward")
+    """128 paths, and 3 and 1 (a partly live workgroup of the seed and sweep kernels)."""
     m, pf, z, _ = small
     w = oracle.portfolio_weights(pf, 30.0)
-    cva, ds, dv, dm = oracle.portfolio_cva_greeks(m.spot_rates, m.drifts, m.volatilities, 30.0,
-                                                  0.5, 0.01, z, pf, w)
-    g = hjm.portfolio_cva_greeks(m, pf, 128, horizon=30.0, precision=hjm.FP64, normals=z)
-    assert g.cva == pytest.approx(cva, rel=1e-10)
-    for got, ref in ((g.d_spot, ds), (g.d_vol, dv), (g.d_drift, dm)):
-        assert np.max(np.abs(got - ref)) <= 1e-4 * np.max(np.abs(ref))
+    for P in (128, 3, 1):
+        _flush("[gpu] portfolio adjoint vs oracle adjoint, %d paths x 30y, FP64 forward" % P)
+        zp = np.ascontiguousarray(z[:P])
+        cva, ds, dv, dm = oracle.portfolio_cva_greeks(m.spot_rates, m.drifts, m.volatilities,
+                                                      30.0, 0.5, 0.01, zp, pf, w)
+        g = hjm.portfolio_cva_greeks(m, pf, P, horizon=30.0, precision=hjm.FP64, normals=zp)
+        assert g.cva == pytest.approx(cva, rel=1e-10)
+        for got, ref in ((g.d_spot, ds), (g.d_vol, dv), (g.d_drift, dm)):
+            assert np.max(np.abs(got - ref)) <= 1e-4 * np.max(np.abs(ref))
 
 
 def _forward_only_ms(m, pf, paths, reps=3, **kw):
