@@ -102,6 +102,30 @@ def test_portfolio_tiny_path_counts(small, oracle):
         assert np.array_equal(s.exposures, big.exposures[:P])
 
 
+def test_portfolio_short_horizons(oracle):
+    """Horizons of 0.5y (two dates, one evolved block), 1y and 2.5y, swaps cut to the
+    horizon: FP64 on the generator's normals equals the oracle per path (1e-10 of scale);
+    the FP32 matrix-core passes on the same in-kernel normals within 2e-6 of each set's
+    gross notional per path; the EE rows after the last payment are zero."""
+    m = hjm.benchmark_market()
+    base = hjm.synthetic_portfolio(24, 5, seed=9)
+    for horizon in (0.5, 1.0, 2.5):
+        _flush("[gpu] portfolio horizon %gy: FP64 vs oracle, FP32 vs FP64" % horizon)
+        E = int(round(horizon / 0.5))
+        pf = Portfolio(np.minimum(base.periods, E), base.sign_notional, base.fixed_rate,
+                       base.set_of, base.hazard)
+        z = hjm.generate_normals(13, 40, int(round(horizon / 0.01)))
+        ex_o = oracle.portfolio_exposure(m.spot_rates, m.drifts, m.volatilities, horizon, 0.5,
+                                         0.01, z, pf)
+        a = hjm.simulate_portfolio(m, pf, 40, horizon=horizon, precision=hjm.FP64, normals=z,
+                                   keep_paths=True)
+        assert _rel(a.exposures, ex_o) < 1e-10
+        b = hjm.simulate_portfolio(m, pf, 40, horizon=horizon, seed=13, keep_paths=True)
+        gross = np.bincount(pf.set_of, weights=np.abs(pf.sign_notional), minlength=pf.nsets)
+        assert np.all(np.abs(b.exposures - a.exposures) <= 2e-6 * gross[None, :, None])
+        assert np.all(a.expected_exposure[:, -1] == 0) and np.all(b.expected_exposure[:, -1] == 0)
+
+
 def test_portfolio_fp64_matrix_path_kernel(oracle):
     """The FP64 path kernel on the matrix cores (the portfolio block map in FP64,
     path_kernel=MATRIX): exposures within 1e-12 of the FP64 oracle's scale on the
