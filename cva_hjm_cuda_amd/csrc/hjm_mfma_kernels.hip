@@ -1105,11 +1105,12 @@ __global__ __launch_bounds__(kFuseBlock) __attribute__((amdgpu_waves_per_eu(2)))
             const uint32_t lo1 = lo ? lo : 1u;
             const bool collect = hi >= lo1 && lo != hi;
             if (nlo) atomicAdd(lo ? &sel[rid].below : &sel[rid].zin, nlo);
-            if (nin) atomicAdd(&sel[rid].nin, nin);
-            if (collect && nin > (uint32_t)slots) {
+            const bool over = collect && nin > (uint32_t)slots;
+            const uint32_t take = (collect && !over) ? nin : 0u;
+            const uint32_t base = nin ? sel_count_and_claim(&sel[rid], nin, take) : 0u;
+            if (over) {
                 atomicOr(&sel[rid].claim, kSelOverflow);
-            } else if (collect && nin) {
-                const uint32_t base = atomicAdd(&sel[rid].claim, nin);
+            } else if (take) {
                 for (uint32_t i = 0; i < nin; ++i) {
                     const int64_t slot = (int64_t)base + i;
                     if (slot < cap) cand[rid * cap + slot] = d.cand[gl][i];

@@ -303,20 +303,33 @@ __device__ __forceinline__ void pf_stats_rows(const PfValueParams v,
                 nst += beg[ww][e] != ~0u ? cnt[1][ww][e] : 0u;
             }
             if (nlo) atomicAdd(s_lo[e] ? &sel[rid].below : &sel[rid].zin, nlo);
-            if (nin) atomicAdd(&sel[rid].nin, nin);
-            base[e] = nst ? atomicAdd(&sel[rid].claim, nst) : 0u;
+            const uint32_t b = nin ? sel_count_and_claim(&sel[rid], nin, nst) : 0u;  // nst <= nin
+            base[e] = nst ? b : 0u;
         }
         __syncthreads();
-        for (int e = e0; e < e1; ++e) {
-            const uint32_t b0 = beg[wave][e];
-            if (b0 == ~0u) continue;
-            uint32_t off = base[e];
-            for (int ww = 0; ww < wave; ++ww) off += beg[ww][e] != ~0u ? cnt[1][ww][e] : 0u;
-            const uint32_t n = cnt[1][wave][e];
+        // wave w writes the dates e0 + w, e0 + w + kWaves, ...: all waves' staged entries of
+        // a date in wave order, one contiguous run of the row's candidates per store
+        // instruction (lane i takes entry i - pre[k] of the slice of wave k)
+        const U* slices = reinterpret_cast<const U*>(stage_raw);
+        for (int e = e0 + wave; e < e1; e += kWaves) {
+            uint32_t pre[kWaves + 1], b0[kWaves];
+            pre[0] = 0;
+#pragma unroll
+            for (int ww = 0; ww < kWaves; ++ww) {
+                b0[ww] = beg[ww][e];
+                pre[ww + 1] = pre[ww] + (b0[ww] != ~0u ? cnt[1][ww][e] : 0u);
+            }
+            const uint32_t off = base[e];
             Real* out = cand + ((int64_t)e * v.nsets + gset) * cap;
-            for (uint32_t i = lane; i < n; i += kWave)
-                if ((int64_t)off + i < cap) out[off + i] = __builtin_bit_cast(Real, slice[b0 + i]);
+            for (uint32_t i = lane; i < pre[kWaves]; i += kWave) {
+                int k = 0;
+#pragma unroll
+                for (int ww = 1; ww < kWaves; ++ww) k += i >= pre[ww] ? 1 : 0;
+                if ((int64_t)off + i < cap)
+                    out[off + i] = __builtin_bit_cast(Real, slices[(size_t)k * stage + b0[k] + (i - pre[k])]);
+            }
         }
+        __syncthreads();   // the slices are read across waves: restarted only after this
     };
     int e_flushed = 0;
     load(0, xn);

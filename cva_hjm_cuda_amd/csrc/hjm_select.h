@@ -13,6 +13,7 @@
 //     zeros the answer is 0 after the first pass.
 #pragma once
 #include <math.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "hjm_common.h"
@@ -372,6 +373,20 @@ __global__ __launch_bounds__(kSelBlock) void bracket_sample_kernel(const T* __re
 // entries do not fit flags the row (kSelOverflow in its claim), whose selection then runs
 // over the whole row.
 constexpr uint32_t kSelOverflow = 0x80000000u;
+
+// Adds n to a row's inside count and takes `slots` candidate slots with it: one 64-bit
+// atomic on the adjacent words (nin, claim), nin in the low half (a row's count stays below
+// 2^32, so it never carries into claim). Returns claim before the add, kSelOverflow
+// included if an earlier counter flagged the row.
+__device__ __forceinline__ uint32_t sel_count_and_claim(SelRowState* s, uint32_t n, uint32_t slots)
+{
+    static_assert(offsetof(SelRowState, nin) % 8 == 0 &&
+                      offsetof(SelRowState, claim) == offsetof(SelRowState, nin) + 4,
+                  "nin and claim form one aligned 64-bit word");
+    const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long*>(&s->nin),
+                                             ((unsigned long long)slots << 32) | n);
+    return (uint32_t)(old >> 32);
+}
 constexpr int kCountSlice = kCountChunk / (kCountBlock / kWave);  // entries read per wave
 constexpr int kCountStage = 512;                                  // entries staged per wave
 
@@ -454,9 +469,10 @@ __global__ __launch_bounds__(kCountBlock) void bracket_count_kernel(const T* __r
             c += s_cnt[1][w];
         }
         if (b) atomicAdd(lo ? &state[row].below : &state[row].zin, b);
-        if (c) atomicAdd(&state[row].nin, c);
+        const uint32_t take = (collect && !any_ovf) ? c : 0u;
+        const uint32_t base = c ? sel_count_and_claim(&state[row], c, take) : 0u;
         if (collect && any_ovf) atomicOr(&state[row].claim, kSelOverflow);
-        s_base = (c && collect && !any_ovf) ? atomicAdd(&state[row].claim, c) : 0u;
+        s_base = take ? base : 0u;
     }
     __syncthreads();
     if (collect && !any_ovf) {
