@@ -47,6 +47,7 @@ struct hjm_pplan_s {
     double* d_sums = nullptr;
     double* d_pfe = nullptr;
     char* d_selws = nullptr;  // bracket-selection workspace (PFE)
+    uint32_t* d_loparts = nullptr;  // counts below the bracket per row and statistics workgroup
     void* d_coef = nullptr;
     void* d_a = nullptr;
     int* d_kmax = nullptr;
@@ -74,6 +75,7 @@ void pplan_release(hjm_portfolio_plan p)
     dev_free(p->d_sums);
     dev_free(p->d_pfe);
     dev_free(p->d_selws);
+    dev_free(p->d_loparts);
     dev_free(p->d_coef);
     dev_free(p->d_vfrag);
     dev_free(p->d_apad);
@@ -251,7 +253,11 @@ int pplan_init(hjm_portfolio_plan p, const hjm_market* market, const hjm_portfol
     if (p->do_pfe) {
         const size_t wsb = bracket_workspace_bytes(pf->nsets * nd, bracket_plan(cfg->paths, p->pfe_rank, (int)rsz),
                                                    (int)rsz);
-        if (wsb) HJM_TRY(dev_alloc(&p->d_selws, wsb));
+        if (wsb) {
+            HJM_TRY(dev_alloc(&p->d_selws, wsb));
+            if (rsz == sizeof(float))
+                HJM_TRY(dev_alloc(&p->d_loparts, (size_t)pf->nsets * nd * pf_stats_blocks(cfg->paths)));
+        }
     }
     HJM_TRY(dev_alloc(&p->d_kmax, kmax.size()));
     HJM_TRY(dev_alloc(&p->d_w, p->w.size()));
@@ -432,9 +438,9 @@ int pplan_enqueue(hjm_portfolio_plan p, bool events = true, float* zs = nullptr,
         HJM_HIP(err, HJM_ERR_LAUNCH);
     }
     err = f64 ? launch_pf_stats_f64(p->vp, (const double*)p->d_ex, p->d_w, p->d_partials,
-                                    p->d_sums, selws, p->pfe_rank, st)
+                                    p->d_sums, selws, p->pfe_rank, nullptr, st)
               : launch_pf_stats_f32(p->vp, (const float*)p->d_ex, p->d_w, p->d_partials,
-                                    p->d_sums, selws, p->pfe_rank, st);
+                                    p->d_sums, selws, p->pfe_rank, p->d_loparts, st);
     HJM_HIP(err, HJM_ERR_LAUNCH);
     HJM_HIP(mark(3), HJM_ERR_DEVICE);
     if (p->do_pfe) {

@@ -109,12 +109,17 @@ hipError_t launch_grouped_column_sum(const double* partials, int nsets, int64_t 
 // receives the number of dates between the slice's flushes)
 int pf_stats_stage(int64_t paths, int64_t pfe_rank, int ndates, int elem_bytes,
                    int* flush_dates = nullptr);
+// lo_parts (or null; FP32 with selws): [ndates * nsets][pf_stats_blocks(paths)] counts
+// below the bracket per row and statistics workgroup, stored plainly and added to the rows'
+// states by one small launch after the pass (null, and FP64 rows: one atomic per row and
+// workgroup instead)
+int64_t pf_stats_blocks(int64_t paths);
 hipError_t launch_pf_stats_f64(const PfValueParams& v, const double* ex, const double* w,
                                double* partials, double* sums, void* selws, int64_t pfe_rank,
-                               hipStream_t st);
+                               uint32_t* lo_parts, hipStream_t st);
 hipError_t launch_pf_stats_f32(const PfValueParams& v, const float* ex, const double* w,
                                double* partials, double* sums, void* selws, int64_t pfe_rank,
-                               hipStream_t st);
+                               uint32_t* lo_parts, hipStream_t st);
 hipError_t launch_radix_select_f32(const float* x, int rows, int64_t n, int64_t k, void* ws,
                                    double* out, hipStream_t st);
 // the sampled-bracket selection split around a statistics pass that does its counting

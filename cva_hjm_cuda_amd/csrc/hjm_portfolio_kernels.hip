@@ -230,7 +230,7 @@ __device__ __forceinline__ void pf_stats_rows(const PfValueParams v,
                                               double* __restrict__ partials,
                                               SelRowState* __restrict__ sel,
                                               Real* __restrict__ cand, int64_t cap, int stage,
-                                              int flush)
+                                              int flush, uint32_t* __restrict__ lo_parts)
 {
     typedef typename SelectBits<Real>::U U;
     constexpr int kWaves = kBlock / kWave;
@@ -302,7 +302,12 @@ __device__ __forceinline__ void pf_stats_rows(const PfValueParams v,
                 nin += cnt[1][ww][e];
                 nst += beg[ww][e] != ~0u ? cnt[1][ww][e] : 0u;
             }
-            if (nlo) atomicAdd(s_lo[e] ? &sel[rid].below : &sel[rid].zin, nlo);
+            // FP32 rows: summed per row by sel_add_row_counts_kernel (FP64 rows keep the
+            // atomic: the branch costs them a wave per SIMD)
+            if (sizeof(Real) == sizeof(float) && lo_parts)
+                lo_parts[rid * gridDim.x + blockIdx.x] = nlo;
+            else if (nlo)
+                atomicAdd(s_lo[e] ? &sel[rid].below : &sel[rid].zin, nlo);
             const uint32_t b = nin ? sel_count_and_claim(&sel[rid], nin, nst) : 0u;  // nst <= nin
             base[e] = nst ? b : 0u;
         }
@@ -430,9 +435,10 @@ __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
                                                           double* __restrict__ partials,
                                                           SelRowState* __restrict__ sel,
                                                           Real* __restrict__ cand, int64_t cap,
-                                                          int stage, int flush)
+                                                          int stage, int flush,
+                                                          uint32_t* __restrict__ lo_parts)
 {
-    pf_stats_rows<Real, kCount>(v, exposures, w, partials, sel, cand, cap, stage, flush);
+    pf_stats_rows<Real, kCount>(v, exposures, w, partials, sel, cand, cap, stage, flush, lo_parts);
 }
 
 // FP32 rows with the PFE counting: held to 80 VGPRs for 6 waves per SIMD, which it fits
@@ -445,10 +451,33 @@ __global__ __launch_bounds__(kBlock) void pf_stats_kernel(const PfValueParams v,
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void pf_stats_count_f32_kernel(
     const PfValueParams v, const float* __restrict__ exposures, const double* __restrict__ w,
     double* __restrict__ partials, SelRowState* __restrict__ sel, float* __restrict__ cand,
-    int64_t cap, int stage, int flush)
+    int64_t cap, int stage, int flush, uint32_t* __restrict__ lo_parts)
 {
-    pf_stats_rows<float, true>(v, exposures, w, partials, sel, cand, cap, stage, flush);
+    pf_stats_rows<float, true>(v, exposures, w, partials, sel, cand, cap, stage, flush, lo_parts);
 }
+
+// Adds each row's below-the-bracket (or zero) counts of the statistics workgroups to its
+// selection state, as the counting's atomics did: one wave per row (integer sums, exact in
+// any order; the row's state is written by this wave alone).
+__global__ __launch_bounds__(kBlock) void sel_add_row_counts_kernel(SelRowState* __restrict__ sel,
+                                                                    const uint32_t* __restrict__ parts,
+                                                                    int rows, int nb)
+{
+    const int lane = threadIdx.x % kWave;
+    const int row = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    if (row >= rows) return;   // whole waves
+    uint32_t c = 0;
+    for (int b = lane; b < nb; b += kWave) c += parts[(int64_t)row * nb + b];
+    for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, kWave);
+    if (lane == 0 && c) {
+        if (sel[row].lo)
+            sel[row].below += c;
+        else
+            sel[row].zin += c;
+    }
+}
+
+int64_t pf_stats_blocks(int64_t paths) { return (paths + kStatPaths - 1) / kStatPaths; }
 
 // out[g][c] = sum_b partials[g][b][c] in block order.
 __global__ __launch_bounds__(kBlock) void grouped_column_sum_kernel(const double* __restrict__ partials,
@@ -537,7 +566,7 @@ int pf_stats_stage(int64_t paths, int64_t pfe_rank, int ndates, int elem_bytes, 
 template <typename Real>
 static hipError_t pf_stats_launch(const PfValueParams& v, const Real* ex, const double* w,
                                   double* partials, double* sums, void* selws, int64_t pfe_rank,
-                                  hipStream_t st)
+                                  uint32_t* lo_parts, hipStream_t st)
 {
     const unsigned nb = (unsigned)((v.paths + kStatPaths - 1) / kStatPaths);
     const dim3 grid(nb, (unsigned)v.nsets);
@@ -552,13 +581,18 @@ static hipError_t pf_stats_launch(const PfValueParams& v, const Real* ex, const 
         const size_t lds = (size_t)(kBlock / kWave) * stage * sizeof(Real);
         if constexpr (sizeof(Real) == sizeof(float))
             pf_stats_count_f32_kernel<<<grid, kBlock, lds, st>>>(v, ex, w, partials, state, cand, bp.cap,
-                                                                stage, flush);
+                                                                stage, flush, lo_parts);
         else
             pf_stats_kernel<Real, true><<<grid, kBlock, lds, st>>>(v, ex, w, partials, state, cand,
-                                                                  bp.cap, stage, flush);
+                                                                  bp.cap, stage, flush, lo_parts);
+        if (sizeof(Real) == sizeof(float) && lo_parts) {
+            const int rows = v.ndates * v.nsets;
+            const unsigned wgs = (unsigned)((rows + kBlock / kWave - 1) / (kBlock / kWave));
+            sel_add_row_counts_kernel<<<wgs, kBlock, 0, st>>>(state, lo_parts, rows, (int)nb);
+        }
     } else {
         pf_stats_kernel<Real, false><<<grid, kBlock, 0, st>>>(v, ex, w, partials, nullptr, nullptr, 0,
-                                                             0, 0);
+                                                             0, 0, nullptr);
     }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
@@ -577,16 +611,16 @@ hipError_t launch_grouped_column_sum(const double* partials, int nsets, int64_t 
 
 hipError_t launch_pf_stats_f64(const PfValueParams& v, const double* ex, const double* w,
                                double* partials, double* sums, void* selws, int64_t pfe_rank,
-                               hipStream_t st)
+                               uint32_t* lo_parts, hipStream_t st)
 {
-    return pf_stats_launch<double>(v, ex, w, partials, sums, selws, pfe_rank, st);
+    return pf_stats_launch<double>(v, ex, w, partials, sums, selws, pfe_rank, lo_parts, st);
 }
 
 hipError_t launch_pf_stats_f32(const PfValueParams& v, const float* ex, const double* w,
                                double* partials, double* sums, void* selws, int64_t pfe_rank,
-                               hipStream_t st)
+                               uint32_t* lo_parts, hipStream_t st)
 {
-    return pf_stats_launch<float>(v, ex, w, partials, sums, selws, pfe_rank, st);
+    return pf_stats_launch<float>(v, ex, w, partials, sums, selws, pfe_rank, lo_parts, st);
 }
 
 }  // namespace hjm
