@@ -289,8 +289,9 @@ __device__ __forceinline__ void pf_stats_rows(const PfValueParams v,
             }
         }
     };
-    // claims the candidate slots of dates [e0, e1) and copies the waves' slices out (every
-    // wave calls it at the same dates)
+    // adds the workgroup's counts of dates [e0, e1) to their rows, claims the rows'
+    // candidate slots and writes the waves' staged entries there (every wave calls it at
+    // the same dates)
     auto flush_slices = [&](int e0, int e1) {
         __syncthreads();
         for (int e = e0 + threadIdx.x; e < e1; e += kBlock) {
